@@ -1,0 +1,190 @@
+"""Benchmark: the plain JSON pipeline on one seeded 2 GiB document, one MI355X.
+
+A step is one device-resident scan of the whole document through the C ABI
+(pars_hip_scan_device: R1 classify .. R5 compaction in one kernel, then R6
+pairing; the reference's N1b tier, bench/bench_boundary.cu:252-260), with
+the input already in HBM.  `value` = input bytes / step time / 1e9 (the
+reference's GB/s, bench/bench_boundary.cu:31-33).
+
+roofline: the dominant kernel is the fused R1-R5 scan; its duration comes from
+HIP events recorded by the library on the step's own stream around that
+kernel.  Algorithmic bytes per launch (per document):
+    B_alg = n (input) + n/8 (structural bitmap) + 4 k (uint32 positions)
+            + 4 k_b (bracket-pair array)
+with k, k_b the document's structural and bracket counts.  peak = 8 TB/s.
+
+cpu_baseline: the reference's own CPU backend (model::scanner + pair_brackets,
+compiled from its sources into oracle/_ref/) on a bounded prefix of the same
+document, one core; the C restatement under oracle/ stands in ("port") where
+that binary was not built.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "input GB/s + % HBM roofline (2 GiB JSON); batched/served vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(doc, sample_bytes: int) -> dict:
+    """Reference CPU path on the first `sample_bytes` of the document, one core."""
+    from oracle import oracle as O
+
+    sample = doc[:sample_bytes]
+    cores = 1
+    if O.ref_available():
+        with tempfile.NamedTemporaryFile(suffix=".json", delete=False) as f:
+            path = f.name
+        try:
+            sample.tofile(path)
+            cmd = [O.REF, "time", "json", path, str(sample.size)]
+            r = json.loads(subprocess.run(cmd, check=True, stdout=subprocess.PIPE, text=True).stdout)
+        finally:
+            os.unlink(path)
+        secs = r["scan_s"] + r["pair_s"]
+        kind = "reference"
+    else:
+        from pars_amd import Plan
+
+        plan = Plan.load("json").blob
+        t0 = time.perf_counter()
+        s = O.scan(plan, sample)
+        O.pair(plan, s["bracket_char"])
+        secs = time.perf_counter() - t0
+        kind = "port"
+    return {
+        "value": round(sample.size / secs / 1e9, 4),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"first {sample.size >> 20} MiB of the 2 GiB document, scan + pair_brackets, {secs:.2f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size-mib", type=int, default=2048)
+    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from pars_amd import Plan, Scanner
+    from pars_amd import _native as N
+    from pars_amd import workloads as W
+
+    n = args.size_mib << 20
+    log(f"generating {n} bytes of JSON (seed 2, depth 32, 30% strings)")
+    t0 = time.perf_counter()
+    doc = W.json(2, n, 32, 0.30)
+    str_frac = W.json.last_string_bytes / n
+    log(f"generated in {time.perf_counter() - t0:.1f} s, string fraction {str_frac:.3f}")
+    cpu = None
+    if not args.no_cpu:  # before the GPU is initialised: the baseline runs as a child process
+        log("cpu baseline")
+        cpu = cpu_baseline(doc, min(n, args.cpu_sample_mib << 20))
+        log(f"cpu baseline: {cpu}")
+
+    import torch
+
+    torch.cuda.init()
+    d_doc = torch.from_numpy(doc).to("cuda")
+    torch.cuda.synchronize()
+
+    sc = Scanner(Plan.load("json"))
+    sc.set_option(N.PARS_OPT_WANT_BITMAP, 1)
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    st = N.StageTimes()
+    ptr = d_doc.data_ptr()
+
+    for i in range(args.warmup):
+        r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
+        log(f"warmup {i}: {r}")
+    k, kb = r.n_struct, r.n_brackets
+
+    scan_ms, pair_ms = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
+        scan_ms.append(st.p1)
+        pair_ms.append(st.p4)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    assert (r.n_struct, r.n_brackets) == (k, kb)
+    log(f"steps done: {ms_step:.3f} ms/step; scan kernel {np.mean(scan_ms):.3f} ms, pairing {np.mean(pair_ms):.3f} ms")
+
+    b_alg = n + n / 8 + 4 * k + 4 * kb
+    t_scan = float(np.mean(scan_ms)) / 1e3
+    achieved = b_alg / t_scan / 1e9
+    value = n / (ms_step / 1e3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "json-2GiB plain pipeline (R1-R6, device-resident, N1b)",
+            "bytes": n,
+            "seed": 2,
+            "max_depth": int(r.max_depth),
+            "string_fraction": round(str_frac, 4),
+            "n_struct": int(k),
+            "n_brackets": int(kb),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_scan_general<1,false> (R1-R5)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "b_alg_bytes": int(b_alg),
+            "kernel_ms": round(t_scan * 1e3, 4),
+        },
+        "e2e_roofline_frac": round(b_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": {"r1_r5": round(float(np.mean(scan_ms)), 4), "r6": round(float(np.mean(pair_ms)), 4)},
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    sc.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

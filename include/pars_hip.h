@@ -1,0 +1,162 @@
+/* pars_hip.h — C ABI of the MI355X (gfx950) scan_plan backend.
+ *
+ * Drop-in for the reference CUDA backend's boundary,
+ * include/pars/backend/cuda/pipeline.cuh:1164-1281 (class gpu::scanner<S> and
+ * the free functions gpu::scan_device / scan_host / scan_chunk).  The plan is a
+ * runtime value (pars_plan, byte-compatible with pars::scan_plan) instead of a
+ * template argument; include/pars_hip.hpp restores the template API on top of
+ * this header.  Plain pointers and sizes only.
+ *
+ * Conventions (differences from the reference are deliberate and listed):
+ *   - every function returns a pars_status; pars_hip_last_error() describes
+ *     the most recent failure on the calling thread (the reference calls
+ *     exit(1), pipeline.cuh:520-555);
+ *   - positions are uint32 byte offsets (reference: int32, which overflows
+ *     past 2^31, pipeline.cuh:828,840); the values are identical below 2^31;
+ *   - device outputs alias the scanner's arena and stay valid until the next
+ *     call on the same scanner (pipeline.cuh:1013-1014, same rule);
+ *   - a scanner is not thread-safe; use one per thread (pipeline.cuh:1264);
+ *   - the input is read only within [0, n): no padding is required and the
+ *     input is never written (the reference reads ceil(n/32)*32 bytes and
+ *     scan_chunk zeroes the pad through a const_cast, pipeline.cuh:1042-1043).
+ */
+#ifndef PARS_HIP_H
+#define PARS_HIP_H
+
+#include "pars_plan.h"
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hipStream_t (`struct ihipStream_t*`); NULL = the null stream. */
+typedef struct ihipStream_t* pars_stream;
+
+typedef struct pars_scanner pars_scanner;
+
+typedef enum {
+	PARS_OK			  = 0,
+	PARS_ERR_ARGUMENT = 1, /* bad pointer, size or option */
+	PARS_ERR_PLAN	  = 2, /* the plan violates a plan.hpp cap */
+	PARS_ERR_MEMORY	  = 3, /* device or pinned allocation failed */
+	PARS_ERR_DEVICE	  = 4, /* a HIP call failed */
+	PARS_ERR_BUDGET	  = 5, /* serving: the request does not fit the HBM budget */
+	PARS_ERR_BUSY	  = 6  /* serving: no free slot right now, retry */
+} pars_status;
+
+/* pipeline.cuh:745-748: per-stage milliseconds measured with events on the
+ * call's own stream.  p1 = R1-R5 (one fused kernel), p2 = 0, p3 = 0 unless the
+ * staged mode splits them, p4 = R6 pairing. */
+typedef struct {
+	double p1, p2, p3, p4;
+} pars_stage_times;
+
+/* pipeline.cuh:722-728 plus the extra outputs this backend produces. */
+typedef struct {
+	const uint32_t* structural;	  /* device: byte offsets of unmasked structural bytes */
+	const int32_t*	pair_pos;	  /* device: partner structural index or -1, or NULL */
+	uint32_t		n_struct;
+	int32_t			balanced;
+	int32_t			well_formed;
+	int32_t			max_depth;
+	/* extensions (not in the reference device_result) */
+	const uint8_t*	chars;		  /* device: structural bytes, or NULL unless enabled */
+	const uint64_t* struct_bitmap; /* device: ceil(n/64) words, bit i = byte i emitted */
+	const uint32_t* bracket_pos;  /* device: byte offsets of unmasked brackets */
+	const uint8_t*	bracket_char; /* device */
+	const int32_t*	bracket_pair; /* device: CPU convention (nest.hpp:18), or NULL */
+	uint32_t		n_brackets;
+	int32_t			unclosed_mode; /* scan.hpp:66-70 */
+} pars_device_result;
+
+/* pipeline.cuh:730-735; buffers are owned by the scanner (pinned host memory),
+ * valid until the next call on it. */
+typedef struct {
+	const uint32_t* structural;
+	const int32_t*	pair_pos; /* NULL unless want_pairs */
+	uint32_t		n_struct;
+	int32_t			balanced;
+	int32_t			well_formed;
+	int32_t			max_depth;
+	int32_t			unclosed_mode;
+} pars_host_result;
+
+/* pipeline.cuh:737-743, widened: `scan` holds the per-descriptor states after
+ * the chunk (3 bits each, plus bit 31 set = valid; 0 = document start, so a
+ * zero-initialised carry starts a document exactly as the reference's `{}`).
+ * depth/bracket_base/min_depth/max_depth carry R6 state across chunks. */
+typedef struct {
+	uint64_t scan;
+	uint32_t out_base;
+	uint32_t bracket_base;
+	int32_t	 depth;
+	int32_t	 min_depth;
+	int32_t	 max_depth;
+	uint32_t reserved;
+} pars_chunk_carry;
+
+/* Options (pars_hip_set_option). */
+typedef enum {
+	PARS_OPT_WANT_CHARS	 = 1, /* also emit structural bytes (default 0) */
+	PARS_OPT_WANT_BITMAP = 2, /* also emit the structural bitmap (default 1) */
+	PARS_OPT_PATH		 = 3  /* 0 = auto, 1 = force the general descriptor path */
+} pars_option;
+
+/* ---- scanner lifetime (reference: gpu::scanner<S> constructor/destructor) ---- */
+int pars_hip_scanner_create(const pars_plan* plan, pars_scanner** out);
+int pars_hip_scanner_destroy(pars_scanner* sc);
+int pars_hip_set_option(pars_scanner* sc, int option, int64_t value);
+
+/* pipeline.cuh:1167-1172 scanner::scan_device.  d_in: device bytes [0, n).
+ * want_pairs: fill pair_pos (structural-index convention) and bracket_pair.
+ * run_pairs: run R6 (flags stay as computed by the depth scan otherwise).
+ * st: optional stage times.  Blocks until results are ready (like the
+ * reference, which returns counts by value). */
+int pars_hip_scan_device(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, int want_pairs, int run_pairs,
+	pars_stage_times* st, pars_device_result* out
+);
+
+/* pipeline.cuh:1176-1212 scanner::scan_host: pinned staging, H2D, scan,
+ * D2H of the structural index (and pair_pos when want_pairs). */
+int pars_hip_scan_host(
+	pars_scanner* sc, const uint8_t* h_in, size_t n, pars_stream stream, int want_pairs,
+	pars_host_result* out
+);
+
+/* pipeline.cuh:1216 scanner::reserve_struct: size the structural output for a
+ * chunked document of up to `cap` structural bytes. */
+int pars_hip_reserve_struct(pars_scanner* sc, size_t cap);
+
+/* pipeline.cuh:1218-1223 scanner::scan_chunk: stage 1 of one chunk, writing at
+ * in->out_base, positions offset by byte_base; works for every plan (the
+ * reference's carry covers one packed 16-slot transform).  Blocks until done. */
+int pars_hip_scan_chunk(
+	pars_scanner* sc, const uint8_t* d_chunk, size_t n, const pars_chunk_carry* in, pars_stream stream,
+	size_t byte_base, pars_chunk_carry* out
+);
+
+/* R6 over everything the chunks of one document emitted (the reference has
+ * no pairing across chunks, docs/model/model.md:355-359). */
+int pars_hip_pair_chunks(
+	pars_scanner* sc, const pars_chunk_carry* last, pars_stream stream, int want_pairs,
+	pars_device_result* out
+);
+
+/* pipeline.cuh:1225 scanner::struct_ptr. */
+const uint32_t* pars_hip_struct_ptr(const pars_scanner* sc);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* pars_hip_last_error(void);
+
+/* Library version string. */
+const char* pars_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARS_HIP_H */

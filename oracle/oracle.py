@@ -1,0 +1,161 @@
+"""Python binding of the CPU oracle (oracle/liboracle.so, oracle/pars_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never touches
+it.  Also drives the reference's own CPU backend (oracle/_ref/ref_driver)
+where it was built, to write golden vectors and to time the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json as _json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF = os.path.join(HERE, "_ref", "ref_driver")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle liboracle.so`")
+        L = C.CDLL(LIB)
+        L.oracle_classifier.argtypes = [C.c_char_p, C.c_void_p]
+        L.oracle_scan.restype = C.c_int
+        L.oracle_scan.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_pair.restype = C.c_int
+        L.oracle_pair.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+        L.oracle_final_states.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+        L.oracle_fnv1a.restype = C.c_uint64
+        L.oracle_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+FNV0 = 0xCBF29CE484222325
+
+
+def fnv(arr) -> int:
+    a = np.ascontiguousarray(arr)
+    return int(lib().oracle_fnv1a(FNV0, a.ctypes.data, a.nbytes))
+
+
+def _u8(data) -> np.ndarray:
+    if isinstance(data, (bytes, bytearray)):
+        return np.frombuffer(bytes(data), dtype=np.uint8)
+    return np.ascontiguousarray(data, dtype=np.uint8)
+
+
+def classifier(plan: bytes) -> np.ndarray:
+    out = np.zeros(256, dtype=np.uint32)
+    lib().oracle_classifier(plan, out.ctypes.data)
+    return out
+
+
+def scan(plan: bytes, data, cap: int | None = None, bitmaps: bool = False) -> dict:
+    """model::scanner<S>::scan (scan.hpp:27-73) + the per-byte masked/structural bitmaps."""
+    d = _u8(data)
+    n = d.size
+    cap = n if cap is None else cap
+    pos = np.empty(cap, np.uint32)
+    ch = np.empty(cap, np.uint8)
+    bpos = np.empty(cap, np.uint32)
+    bch = np.empty(cap, np.uint8)
+    counts = np.zeros(2, np.uint64)
+    words = (n + 63) // 64
+    mb = np.zeros(words, np.uint64) if bitmaps else None
+    sb = np.zeros(words, np.uint64) if bitmaps else None
+    unclosed = lib().oracle_scan(
+        plan, d.ctypes.data, n, pos.ctypes.data, ch.ctypes.data, bpos.ctypes.data, bch.ctypes.data, cap,
+        counts.ctypes.data, mb.ctypes.data if bitmaps else None, sb.ctypes.data if bitmaps else None,
+    )
+    k, kb = int(counts[0]), int(counts[1])
+    if k > cap or kb > cap:
+        raise ValueError("output capacity too small")
+    r = {"positions": pos[:k].copy(), "chars": ch[:k].copy(), "bracket_pos": bpos[:kb].copy(),
+         "bracket_char": bch[:kb].copy(), "unclosed_mode": bool(unclosed)}
+    if bitmaps:
+        r["masked_bitmap"] = mb
+        r["struct_bitmap"] = sb
+    return r
+
+
+def pair(plan: bytes, bracket_char) -> dict:
+    """model::pair_brackets<S> (nest.hpp:45-92)."""
+    b = _u8(bracket_char)
+    out = np.empty(b.size, np.int32)
+    flags = np.zeros(3, np.int32)
+    if lib().oracle_pair(plan, b.ctypes.data, b.size, out.ctypes.data, flags.ctypes.data) != 0:
+        raise MemoryError("oracle_pair")
+    return {"pair": out, "balanced": bool(flags[0]), "well_formed": bool(flags[1]), "max_depth": int(flags[2])}
+
+
+def full(plan: bytes, data) -> dict:
+    r = scan(plan, data)
+    r.update(pair(plan, r["bracket_char"]))
+    return r
+
+
+def final_states(plan: bytes, data, in_states=None) -> np.ndarray:
+    d = _u8(data)
+    out = np.zeros(8, np.uint8)
+    ins = None if in_states is None else np.ascontiguousarray(in_states, np.uint8)
+    lib().oracle_final_states(plan, d.ctypes.data, d.size, ins.ctypes.data if ins is not None else None,
+                              out.ctypes.data)
+    return out
+
+
+def pair_pos_device(positions: np.ndarray, bracket_pos: np.ndarray, pair_cpu: np.ndarray) -> np.ndarray:
+    """The reference GPU convention (pipeline.cuh:492-500): per structural index, the
+    partner's structural index or -1, derived from the CPU pair table."""
+    k = positions.size
+    out = np.full(k, -1, np.int32)
+    if bracket_pos.size:
+        sidx = np.searchsorted(positions, bracket_pos).astype(np.int64)
+        ok = pair_cpu >= 0
+        out[sidx[ok]] = sidx[pair_cpu[ok]].astype(np.int32)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the reference's own CPU backend (built from its sources into oracle/_ref/)
+# ---------------------------------------------------------------------------
+def ref_available() -> bool:
+    return os.path.exists(REF)
+
+
+def ref_full(grammar: str, data) -> dict:
+    d = _u8(data)
+    with tempfile.TemporaryDirectory() as td:
+        src, dst = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        d.tofile(src)
+        subprocess.run([REF, "scan", grammar, src, dst], check=True)
+        raw = open(dst, "rb").read()
+    if raw[:8] != b"PRSREF01":
+        raise ValueError("bad ref output")
+    np_, nb = np.frombuffer(raw, np.uint64, 2, 8)
+    np_, nb = int(np_), int(nb)
+    hdr = np.frombuffer(raw, np.int32, 4, 24)
+    o = 40
+    pos = np.frombuffer(raw, np.uint32, np_, o); o += 4 * np_
+    ch = np.frombuffer(raw, np.uint8, np_, o); o += np_
+    bpos = np.frombuffer(raw, np.uint32, nb, o); o += 4 * nb
+    bch = np.frombuffer(raw, np.uint8, nb, o); o += nb
+    pr = np.frombuffer(raw, np.int32, nb, o)
+    return {"positions": pos.copy(), "chars": ch.copy(), "bracket_pos": bpos.copy(), "bracket_char": bch.copy(),
+            "pair": pr.copy(), "unclosed_mode": bool(hdr[0]), "balanced": bool(hdr[1]),
+            "well_formed": bool(hdr[2]), "max_depth": int(hdr[3])}
+
+
+def ref_time(grammar: str, path: str, max_bytes: int) -> dict:
+    r = subprocess.run([REF, "time", grammar, path, str(max_bytes)], check=True, stdout=subprocess.PIPE, text=True)
+    return _json.loads(r.stdout)

@@ -1,0 +1,63 @@
+/* pars_oracle.h — CPU restatement of the reference's scalar backend.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this; it is the checker, never the
+ * product path.  Functions and the reference code they restate:
+ *
+ *   oracle_classifier  include/pars/model/classify.hpp:39-55
+ *   oracle_scan        include/pars/model/scan.hpp:27-73
+ *   oracle_pair        include/pars/model/nest.hpp:45-92
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   - golden vectors written by the reference compiled from its own sources
+ *     (oracle/ref_driver.cpp, built by oracle/Makefile into oracle/_ref/),
+ *     committed under tests/golden/ and compared in tests/test_oracle.py;
+ *   - the reference's known-answer tests (tests/test_model.cpp:152-257)
+ *     re-run against this file in tests/test_oracle.py;
+ *   - brute force: every string up to length 8 over each descriptor's class
+ *     alphabet against a textbook DFA walk, and a stack matcher for pairing on
+ *     balanced input (tests/test_oracle.py).
+ */
+#ifndef PARS_ORACLE_H
+#define PARS_ORACLE_H
+
+#include "../include/pars_plan.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* classify.hpp:39-55: byte -> class bits (PARS_CBIT_*). */
+void oracle_classifier(const pars_plan* S, uint32_t bits[256]);
+
+/* scan.hpp:27-73.  Writes at most `cap` entries into each output array (any
+ * may be NULL) and returns the true counts in counts[0] (structural) and
+ * counts[1] (brackets).  `masked_bits` (NULL or ceil(n/64) words) receives the
+ * per-byte OR of the descriptors' masked bits; `struct_bits` (NULL or
+ * ceil(n/64) words) the unmasked structural bytes.  Returns unclosed_mode. */
+int oracle_scan(
+	const pars_plan* S, const uint8_t* data, size_t n, uint32_t* positions, uint8_t* chars,
+	uint32_t* bracket_pos, uint8_t* bracket_char, size_t cap, uint64_t counts[2],
+	uint64_t* masked_bits, uint64_t* struct_bits
+);
+
+/* nest.hpp:45-92.  pair[nb] receives the partner bracket index or -1;
+ * flags[0..2] = balanced, well_formed, max_depth.  Returns 0, or -1 when out
+ * of memory. */
+int oracle_pair(const pars_plan* S, const uint8_t* bchar, size_t nb, int32_t* pair, int32_t flags[3]);
+
+/* Final per-descriptor state after scanning `data` from `in_states` (NULL =
+ * start states): the carry a chunked scan threads between chunks. */
+void oracle_final_states(
+	const pars_plan* S, const uint8_t* data, size_t n, const uint8_t* in_states, uint8_t* out_states
+);
+
+/* 64-bit FNV-1a over a byte range, chained from `h` (start with
+ * 0xcbf29ce484222325). */
+uint64_t oracle_fnv1a(uint64_t h, const void* p, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

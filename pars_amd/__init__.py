@@ -1,0 +1,244 @@
+"""pars_amd — MI355X (gfx950) backend for pars scan plans.
+
+Python mirror of the reference backend's boundary
+(include/pars/backend/cuda/pipeline.cuh:1164-1281): a `Scanner` per plan with
+`scan_device`, `scan_host`, `reserve_struct`, `scan_chunk` and `struct_ptr`,
+and the free functions `scan_device` / `scan_host` / `scan_chunk` that keep one
+scanner per plan per thread (pipeline.cuh:1260-1281).  Everything calls the C
+ABI in include/pars_hip.h (libpars_hip.so); there is no CPU fallback: when
+the library is missing the first call raises.
+
+Device memory is passed as raw device pointers (ints); any allocator works
+(the tests and bench use torch tensors' data_ptr()).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+from . import _native as N
+
+__all__ = [
+    "ParsError",
+    "Plan",
+    "Scanner",
+    "DeviceScan",
+    "HostScan",
+    "ChunkCarry",
+    "scan_device",
+    "scan_host",
+    "scan_chunk",
+    "PLAN_DIR",
+]
+
+PLAN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plans")
+ChunkCarry = N.ChunkCarry
+
+
+class ParsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{N.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def _check(status: int) -> None:
+    if status != N.PARS_OK:
+        raise ParsError(status, N.lib().pars_hip_last_error().decode(errors="replace"))
+
+
+class Plan:
+    """A compiled scan plan: the 3177 bytes of pars::scan_plan (include/pars/plan.hpp:39-79)."""
+
+    def __init__(self, blob: bytes, name: str = "plan"):
+        if len(blob) != N.PLAN_BYTES:
+            raise ValueError(f"a plan is {N.PLAN_BYTES} bytes, got {len(blob)}")
+        self.blob = bytes(blob)
+        self.name = name
+
+    @classmethod
+    def load(cls, name_or_path: str) -> "Plan":
+        path = name_or_path
+        if not os.path.exists(path):
+            path = os.path.join(PLAN_DIR, name_or_path + ".plan")
+        with open(path, "rb") as f:
+            return cls(f.read(), os.path.splitext(os.path.basename(path))[0])
+
+    # plan.hpp field views
+    @property
+    def punctuation(self) -> bytes:
+        return self.blob[0:8][: self.blob[44]]
+
+    @property
+    def brackets(self) -> list[tuple[int, int]]:
+        nb = self.blob[45]
+        return [(self.blob[8 + 2 * i], self.blob[9 + 2 * i]) for i in range(nb)]
+
+    @property
+    def n_descs(self) -> int:
+        return self.blob[48 + 8 * 391]
+
+    def desc(self, d: int) -> dict:
+        o = 48 + 391 * d
+        b = self.blob
+        return {
+            "kind": b[o],
+            "n_states": b[o + 1],
+            "n_classes": b[o + 2],
+            "start": b[o + 3],
+            "ok": b[o + 4],
+            "delim": b[o + 5],
+            "escape": b[o + 6],
+            "rep_of": b[o + 7 : o + 263],
+            "trans": [list(b[o + 263 + 8 * s : o + 271 + 8 * s]) for s in range(8)],
+            "masked": [list(b[o + 327 + 8 * s : o + 335 + 8 * s]) for s in range(8)],
+        }
+
+    def __repr__(self) -> str:
+        return f"Plan({self.name!r}, descs={self.n_descs}, punct={self.punctuation!r}, brackets={self.brackets})"
+
+
+class DeviceScan:
+    """pipeline.cuh:722-728 device_result; pointers alias the scanner's arena."""
+
+    def __init__(self, r: N.DeviceResult, n: int):
+        self.n = n
+        self.structural = r.structural or 0
+        self.pair_pos = r.pair_pos or 0
+        self.n_struct = r.n_struct
+        self.balanced = bool(r.balanced)
+        self.well_formed = bool(r.well_formed)
+        self.max_depth = r.max_depth
+        self.chars = r.chars or 0
+        self.struct_bitmap = r.struct_bitmap or 0
+        self.bracket_pos = r.bracket_pos or 0
+        self.bracket_char = r.bracket_char or 0
+        self.bracket_pair = r.bracket_pair or 0
+        self.n_brackets = r.n_brackets
+        self.unclosed_mode = bool(r.unclosed_mode)
+
+    def __repr__(self) -> str:
+        return (
+            f"DeviceScan(n_struct={self.n_struct}, n_brackets={self.n_brackets}, balanced={self.balanced}, "
+            f"well_formed={self.well_formed}, max_depth={self.max_depth}, unclosed={self.unclosed_mode})"
+        )
+
+
+class HostScan:
+    """pipeline.cuh:730-735 host_result (arrays copied out of the pinned buffers)."""
+
+    def __init__(self, r: N.HostResult):
+        import numpy as np
+
+        n = r.n_struct
+        self.structural = np.ctypeslib.as_array(r.structural, (n,)).copy() if n else np.zeros(0, np.uint32)
+        if r.pair_pos:
+            self.pair_pos = np.ctypeslib.as_array(r.pair_pos, (n,)).copy() if n else np.zeros(0, np.int32)
+        else:
+            self.pair_pos = None
+        self.balanced = bool(r.balanced)
+        self.well_formed = bool(r.well_formed)
+        self.max_depth = r.max_depth
+        self.unclosed_mode = bool(r.unclosed_mode)
+
+
+class Scanner:
+    """gpu::scanner<S> (pipeline.cuh:1164-1258) for a runtime plan."""
+
+    def __init__(self, plan: Plan | str | bytes):
+        if isinstance(plan, str):
+            plan = Plan.load(plan)
+        elif isinstance(plan, (bytes, bytearray)):
+            plan = Plan(bytes(plan))
+        self.plan = plan
+        self._lib = N.lib()
+        h = C.c_void_p()
+        _check(self._lib.pars_hip_scanner_create(plan.blob, C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pars_hip_scanner_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_option(self, option: int, value: int) -> None:
+        _check(self._lib.pars_hip_set_option(self._h, option, value))
+
+    def scan_device(
+        self, d_data: int, n: int, stream: int | None = None, want_pairs: bool = False,
+        run_pairs: bool = True, stage_times: N.StageTimes | None = None,
+    ) -> DeviceScan:
+        r = N.DeviceResult()
+        _check(
+            self._lib.pars_hip_scan_device(
+                self._h, C.c_void_p(d_data), n, C.c_void_p(stream or 0), int(want_pairs), int(run_pairs),
+                C.byref(stage_times) if stage_times is not None else None, C.byref(r),
+            )
+        )
+        return DeviceScan(r, n)
+
+    def scan_host(self, data: bytes, stream: int | None = None, want_pairs: bool = False) -> HostScan:
+        r = N.HostResult()
+        _check(self._lib.pars_hip_scan_host(self._h, bytes(data), len(data), C.c_void_p(stream or 0),
+                                            int(want_pairs), C.byref(r)))
+        return HostScan(r)
+
+    def reserve_struct(self, cap: int) -> None:
+        _check(self._lib.pars_hip_reserve_struct(self._h, cap))
+
+    def scan_chunk(self, d_chunk: int, n: int, carry: N.ChunkCarry, stream: int | None = None,
+                   byte_base: int = 0) -> N.ChunkCarry:
+        out = N.ChunkCarry()
+        _check(self._lib.pars_hip_scan_chunk(self._h, C.c_void_p(d_chunk), n, C.byref(carry),
+                                             C.c_void_p(stream or 0), byte_base, C.byref(out)))
+        return out
+
+    def pair_chunks(self, last: N.ChunkCarry, stream: int | None = None, want_pairs: bool = False) -> DeviceScan:
+        r = N.DeviceResult()
+        _check(self._lib.pars_hip_pair_chunks(self._h, C.byref(last), C.c_void_p(stream or 0),
+                                              int(want_pairs), C.byref(r)))
+        return DeviceScan(r, 0)
+
+    def struct_ptr(self) -> int:
+        return self._lib.pars_hip_struct_ptr(self._h) or 0
+
+
+_tls = threading.local()
+
+
+def _thread_scanner(plan: Plan) -> Scanner:
+    cache = getattr(_tls, "scanners", None)
+    if cache is None:
+        cache = _tls.scanners = {}
+    sc = cache.get(plan.blob)
+    if sc is None:
+        sc = cache[plan.blob] = Scanner(plan)
+    return sc
+
+
+def scan_device(plan: Plan, d_data: int, n: int, stream: int | None = None, want_pairs: bool = False) -> DeviceScan:
+    """pipeline.cuh:1260-1266 (one scanner per plan per thread)."""
+    return _thread_scanner(plan).scan_device(d_data, n, stream, want_pairs)
+
+
+def scan_host(plan: Plan, data: bytes, want_pairs: bool = False) -> HostScan:
+    """pipeline.cuh:1268-1272."""
+    return _thread_scanner(plan).scan_host(data, None, want_pairs)
+
+
+def scan_chunk(plan: Plan, d_chunk: int, n: int, carry: N.ChunkCarry, stream: int | None = None,
+               byte_base: int = 0) -> N.ChunkCarry:
+    """pipeline.cuh:1274-1281."""
+    return _thread_scanner(plan).scan_chunk(d_chunk, n, carry, stream, byte_base)

@@ -1,0 +1,147 @@
+"""ctypes view of include/pars_hip.h (structures and prototypes)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpars_hip.so")
+GEN_PATH = os.path.join(HERE, "libpars_gen.so")
+
+PLAN_BYTES = 3177
+
+PARS_OK = 0
+STATUS_NAMES = {
+    0: "PARS_OK",
+    1: "PARS_ERR_ARGUMENT",
+    2: "PARS_ERR_PLAN",
+    3: "PARS_ERR_MEMORY",
+    4: "PARS_ERR_DEVICE",
+    5: "PARS_ERR_BUDGET",
+    6: "PARS_ERR_BUSY",
+}
+
+PARS_OPT_WANT_CHARS = 1
+PARS_OPT_WANT_BITMAP = 2
+PARS_OPT_PATH = 3
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("p1", C.c_double), ("p2", C.c_double), ("p3", C.c_double), ("p4", C.c_double)]
+
+
+class DeviceResult(C.Structure):
+    _fields_ = [
+        ("structural", C.c_void_p),
+        ("pair_pos", C.c_void_p),
+        ("n_struct", C.c_uint32),
+        ("balanced", C.c_int32),
+        ("well_formed", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("chars", C.c_void_p),
+        ("struct_bitmap", C.c_void_p),
+        ("bracket_pos", C.c_void_p),
+        ("bracket_char", C.c_void_p),
+        ("bracket_pair", C.c_void_p),
+        ("n_brackets", C.c_uint32),
+        ("unclosed_mode", C.c_int32),
+    ]
+
+
+class HostResult(C.Structure):
+    _fields_ = [
+        ("structural", C.POINTER(C.c_uint32)),
+        ("pair_pos", C.POINTER(C.c_int32)),
+        ("n_struct", C.c_uint32),
+        ("balanced", C.c_int32),
+        ("well_formed", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("unclosed_mode", C.c_int32),
+    ]
+
+
+class ChunkCarry(C.Structure):
+    _fields_ = [
+        ("scan", C.c_uint64),
+        ("out_base", C.c_uint32),
+        ("bracket_base", C.c_uint32),
+        ("depth", C.c_int32),
+        ("min_depth", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/pars_hip.h declares
+PROTOTYPES = {
+    "pars_hip_scanner_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "pars_hip_scanner_destroy": (C.c_int, [C.c_void_p]),
+    "pars_hip_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+    "pars_hip_scan_device": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int, C.POINTER(StageTimes),
+         C.POINTER(DeviceResult)],
+    ),
+    "pars_hip_scan_host": (
+        C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_int, C.POINTER(HostResult)]
+    ),
+    "pars_hip_reserve_struct": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "pars_hip_scan_chunk": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(ChunkCarry), C.c_void_p, C.c_size_t,
+         C.POINTER(ChunkCarry)],
+    ),
+    "pars_hip_pair_chunks": (
+        C.c_int, [C.c_void_p, C.POINTER(ChunkCarry), C.c_void_p, C.c_int, C.POINTER(DeviceResult)]
+    ),
+    "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
+    "pars_hip_last_error": (C.c_char_p, []),
+    "pars_hip_version": (C.c_char_p, []),
+}
+
+GEN_PROTOTYPES = {
+    "pars_gen_json": (
+        C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]
+    ),
+    "pars_gen_csv": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p]),
+    "pars_gen_clike": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int, C.c_uint64, C.c_void_p]),
+    "pars_gen_alpha": (C.c_int, [C.c_uint64, C.c_uint64, C.c_char_p, C.c_uint64, C.c_void_p]),
+    "pars_gen_sizes": (
+        C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_uint64, C.c_void_p]
+    ),
+    "pars_gen_doc_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+}
+
+
+def _bind(lib, protos):
+    for name, (res, args) in protos.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+_gen = None
+
+
+def lib():
+    """The HIP backend.  Raises if it was not built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m pars_amd.build` "
+                "(the scan path has no CPU fallback)"
+            )
+        _lib = _bind(C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL), PROTOTYPES)
+    return _lib
+
+
+def gen():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_PATH):
+            raise RuntimeError(f"{GEN_PATH} is missing: build it with `python -m pars_amd.build`")
+        _gen = _bind(C.CDLL(GEN_PATH), GEN_PROTOTYPES)
+    return _gen

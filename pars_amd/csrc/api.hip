@@ -1,0 +1,533 @@
+// api.hip — host orchestration behind the C ABI (include/pars_hip.h).
+//
+// Reference boundary: include/pars/backend/cuda/pipeline.cuh:1164-1281.  The
+// reference sizes its outputs after a blocking count readback (:890-895) and
+// syncs again for the R6 flags (:972-981).  Here the whole chain is enqueued
+// without a host round trip — every kernel reads its sizes from the device
+// summary — and one synchronisation at the end reads the summary back.
+// Outputs are sized from the input length (structural <= n) and an optimistic
+// bracket capacity; if the device reports more brackets than fit, the scan is
+// rerun once with the exact capacity.
+#include "../../include/pars_hip.h"
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+namespace pars_hip {
+uint64_t pairing_max_chunks(uint64_t cap_br);
+}
+
+using namespace pars_hip;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+	char	buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+	return fail(PARS_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct DevBuf {
+	void*  p   = nullptr;
+	size_t cap = 0;
+
+	void release() {
+		if (p)
+			(void)hipFree(p);
+		p	= nullptr;
+		cap = 0;
+	}
+
+	// Grow-only; `zero` clears a fresh allocation (look-back records, tickets).
+	int ensure(size_t bytes, bool zero = false) {
+		if (bytes <= cap && p)
+			return PARS_OK;
+		release();
+		const size_t want = bytes ? bytes : 64;
+		hipError_t	 e	  = hipMalloc(&p, want);
+		if (e != hipSuccess) {
+			p = nullptr;
+			return fail(PARS_ERR_MEMORY, "hipMalloc(%zu bytes): %s", want, hipGetErrorString(e));
+		}
+		cap = want;
+		if (zero) {
+			e = hipMemset(p, 0, want);
+			if (e != hipSuccess)
+				return hip_fail(e, "hipMemset");
+		}
+		return PARS_OK;
+	}
+
+	template<class T>
+	T* as() const {
+		return static_cast<T*>(p);
+	}
+};
+
+}  // namespace
+
+struct pars_scanner {
+	pars_plan plan;
+	DevPlan	  hplan;
+	DevBuf	  dplan;
+	DevBuf	  recs, ticket, summary;
+	DevBuf	  positions, chars, bitmap, pair_pos;
+	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, pair, hist, hist_part;
+	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
+	uint32_t  epoch		 = 0;
+	int		  want_chars = 0, want_bitmap = 1, path = 0;
+	DevSummary* h_sum	 = nullptr;	 // pinned
+	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
+	// scan_host staging
+	uint8_t*  h_in		  = nullptr;
+	size_t	  h_in_cap	  = 0;
+	DevBuf	  d_in;
+	uint32_t* h_struct	  = nullptr;
+	size_t	  h_struct_cap = 0;
+	int32_t*  h_pairs	  = nullptr;
+	size_t	  h_pairs_cap = 0;
+
+	~pars_scanner() {
+		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
+						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &pair, &hist, &hist_part, &d_in})
+			b->release();
+		if (h_sum)
+			(void)hipHostFree(h_sum);
+		if (h_in)
+			(void)hipHostFree(h_in);
+		if (h_struct)
+			(void)hipHostFree(h_struct);
+		if (h_pairs)
+			(void)hipHostFree(h_pairs);
+		for (hipEvent_t& e : ev)
+			if (e)
+				(void)hipEventDestroy(e);
+	}
+
+	uint32_t next_epoch() {
+		epoch = (epoch + 1) & 0x3FFFFFFFu;
+		if (epoch == 0)
+			epoch = 1;
+		return epoch;
+	}
+};
+
+namespace {
+
+int ensure_pinned(void** p, size_t& cap, size_t bytes) {
+	if (bytes <= cap && *p)
+		return PARS_OK;
+	if (*p)
+		(void)hipHostFree(*p);
+	*p				  = nullptr;
+	cap				  = 0;
+	const size_t want = std::max<size_t>(bytes, 64);
+	hipError_t	 e	  = hipHostMalloc(p, want, hipHostMallocDefault);
+	if (e != hipSuccess) {
+		*p = nullptr;
+		return fail(PARS_ERR_MEMORY, "hipHostMalloc(%zu): %s", want, hipGetErrorString(e));
+	}
+	cap = want;
+	return PARS_OK;
+}
+
+// Capacity for per-structural and per-bracket outputs of a scan of n bytes.
+int ensure_outputs(pars_scanner* sc, uint64_t n_tiles, uint64_t cap_struct, uint64_t cap_br, bool want_pairs) {
+	int r;
+	if ((r = sc->recs.ensure(n_tiles * kRecWords * sizeof(uint64_t), true)))
+		return r;
+	if ((r = sc->positions.ensure(cap_struct * 4)))
+		return r;
+	if (sc->want_chars && (r = sc->chars.ensure(cap_struct)))
+		return r;
+	if (want_pairs && (r = sc->pair_pos.ensure(cap_struct * 4)))
+		return r;
+	for (DevBuf* b : {&sc->bpos, &sc->bsidx, &sc->bkey, &sc->ukeys0, &sc->ukeys1, &sc->order0, &sc->order1, &sc->pair})
+		if ((r = b->ensure(cap_br * 4)))
+			return r;
+	if ((r = sc->bchar.ensure(cap_br)))
+		return r;
+	const uint64_t chunks = pairing_max_chunks(cap_br);
+	if ((r = sc->hist.ensure(256 * std::max<uint64_t>(chunks, 1) * 4)))
+		return r;
+	if ((r = sc->hist_part.ensure(256 * 4)))
+		return r;
+	sc->cap_struct = cap_struct;
+	sc->cap_br	   = cap_br;
+	return PARS_OK;
+}
+
+ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool want_pairs) {
+	ScanArgs a {};
+	a.data		 = d_in;
+	a.n			 = n;
+	a.n_tiles	 = (uint32_t)((n + kTile - 1) / kTile);
+	a.epoch		 = sc->next_epoch();
+	a.plan		 = sc->dplan.as<DevPlan>();
+	a.recs		 = sc->recs.as<uint64_t>();
+	a.ticket	 = sc->ticket.as<uint32_t>();
+	a.sum		 = sc->summary.as<DevSummary>();
+	a.positions	 = sc->positions.as<uint32_t>();
+	a.chars		 = sc->want_chars ? sc->chars.as<uint8_t>() : nullptr;
+	a.bitmap	 = sc->want_bitmap ? sc->bitmap.as<uint64_t>() : nullptr;
+	a.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	a.bpos		 = sc->bpos.as<uint32_t>();
+	a.bsidx		 = sc->bsidx.as<uint32_t>();
+	a.bkey		 = sc->bkey.as<int32_t>();
+	a.bchar		 = sc->bchar.as<uint8_t>();
+	a.cap_struct = sc->cap_struct;
+	a.cap_br	 = sc->cap_br;
+	a.in_states	 = sc->hplan.start_packed;
+	return a;
+}
+
+PairArgs make_pair_args(pars_scanner* sc, bool want_pairs) {
+	PairArgs p {};
+	p.sum		 = sc->summary.as<DevSummary>();
+	p.plan		 = sc->dplan.as<DevPlan>();
+	p.bkey		 = sc->bkey.as<int32_t>();
+	p.bchar		 = sc->bchar.as<uint8_t>();
+	p.bsidx		 = sc->bsidx.as<uint32_t>();
+	p.cap_br	 = sc->cap_br;
+	p.ukeys[0]	 = sc->ukeys0.as<uint32_t>();
+	p.ukeys[1]	 = sc->ukeys1.as<uint32_t>();
+	p.order[0]	 = sc->order0.as<uint32_t>();
+	p.order[1]	 = sc->order1.as<uint32_t>();
+	p.hist		 = sc->hist.as<uint32_t>();
+	p.hist_part	 = sc->hist_part.as<uint32_t>();
+	p.max_chunks = pairing_max_chunks(sc->cap_br);
+	p.pair		 = sc->pair.as<int32_t>();
+	p.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	return p;
+}
+
+int read_summary(pars_scanner* sc, hipStream_t s) {
+	hipError_t e = hipMemcpyAsync(sc->h_sum, sc->summary.p, sizeof(DevSummary), hipMemcpyDeviceToHost, s);
+	if (e != hipSuccess)
+		return hip_fail(e, "summary readback");
+	e = hipStreamSynchronize(s);
+	if (e != hipSuccess)
+		return hip_fail(e, "hipStreamSynchronize");
+	if (sc->h_sum->overflow & 0x80000000u)
+		return fail(PARS_ERR_DEVICE, "scan look-back exceeded its spin bound (internal error)");
+	return PARS_OK;
+}
+
+void fill_result(pars_scanner* sc, uint64_t n, bool want_pairs, bool run_pairs, pars_device_result* out) {
+	const DevSummary& h = *sc->h_sum;
+	std::memset(out, 0, sizeof *out);
+	out->structural	   = sc->positions.as<uint32_t>();
+	out->pair_pos	   = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	out->n_struct	   = h.n_struct;
+	out->balanced	   = (h.min_bal >= 0) && (h.last_bal == 0);
+	out->well_formed   = out->balanced && !h.mismatched && (h.n_brackets % 2 == 0);
+	out->max_depth	   = h.max_bal;
+	out->chars		   = sc->want_chars ? sc->chars.as<uint8_t>() : nullptr;
+	out->struct_bitmap = sc->want_bitmap ? sc->bitmap.as<uint64_t>() : nullptr;
+	out->bracket_pos   = sc->bpos.as<uint32_t>();
+	out->bracket_char  = sc->bchar.as<uint8_t>();
+	out->bracket_pair  = run_pairs ? sc->pair.as<int32_t>() : nullptr;
+	out->n_brackets	   = h.n_brackets;
+	const uint32_t nd  = sc->hplan.n_descs;
+	const uint32_t m   = nd ? ((1u << (3 * nd)) - 1u) : 0u;
+	out->unclosed_mode = n ? ((h.final_states & m) != (sc->hplan.start_packed & m)) : 0;
+	if (!run_pairs)
+		out->well_formed = out->balanced && (h.n_brackets % 2 == 0);
+}
+
+// One full document: reset, R1-R5, optional R6; no host sync inside.
+int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStream_t s, bool want_pairs,
+					 bool run_pairs, bool timed) {
+	hipError_t e;
+	if ((e = launch_summary_reset(sc->summary.as<DevSummary>(), SummaryInit {}, s)) != hipSuccess)
+		return hip_fail(e, "summary reset");
+	if (timed && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	ScanArgs a = make_scan_args(sc, d_in, n, want_pairs);
+	if ((e = launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s)) != hipSuccess)
+		return hip_fail(e, "scan kernel launch");
+	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if (run_pairs) {
+		PairArgs p = make_pair_args(sc, want_pairs);
+		if ((e = launch_pairing(p, s)) != hipSuccess)
+			return hip_fail(e, "pairing launch");
+	}
+	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	return PARS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pars_hip_version(void) {
+	return "pars-hip 0.1 (gfx950)";
+}
+
+const char* pars_hip_last_error(void) {
+	return g_err.c_str();
+}
+
+int pars_hip_scanner_create(const pars_plan* plan, pars_scanner** out) {
+	if (!plan || !out)
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	*out				= nullptr;
+	pars_scanner* sc	= new (std::nothrow) pars_scanner();
+	if (!sc)
+		return fail(PARS_ERR_MEMORY, "host allocation failed");
+	std::memcpy(&sc->plan, plan, sizeof(pars_plan));
+	const int lr = lower_plan(sc->plan, sc->hplan);
+	if (lr != 0) {
+		delete sc;
+		return fail(PARS_ERR_PLAN, "plan rejected (code %d): descriptor tables exceed the plan.hpp caps", lr);
+	}
+	int r;
+	if ((r = sc->dplan.ensure(sizeof(DevPlan))) || (r = sc->ticket.ensure(64, true))
+		|| (r = sc->summary.ensure(sizeof(DevSummary), true))) {
+		delete sc;
+		return r;
+	}
+	hipError_t e = hipMemcpy(sc->dplan.p, &sc->hplan, sizeof(DevPlan), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipHostMalloc((void**)&sc->h_sum, sizeof(DevSummary), hipHostMallocDefault);
+	for (int i = 0; i < 3 && e == hipSuccess; ++i) e = hipEventCreate(&sc->ev[i]);
+	if (e != hipSuccess) {
+		delete sc;
+		return hip_fail(e, "scanner setup");
+	}
+	*out = sc;
+	return PARS_OK;
+}
+
+int pars_hip_scanner_destroy(pars_scanner* sc) {
+	delete sc;
+	return PARS_OK;
+}
+
+int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
+	if (!sc)
+		return fail(PARS_ERR_ARGUMENT, "null scanner");
+	switch (option) {
+	case PARS_OPT_WANT_CHARS: sc->want_chars = value != 0; return PARS_OK;
+	case PARS_OPT_WANT_BITMAP: sc->want_bitmap = value != 0; return PARS_OK;
+	case PARS_OPT_PATH:
+		if (value < 0 || value > 1)
+			return fail(PARS_ERR_ARGUMENT, "path must be 0 (auto) or 1 (general)");
+		sc->path = (int)value;
+		return PARS_OK;
+	}
+	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
+}
+
+int pars_hip_scan_device(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, int want_pairs,
+						 int run_pairs, pars_stage_times* st, pars_device_result* out) {
+	if (!sc || !out || (n && !d_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if ((uint64_t)n > 0xFFFFFFFFull)
+		return fail(PARS_ERR_ARGUMENT, "documents are limited to 2^32-1 bytes (uint32 positions)");
+	hipStream_t s = (hipStream_t)stream;
+	if (st)
+		*st = pars_stage_times {0, 0, 0, 0};
+	if (n == 0) {
+		std::memset(out, 0, sizeof *out);
+		out->balanced = out->well_formed = 1;
+		return PARS_OK;
+	}
+	const uint64_t n_tiles = (n + kTile - 1) / kTile;
+	if (!sc->br_hint)
+		sc->br_hint = std::max<uint64_t>(n / 8, 1u << 16);
+	uint64_t cap_br = std::min<uint64_t>(std::max<uint64_t>(sc->br_hint, sc->cap_br), n);
+	int		 r;
+	if ((r = ensure_outputs(sc, n_tiles, std::max<uint64_t>(n, sc->cap_struct), cap_br, want_pairs != 0)))
+		return r;
+	if (sc->want_bitmap && (r = sc->bitmap.ensure(((n + 63) / 64) * 8)))
+		return r;
+	for (int attempt = 0; attempt < 2; ++attempt) {
+		if ((r = enqueue_document(sc, d_in, n, s, want_pairs != 0, run_pairs != 0, st != nullptr)))
+			return r;
+		if ((r = read_summary(sc, s)))
+			return r;
+		if (sc->h_sum->n_brackets <= sc->cap_br)
+			break;
+		// more brackets than the optimistic capacity: grow exactly and rerun
+		sc->br_hint = sc->h_sum->n_brackets;
+		if ((r = ensure_outputs(sc, n_tiles, sc->cap_struct, sc->h_sum->n_brackets, want_pairs != 0)))
+			return r;
+	}
+	if (st) {
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
+			st->p1 = ms;
+		if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
+			st->p4 = ms;
+	}
+	fill_result(sc, n, want_pairs != 0, run_pairs != 0, out);
+	return PARS_OK;
+}
+
+int pars_hip_scan_host(pars_scanner* sc, const uint8_t* h_in, size_t n, pars_stream stream, int want_pairs,
+					   pars_host_result* out) {
+	if (!sc || !out || (n && !h_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	hipStream_t s = (hipStream_t)stream;
+	int			r;
+	if ((r = ensure_pinned((void**)&sc->h_in, sc->h_in_cap, n)) || (r = sc->d_in.ensure(std::max<size_t>(n, 64))))
+		return r;
+	if (n) {
+		std::memcpy(sc->h_in, h_in, n);
+		hipError_t e = hipMemcpyAsync(sc->d_in.p, sc->h_in, n, hipMemcpyHostToDevice, s);
+		if (e != hipSuccess)
+			return hip_fail(e, "H2D");
+	}
+	pars_device_result dr;
+	if ((r = pars_hip_scan_device(sc, sc->d_in.as<uint8_t>(), n, stream, want_pairs, 1, nullptr, &dr)))
+		return r;
+	std::memset(out, 0, sizeof *out);
+	out->n_struct	   = dr.n_struct;
+	out->balanced	   = dr.balanced;
+	out->well_formed   = dr.well_formed;
+	out->max_depth	   = dr.max_depth;
+	out->unclosed_mode = dr.unclosed_mode;
+	if ((r = ensure_pinned((void**)&sc->h_struct, sc->h_struct_cap, (size_t)dr.n_struct * 4)))
+		return r;
+	out->structural = sc->h_struct;
+	hipError_t e	= hipSuccess;
+	if (dr.n_struct)
+		e = hipMemcpyAsync(sc->h_struct, dr.structural, (size_t)dr.n_struct * 4, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess && want_pairs) {
+		if ((r = ensure_pinned((void**)&sc->h_pairs, sc->h_pairs_cap, (size_t)dr.n_struct * 4)))
+			return r;
+		out->pair_pos = sc->h_pairs;
+		if (dr.n_struct)
+			e = hipMemcpyAsync(sc->h_pairs, dr.pair_pos, (size_t)dr.n_struct * 4, hipMemcpyDeviceToHost, s);
+	}
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(s);
+	return e == hipSuccess ? PARS_OK : hip_fail(e, "D2H");
+}
+
+int pars_hip_reserve_struct(pars_scanner* sc, size_t cap) {
+	if (!sc)
+		return fail(PARS_ERR_ARGUMENT, "null scanner");
+	const uint64_t c = std::max<uint64_t>(cap, 1);
+	int			   r = ensure_outputs(sc, std::max<uint64_t>(sc->recs.cap / (kRecWords * 8), 1),
+								  std::max<uint64_t>(c, sc->cap_struct), std::max<uint64_t>(c, sc->cap_br), false);
+	if (r)
+		return r;
+	if (sc->want_chars && (r = sc->chars.ensure(sc->cap_struct)))
+		return r;
+	return PARS_OK;
+}
+
+const uint32_t* pars_hip_struct_ptr(const pars_scanner* sc) {
+	return sc ? static_cast<const uint32_t*>(sc->positions.p) : nullptr;
+}
+
+int pars_hip_scan_chunk(pars_scanner* sc, const uint8_t* d_chunk, size_t n, const pars_chunk_carry* in,
+						pars_stream stream, size_t byte_base, pars_chunk_carry* out) {
+	if (!sc || !in || !out || (n && !d_chunk))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	hipStream_t		 s	 = (hipStream_t)stream;
+	pars_chunk_carry c	 = *in;
+	const bool		 pad = (c.scan >> 31) & 1u;
+	if (!pad) {	 // document start
+		c.depth = c.min_depth = c.max_depth = 0;
+	}
+	if (n == 0) {
+		*out = c;
+		return PARS_OK;
+	}
+	const uint64_t n_tiles = (n + kTile - 1) / kTile;
+	const uint64_t need	   = (uint64_t)c.out_base + n;
+	if (need > sc->cap_struct)
+		return fail(PARS_ERR_ARGUMENT, "chunk output past reserve_struct capacity (%llu > %llu)",
+					(unsigned long long)need, (unsigned long long)sc->cap_struct);
+	int r;
+	if ((r = ensure_outputs(sc, n_tiles, sc->cap_struct, sc->cap_br, false)))
+		return r;
+	ScanArgs a	 = make_scan_args(sc, d_chunk, n, false);
+	a.bitmap	 = nullptr;	 // the per-document bitmap is not chunk-addressed
+	a.byte_base	 = byte_base;
+	a.out_base	 = c.out_base;
+	a.br_base	 = c.bracket_base;
+	a.depth_base = c.depth;
+	a.in_states	 = pad ? (uint32_t)(c.scan & 0xFFFFFFu) : sc->hplan.start_packed;
+	SummaryInit si;
+	si.n_struct	  = c.out_base;
+	si.n_brackets = c.bracket_base;
+	si.depth	  = c.depth;
+	si.min_depth  = c.min_depth;
+	si.max_depth  = c.max_depth;
+	hipError_t e  = launch_summary_reset(a.sum, si, s);
+	if (e != hipSuccess)
+		return hip_fail(e, "summary reset");
+	if ((e = launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s)) != hipSuccess)
+		return hip_fail(e, "scan kernel launch");
+	if ((r = read_summary(sc, s)))
+		return r;
+	const DevSummary& h = *sc->h_sum;
+	out->scan			= (1ull << 31) | (uint64_t)(h.final_states & 0xFFFFFFu);
+	out->out_base		= h.n_struct;
+	out->bracket_base	= h.n_brackets;
+	out->depth			= h.last_bal;
+	out->min_depth		= h.min_bal;
+	out->max_depth		= h.max_bal;
+	out->reserved		= 0;
+	return PARS_OK;
+}
+
+int pars_hip_pair_chunks(pars_scanner* sc, const pars_chunk_carry* last, pars_stream stream, int want_pairs,
+						 pars_device_result* out) {
+	if (!sc || !last || !out)
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if (last->bracket_base > sc->cap_br || last->out_base > sc->cap_struct)
+		return fail(PARS_ERR_ARGUMENT, "carry exceeds the reserved capacity");
+	hipStream_t s = (hipStream_t)stream;
+	int			r;
+	if (want_pairs && (r = sc->pair_pos.ensure((size_t)sc->cap_struct * 4)))
+		return r;
+	SummaryInit si;
+	si.n_struct	  = last->out_base;
+	si.n_brackets = last->bracket_base;
+	si.depth	  = last->depth;
+	si.min_depth  = last->min_depth;
+	si.max_depth  = last->max_depth;
+	hipError_t e  = launch_summary_reset(sc->summary.as<DevSummary>(), si, s);
+	if (e != hipSuccess)
+		return hip_fail(e, "summary reset");
+	if (want_pairs && last->out_base) {
+		if ((e = launch_fill_i32(sc->pair_pos.as<int32_t>(), last->out_base, -1, s)) != hipSuccess)
+			return hip_fail(e, "pair_pos fill");
+	}
+	PairArgs p = make_pair_args(sc, want_pairs != 0);
+	if ((e = launch_pairing(p, s)) != hipSuccess)
+		return hip_fail(e, "pairing launch");
+	if ((r = read_summary(sc, s)))
+		return r;
+	fill_result(sc, 1, want_pairs != 0, true, out);
+	out->unclosed_mode = ((last->scan & 0xFFFFFFu) & ((1u << (3 * sc->hplan.n_descs)) - 1u))
+					  != (sc->hplan.start_packed & ((1u << (3 * sc->hplan.n_descs)) - 1u));
+	out->struct_bitmap = nullptr;
+	return PARS_OK;
+}
+
+}  // extern "C"

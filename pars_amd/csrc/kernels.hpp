@@ -1,0 +1,104 @@
+// kernels.hpp — launch interface between the host orchestration and the HIP
+// kernels (all of which live in this directory).
+#pragma once
+
+#include "plan_dev.hpp"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pars_hip {
+
+constexpr int kBlock = 256;			   // threads per scan block (4 waves)
+constexpr int kBPL	 = 64;			   // bytes per lane
+constexpr int kTile	 = kBlock * kBPL;  // bytes per scan tile (16 KiB)
+constexpr int kRecWords = 16;		   // look-back words per tile record
+
+// Record layout (u64 words): [0..7] state-chain aggregates, one per
+// descriptor (3-bit packed map); [8] state-chain inclusive (packed states);
+// [9..11] count aggregates (structural, brackets, depth); [12..14] count
+// inclusives.
+enum : int { kRecSAgg = 0, kRecSIncl = 8, kRecCAgg = 9, kRecCIncl = 12 };
+
+// Device-side result summary of one scan (written by the kernels, read back by
+// the host once at the end of a call).
+struct DevSummary {
+	uint32_t n_struct;	   // total structural count (may exceed the output capacity)
+	uint32_t n_brackets;   // total bracket count
+	int32_t	 last_bal;	   // depth after the last bracket
+	uint32_t final_states; // packed per-descriptor states after the last byte
+	int32_t	 min_bal;	   // min over bracket prefix depths and 0
+	int32_t	 max_bal;	   // max over bracket prefix depths and 0
+	uint32_t mismatched;   // some sorted pair is not a matching open/close
+	uint32_t sort_passes;  // radix passes the pairing sort needs
+	int32_t	 kmin;		   // smallest pairing key
+	uint32_t n_chunks;	   // pairing sort chunks
+	uint32_t overflow;	   // an output exceeded its capacity
+	uint32_t pad[5];
+};
+
+struct ScanArgs {
+	const uint8_t*	data;
+	uint64_t		n;
+	uint32_t		n_tiles;
+	uint32_t		epoch;
+	const DevPlan*	plan;
+	uint64_t*		recs;
+	uint32_t*		ticket;
+	DevSummary*		sum;
+	uint32_t*		positions;
+	uint8_t*		chars;	   // optional
+	uint64_t*		bitmap;	   // optional, ceil(n/64) words
+	int32_t*		pair_pos;  // optional: -1 written for every structural index
+	uint32_t*		bpos;
+	uint32_t*		bsidx;
+	int32_t*		bkey;
+	uint8_t*		bchar;
+	uint64_t		cap_struct;
+	uint64_t		cap_br;
+	uint64_t		byte_base;	 // added to every emitted position (chunked mode)
+	uint32_t		out_base;	 // structural index of the first output (chunked mode)
+	uint32_t		br_base;	 // bracket index of the first bracket (chunked mode)
+	int32_t			depth_base;	 // depth before the first byte (chunked mode)
+	uint32_t		in_states;	 // packed per-descriptor states before the first byte
+};
+
+// R1-R5: one pass over the input (plus summary reset), general descriptor path.
+hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
+// Initial summary values: a fresh document starts all at 0; a chunk of a
+// longer document starts from the running values its carry holds.
+struct SummaryInit {
+	uint32_t n_struct	= 0;
+	uint32_t n_brackets = 0;
+	int32_t	 depth		= 0;
+	int32_t	 min_depth	= 0;
+	int32_t	 max_depth	= 0;
+};
+
+hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStream_t s);
+
+// R6: stable sort of bracket indices by depth key, then adjacent pairing.
+struct PairArgs {
+	DevSummary*	   sum;
+	const DevPlan* plan;
+	const int32_t* bkey;
+	const uint8_t* bchar;
+	const uint32_t* bsidx;
+	uint32_t	   br_base;	  // bsidx values are relative to this structural base
+	uint64_t	   cap_br;
+	uint32_t*	   ukeys[2];  // radix ping-pong keys
+	uint32_t*	   order[2];  // radix ping-pong values (bracket indices)
+	uint32_t*	   hist;	  // 256 * max_chunks
+	uint32_t*	   hist_part; // scan partials
+	uint64_t	   max_chunks;
+	int32_t*	   pair;	  // CPU convention, per bracket index (optional)
+	int32_t*	   pair_pos;  // device convention, per structural index (optional)
+	uint32_t	   out_base;  // structural index base of pair_pos entries
+};
+
+hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
+
+// Fill n int32 values (graph-safe replacement for hipMemsetAsync).
+hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
+
+}  // namespace pars_hip

@@ -1,0 +1,299 @@
+// pair_kernels.hip — R6: depth-sort bracket pairing, sized entirely on device.
+//
+// Restates model::pair_brackets (include/pars/model/nest.hpp:45-92), the CPU
+// semantics the results must equal bit for bit: key = depth-1 for an open,
+// depth for a close; a STABLE sort by key; sorted neighbours (2i, 2i+1) pair
+// when they are a matching open/close.  The reference GPU chain
+// (pipeline.cuh:932-996) clamps keys at 0 and sorts unstably through Thrust,
+// which diverges from the CPU on malformed input; here the full key range is
+// sorted stably, so the two agree on every input.
+//
+// Layout: the scan kernel leaves bkey/bchar/bsidx per bracket.  The radix sort
+// runs 8-bit LSD passes over u = key - kmin; the pass count (1..4) comes from
+// the key range found on device, so no host round trip sizes anything.  Each
+// pass: per-chunk digit histogram (LDS atomics; counts are order-free), one
+// device-wide exclusive scan of the digit-major histogram, then a stable
+// scatter that ranks equal digits in input order with wave ballots.
+#include "dev_common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace pars_hip {
+
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortRows	 = 8;
+constexpr int kChunk	 = kSortBlock * kSortRows;	// brackets per sort chunk
+constexpr int kScanGrid	 = 256;						// blocks of the histogram scan
+
+__device__ __forceinline__ uint32_t n_brackets(const PairArgs& a) {
+	const uint32_t n = a.sum->n_brackets;
+	return (uint64_t)n > a.cap_br ? (uint32_t)a.cap_br : n;
+}
+
+__global__ void k_sort_prep(PairArgs a) {
+	if (threadIdx.x != 0 || blockIdx.x != 0)
+		return;
+	DevSummary*	   s	 = a.sum;
+	const uint32_t nb	 = n_brackets(a);
+	const int32_t  kmin	 = s->min_bal - 1;	// open keys are depth-1, depths >= min_bal
+	const int32_t  kmax	 = s->max_bal;
+	const uint32_t range = (uint32_t)(kmax - kmin);
+	const uint32_t bits	 = range ? 32u - (uint32_t)__clz(range) : 0u;
+	uint32_t	   p	 = (bits + 7u) / 8u;
+	s->kmin				 = kmin;
+	s->sort_passes		 = p < 1u ? 1u : p;
+	s->n_chunks			 = (nb + kChunk - 1) / kChunk;
+	if ((uint64_t)s->n_chunks > a.max_chunks)
+		s->overflow |= 2u;
+}
+
+__device__ __forceinline__ uint32_t key_at(const PairArgs& a, int pass, uint32_t i, int32_t kmin) {
+	return pass == 0 ? (uint32_t)(a.bkey[i] - kmin) : a.ukeys[(pass - 1) & 1][i];
+}
+
+template<int PASS>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(PairArgs a) {
+	if ((uint32_t)PASS >= a.sum->sort_passes)
+		return;
+	__shared__ uint32_t h[256];
+	const uint32_t		nb = n_brackets(a), nc = a.sum->n_chunks;
+	const int32_t		kmin = a.sum->kmin;
+	const int			tid	 = threadIdx.x;
+	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+		h[tid] = 0;
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < kSortRows; ++r) {
+			const uint32_t i = c * kChunk + r * kSortBlock + tid;
+			if (i < nb)
+				atomicAdd(&h[(key_at(a, PASS, i, kmin) >> (8 * PASS)) & 255u], 1u);
+		}
+		__syncthreads();
+		a.hist[(uint64_t)tid * nc + c] = h[tid];
+		__syncthreads();
+	}
+}
+
+// Device-wide exclusive scan of hist[0 .. 256*nc), three phases, fixed grid.
+__device__ __forceinline__ void scan_range(uint32_t len, uint32_t b, uint32_t& lo, uint32_t& hi) {
+	const uint64_t per = ((uint64_t)len + kScanGrid - 1) / kScanGrid;
+	lo				   = (uint32_t)min<uint64_t>((uint64_t)b * per, len);
+	hi				   = (uint32_t)min<uint64_t>((uint64_t)lo + per, len);
+}
+
+__device__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t& total) {
+	const int	   lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const uint32_t inc	= wave_incl_sum(v);
+	if (lane == 63)
+		s_w[wid] = inc;
+	__syncthreads();
+	uint32_t p = 0, t = 0;
+#pragma unroll
+	for (int w = 0; w < kSortBlock / 64; ++w) {
+		if (w < wid)
+			p += s_w[w];
+		t += s_w[w];
+	}
+	__syncthreads();
+	total = t;
+	return p + inc - v;
+}
+
+template<int PASS>
+__global__ __launch_bounds__(kSortBlock) void k_scan_partial(PairArgs a) {
+	if ((uint32_t)PASS >= a.sum->sort_passes)
+		return;
+	__shared__ uint32_t s_w[4];
+	const uint32_t		len = 256u * a.sum->n_chunks;
+	uint32_t			lo, hi;
+	scan_range(len, blockIdx.x, lo, hi);
+	uint32_t acc = 0;
+	for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortBlock) acc += a.hist[i];
+	uint32_t tot;
+	(void)block_excl_sum(acc, s_w, tot);
+	if (threadIdx.x == 0)
+		a.hist_part[blockIdx.x] = tot;
+}
+
+template<int PASS>
+__global__ __launch_bounds__(kSortBlock) void k_scan_top(PairArgs a) {
+	if ((uint32_t)PASS >= a.sum->sort_passes)
+		return;
+	__shared__ uint32_t s_w[4];
+	uint32_t			tot;
+	const uint32_t		v = a.hist_part[threadIdx.x];  // kScanGrid == kSortBlock
+	a.hist_part[threadIdx.x] = block_excl_sum(v, s_w, tot);
+}
+
+template<int PASS>
+__global__ __launch_bounds__(kSortBlock) void k_scan_apply(PairArgs a) {
+	if ((uint32_t)PASS >= a.sum->sort_passes)
+		return;
+	__shared__ uint32_t s_w[4];
+	const uint32_t		len = 256u * a.sum->n_chunks;
+	uint32_t			lo, hi;
+	scan_range(len, blockIdx.x, lo, hi);
+	uint32_t run = a.hist_part[blockIdx.x];
+	for (uint32_t i0 = lo; i0 < hi; i0 += kSortBlock) {
+		const uint32_t i = i0 + threadIdx.x;
+		const uint32_t v = i < hi ? a.hist[i] : 0u;
+		uint32_t	   tot;
+		const uint32_t ex = block_excl_sum(v, s_w, tot);
+		if (i < hi)
+			a.hist[i] = run + ex;
+		run += tot;
+	}
+}
+
+template<int PASS>
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
+	const uint32_t passes = a.sum->sort_passes;
+	if ((uint32_t)PASS >= passes)
+		return;
+	__shared__ uint32_t s_base[256];
+	__shared__ uint32_t s_wc[4][256];
+	__shared__ uint32_t s_wo[4][256];
+	const uint32_t		nb = n_brackets(a), nc = a.sum->n_chunks;
+	const int32_t		kmin = a.sum->kmin;
+	const bool			last = (uint32_t)PASS + 1 == passes;
+	const int			tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	uint32_t*			okey = a.ukeys[PASS & 1];
+	uint32_t*			oval = a.order[PASS & 1];
+	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
+		s_base[tid] = a.hist[(uint64_t)tid * nc + c];
+#pragma unroll
+		for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
+		__syncthreads();
+		for (int r = 0; r < kSortRows; ++r) {
+			const uint32_t i	 = c * kChunk + r * kSortBlock + tid;
+			const bool	   valid = i < nb;
+			uint32_t	   u = 0, v = 0, d = 0;
+			if (valid) {
+				u = key_at(a, PASS, i, kmin);
+				v = PASS == 0 ? i : a.order[(PASS - 1) & 1][i];
+				d = (u >> (8 * PASS)) & 255u;
+			}
+			// lanes holding the same digit, by ballots over the digit's bits
+			uint64_t peers = __ballot(valid);
+#pragma unroll
+			for (int b = 0; b < 8; ++b) {
+				const bool	   bit = (d >> b) & 1u;
+				const uint64_t bb  = __ballot(valid && bit);
+				peers &= bit ? bb : ~bb;
+			}
+			const uint32_t wrank = (uint32_t)__popcll(peers & lanemask_lt());
+			if (valid && wrank == 0)
+				s_wc[wid][d] = (uint32_t)__popcll(peers);
+			__syncthreads();
+			{
+				uint32_t run = s_base[tid];
+#pragma unroll
+				for (int w = 0; w < 4; ++w) {
+					const uint32_t t = s_wc[w][tid];
+					s_wo[w][tid]	 = run;
+					run += t;
+					s_wc[w][tid] = 0;
+				}
+				s_base[tid] = run;
+			}
+			__syncthreads();
+			if (valid) {
+				const uint32_t dst = s_wo[wid][d] + wrank;
+				if (!last)
+					okey[dst] = u;
+				oval[dst] = v;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// nest.hpp:78-90: sorted neighbours (2i, 2i+1) pair if open/close and matching.
+__global__ __launch_bounds__(256) void k_pair(PairArgs a) {
+	const uint32_t	nb	  = n_brackets(a);
+	const uint32_t	pass  = a.sum->sort_passes;
+	const uint32_t* order = a.order[(pass - 1) & 1];
+	const DevPlan*	P	  = a.plan;
+	bool			bad	  = false;
+	for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; 2ull * g < nb; g += gridDim.x * blockDim.x) {
+		const uint32_t s  = 2u * g;
+		const uint32_t ia = order[s];
+		if (s + 1 < nb) {
+			const uint32_t ib = order[s + 1];
+			const uint8_t  ca = a.bchar[ia], cb = a.bchar[ib];
+			const bool ok = (P->br_flags[ca] & 1u) && (P->br_flags[cb] & 2u) && P->br_close_of[ca] == cb;
+			if (a.pair) {
+				a.pair[ia] = ok ? (int32_t)ib : -1;
+				a.pair[ib] = ok ? (int32_t)ia : -1;
+			}
+			if (a.pair_pos) {
+				const uint32_t sa = a.bsidx[ia], sb = a.bsidx[ib];
+				a.pair_pos[sa]	  = ok ? (int32_t)sb : -1;
+				a.pair_pos[sb]	  = ok ? (int32_t)sa : -1;
+			}
+			bad |= !ok;
+		} else {
+			if (a.pair)
+				a.pair[ia] = -1;
+			if (a.pair_pos)
+				a.pair_pos[a.bsidx[ia]] = -1;
+		}
+	}
+	if (__ballot(bad) && lane_id() == 0)
+		atomicOr(&a.sum->mismatched, 1u);
+}
+
+template<int PASS>
+hipError_t launch_pass(const PairArgs& a, unsigned grid, hipStream_t s) {
+	hipLaunchKernelGGL((k_sort_hist<PASS>), dim3(grid), dim3(kSortBlock), 0, s, a);
+	hipLaunchKernelGGL((k_scan_partial<PASS>), dim3(kScanGrid), dim3(kSortBlock), 0, s, a);
+	hipLaunchKernelGGL((k_scan_top<PASS>), dim3(1), dim3(kSortBlock), 0, s, a);
+	hipLaunchKernelGGL((k_scan_apply<PASS>), dim3(kScanGrid), dim3(kSortBlock), 0, s, a);
+	hipLaunchKernelGGL((k_sort_scatter<PASS>), dim3(grid), dim3(kSortBlock), 0, s, a);
+	return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
+	if (a.cap_br == 0)
+		return hipSuccess;
+	const uint64_t chunks = (a.cap_br + kChunk - 1) / kChunk;
+	const unsigned grid	  = (unsigned)(chunks < 2048 ? (chunks ? chunks : 1) : 2048);
+	hipLaunchKernelGGL(k_sort_prep, dim3(1), dim3(64), 0, s, a);
+	hipError_t e;
+	if ((e = launch_pass<0>(a, grid, s)) != hipSuccess)
+		return e;
+	if ((e = launch_pass<1>(a, grid, s)) != hipSuccess)
+		return e;
+	if ((e = launch_pass<2>(a, grid, s)) != hipSuccess)
+		return e;
+	if ((e = launch_pass<3>(a, grid, s)) != hipSuccess)
+		return e;
+	const uint64_t pairs = (a.cap_br + 1) / 2;
+	const unsigned pgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((pairs + 255) / 256, 1), 4096);
+	hipLaunchKernelGGL(k_pair, dim3(pgrid), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
+	for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
+hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t s) {
+	if (!n)
+		return hipSuccess;
+	const uint64_t g = std::min<uint64_t>((n + 255) / 256, 4096);
+	hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+	return hipGetLastError();
+}
+
+uint64_t pairing_max_chunks(uint64_t cap_br) {
+	return (cap_br + kChunk - 1) / kChunk;
+}
+
+}  // namespace pars_hip

@@ -1,0 +1,135 @@
+// plan_dev.cpp — lowering of a pars_plan into DevPlan tables (host side).
+#include "plan_dev.hpp"
+
+#include <string.h>
+
+namespace pars_hip {
+
+namespace {
+
+// classify.hpp:39-55 restricted to the bits R3/R5/R6 consume.
+void class_bits(const pars_plan& S, uint8_t st[256], uint8_t oc[256], uint8_t op[256]) {
+	memset(st, 0, 256);
+	memset(oc, 0, 256);
+	memset(op, 0, 256);
+	for (unsigned i = 0; i < S.n_punctuation; ++i) st[(uint8_t)S.punctuation[i]] = 1;
+	for (unsigned i = 0; i < S.n_brackets; ++i) {
+		const uint8_t o = (uint8_t)S.brackets[i].open, c = (uint8_t)S.brackets[i].close;
+		st[o] = st[c] = 1;
+		oc[o] = oc[c] = 1;
+		op[o]		  = 1;  // nest.hpp:26-31: a byte that opens any bracket counts as open
+	}
+}
+
+// One affine descriptor with a prefix escape, exactly the four-state automaton
+// compile/plan.hpp:495-558 yields for `"` (\\ any | [^"\\])* `"`:
+//   out  --delim--> in,  in --delim--> closed, in --escape--> esc, esc --any--> in,
+//   closed behaves as out; masked exactly in `in` and `esc`.
+bool is_escape_desc(const pars_lex_desc& d, uint8_t& dl, uint8_t& es, uint8_t st[4]) {
+	if (d.n_states != 4 || d.n_classes != 3 || d.kind != PARS_LEX_AFFINE || !d.delim || !d.escape)
+		return false;
+	const uint8_t q = d.rep_of[(uint8_t)d.delim], e = d.rep_of[(uint8_t)d.escape];
+	if (q == e)
+		return false;
+	uint8_t o = 0xFF;
+	for (int b = 0; b < 256; ++b) {
+		const uint8_t c = d.rep_of[b];
+		if (b == (uint8_t)d.delim || b == (uint8_t)d.escape) {
+			if (c != (b == (uint8_t)d.delim ? q : e))
+				return false;
+			continue;
+		}
+		if (o == 0xFF)
+			o = c;
+		if (c != o)
+			return false;
+	}
+	if (o == 0xFF || o == q || o == e)
+		return false;
+	const uint8_t s_out = d.start, s_in = d.trans[s_out][q];
+	const uint8_t s_cl = d.trans[s_in][q], s_esc = d.trans[s_in][e];
+	auto row = [&](uint8_t s, uint8_t to, uint8_t tq, uint8_t te, uint8_t m) {
+		return d.trans[s][o] == to && d.trans[s][q] == tq && d.trans[s][e] == te && d.masked[s][o] == m
+			&& d.masked[s][q] == m && d.masked[s][e] == m;
+	};
+	if (!(row(s_out, s_out, s_in, s_out, 0) && row(s_in, s_in, s_cl, s_esc, 1)
+		  && row(s_cl, s_out, s_in, s_out, 0) && row(s_esc, s_in, s_in, s_in, 1)))
+		return false;
+	if (s_out == s_in || s_out == s_cl || s_out == s_esc || s_in == s_cl || s_in == s_esc || s_cl == s_esc)
+		return false;
+	dl	  = (uint8_t)d.delim;
+	es	  = (uint8_t)d.escape;
+	st[0] = s_out;
+	st[1] = s_in;
+	st[2] = s_cl;
+	st[3] = s_esc;
+	return true;
+}
+
+}  // namespace
+
+int lower_plan(const pars_plan& S, DevPlan& P) {
+	memset(&P, 0, sizeof(P));
+	if (S.n_descs > kMaxDescs || S.n_punctuation > PARS_MAX_PUNCT || S.n_brackets > PARS_MAX_BRACKETS)
+		return -1;
+	for (unsigned d = 0; d < S.n_descs; ++d) {
+		const pars_lex_desc& D = S.descs[d];
+		if (D.n_states < 1 || D.n_states > PARS_MAX_DFA_STATES || D.n_classes < 1
+			|| D.n_classes > PARS_MAX_DFA_CLASSES || D.start >= D.n_states)
+			return -2;
+		for (int b = 0; b < 256; ++b)
+			if (D.rep_of[b] >= D.n_classes)
+				return -3;
+		for (int s = 0; s < D.n_states; ++s)
+			for (int c = 0; c < D.n_classes; ++c)
+				if (D.trans[s][c] >= D.n_states || D.masked[s][c] > 1)
+					return -4;
+	}
+
+	uint8_t st[256], oc[256], op[256];
+	class_bits(S, st, oc, op);
+	for (int j = 0; j < 8; ++j)
+		for (int c = 0; c < 256; ++c)
+			P.cls[j][c] = ((uint32_t)st[c] << j) | ((uint32_t)oc[c] << (8 + j)) | ((uint32_t)op[c] << (16 + j));
+
+	P.n_descs = S.n_descs;
+	for (unsigned d = 0; d < S.n_descs; ++d) {
+		const pars_lex_desc& D = S.descs[d];
+		P.start[d]			   = D.start;
+		P.n_states[d]		   = D.n_states;
+		P.start_packed |= (uint32_t)D.start << (3 * d);
+		if (D.n_states > 4)
+			P.wide = 1;
+		for (int c = 0; c < 256; ++c) {
+			const uint8_t cl = D.rep_of[c];
+			uint8_t		  nx[8], mk[8];
+			for (int s = 0; s < 8; ++s) {
+				nx[s] = s < D.n_states ? D.trans[s][cl] : (uint8_t)s;
+				mk[s] = s < D.n_states ? D.masked[s][cl] : 0;
+			}
+			auto pack4 = [](const uint8_t* v) {
+				return (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+			};
+			P.dt[d][c][0] = pack4(nx);
+			P.dt[d][c][1] = pack4(nx + 4);
+			P.dt[d][c][2] = pack4(mk);
+			P.dt[d][c][3] = pack4(mk + 4);
+		}
+	}
+
+	// nest.hpp:26-48: is_open_char / is_close_char / match_brackets
+	for (unsigned i = 0; i < S.n_brackets; ++i) {
+		const uint8_t o = (uint8_t)S.brackets[i].open, c = (uint8_t)S.brackets[i].close;
+		if (!(P.br_flags[o] & 1))
+			P.br_close_of[o] = c;  // the first spec opening with o decides its partner
+		P.br_flags[o] |= 1;
+		P.br_flags[c] |= 2;
+	}
+
+	P.fast_kind = kFastNone;
+	if (S.n_descs == 1 && is_escape_desc(S.descs[0], P.fk_delim, P.fk_escape, P.fk_state))
+		P.fast_kind = kFastEscape;
+	return 0;
+}
+
+}  // namespace pars_hip

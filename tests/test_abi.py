@@ -1,0 +1,69 @@
+"""CPU tests of the C-ABI boundary: the library loads and exports every entry
+point include/pars_hip.h declares; plans are the reference's 3177-byte layout."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from pars_amd import Plan, _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols() -> list[str]:
+    names = set()
+    for h in ("pars_hip.h",):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"\b(pars_hip_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_the_boundary():
+    names = declared_symbols()
+    for required in ("pars_hip_scanner_create", "pars_hip_scan_device", "pars_hip_scan_host",
+                     "pars_hip_reserve_struct", "pars_hip_scan_chunk", "pars_hip_struct_ptr"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(N.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_python_binds_every_declared_symbol():
+    assert set(declared_symbols()) <= set(N.PROTOTYPES)
+
+
+def test_version_string():
+    assert b"gfx950" in N.lib().pars_hip_version()
+
+
+@pytest.mark.parametrize("g", ["json", "csv", "clike", "clike6", "toml", "ini", "xml"])
+def test_plan_blobs_have_scan_plan_layout(g):
+    p = Plan.load(g)
+    assert len(p.blob) == 3177
+    assert 1 <= p.n_descs <= 8
+    for d in range(p.n_descs):
+        desc = p.desc(d)
+        assert 1 <= desc["n_states"] <= 8 and 1 <= desc["n_classes"] <= 8
+
+
+def test_plan_shapes_match_reference_static_asserts():
+    # tests/test_model.cpp:32-40, 52-53, 74-75
+    j, c, l = Plan.load("json"), Plan.load("csv"), Plan.load("clike")
+    assert len(j.brackets) == 2 and len(j.punctuation) == 2 and j.desc(0)["kind"] == 0
+    assert len(c.brackets) == 0 and len(c.punctuation) == 2 and c.desc(0)["kind"] == 2
+    assert len(l.brackets) == 1 and len(l.punctuation) == 1 and any(l.desc(d)["kind"] == 3 for d in range(l.n_descs))
+    six = Plan.load("clike6")
+    assert six.n_descs == 5 and sum(six.desc(d)["n_states"] for d in range(5)) == 18  # > the reference's 16
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(N, "_lib", None)
+    monkeypatch.setattr(N, "LIB_PATH", str(tmp_path / "absent.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        N.lib()

@@ -1,0 +1,171 @@
+// pars_hip.hpp — C++ facade over the C ABI that restores the reference
+// backend's template API (include/pars/backend/cuda/pipeline.cuh:1164-1281):
+//
+//   pars::hip::scanner<S>      scan_device / scan_host / reserve_struct /
+//                              scan_chunk / struct_ptr
+//   pars::hip::scan_device<S>, scan_host<S>, scan_chunk<S>
+//                              one scanner per plan per thread (:1260-1281)
+//
+// S is the caller's compiled plan (a `pars::scan_plan` constant from
+// compile::make_scan_plan<G>(), or a `pars_plan`); its bytes are handed over
+// as-is, which is why pars_plan keeps scan_plan's exact layout.
+// `hipStream_t` replaces `cudaStream_t`; errors throw pars::hip::error (the
+// reference prints and calls exit(1), pipeline.cuh:520-555).
+#pragma once
+
+#include "pars_hip.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace pars::hip {
+
+class error : public std::runtime_error {
+public:
+	error(int status, const char* what) : std::runtime_error(std::string(what)), status_(status) {}
+
+	int status() const noexcept { return status_; }
+
+private:
+	int status_;
+};
+
+inline void check(int status) {
+	if (status != PARS_OK)
+		throw error(status, pars_hip_last_error());
+}
+
+// pipeline.cuh:722-728 (positions are handed out as int32 like the reference;
+// the values are identical below 2^31, the uint32 view is `positions`).
+struct device_result {
+	const std::int32_t*	 structural = nullptr;
+	const std::int32_t*	 pair_pos	= nullptr;
+	std::uint32_t		 n_struct	= 0;
+	bool				 balanced = true, well_formed = true;
+	int					 max_depth	= 0;
+	// extensions
+	const std::uint32_t* positions	   = nullptr;
+	const std::uint8_t*	 chars		   = nullptr;
+	const std::uint64_t* struct_bitmap = nullptr;
+	const std::uint32_t* bracket_pos   = nullptr;
+	const std::uint8_t*	 bracket_char  = nullptr;
+	const std::int32_t*	 bracket_pair  = nullptr;
+	std::uint32_t		 n_brackets	   = 0;
+	bool				 unclosed_mode = false;
+};
+
+// pipeline.cuh:730-735
+struct host_result {
+	std::vector<std::int32_t> structural;
+	std::vector<std::int32_t> pair_pos;
+	bool					  balanced = true, well_formed = true;
+	int						  max_depth		= 0;
+	bool					  unclosed_mode = false;
+};
+
+// pipeline.cuh:737-743 (zero-initialised = document start, as in the reference)
+using chunk_carry = pars_chunk_carry;
+
+// pipeline.cuh:745-748
+using stage_times = pars_stage_times;
+
+// A scanner for a plan known at run time.
+class runtime_scanner {
+public:
+	explicit runtime_scanner(const pars_plan& plan) { check(pars_hip_scanner_create(&plan, &h_)); }
+
+	runtime_scanner(const runtime_scanner&)			   = delete;
+	runtime_scanner& operator=(const runtime_scanner&) = delete;
+
+	~runtime_scanner() { pars_hip_scanner_destroy(h_); }
+
+	device_result scan_device(const std::uint8_t* d_data, std::size_t n, hipStream_t s = nullptr,
+							  bool want_pairs = false, bool run_pairs = true, stage_times* st = nullptr) const {
+		pars_device_result r {};
+		check(pars_hip_scan_device(h_, d_data, n, s, want_pairs, run_pairs, st, &r));
+		device_result o;
+		o.structural	= reinterpret_cast<const std::int32_t*>(r.structural);
+		o.pair_pos		= r.pair_pos;
+		o.n_struct		= r.n_struct;
+		o.balanced		= r.balanced != 0;
+		o.well_formed	= r.well_formed != 0;
+		o.max_depth		= r.max_depth;
+		o.positions		= r.structural;
+		o.chars			= r.chars;
+		o.struct_bitmap = r.struct_bitmap;
+		o.bracket_pos	= r.bracket_pos;
+		o.bracket_char	= r.bracket_char;
+		o.bracket_pair	= r.bracket_pair;
+		o.n_brackets	= r.n_brackets;
+		o.unclosed_mode = r.unclosed_mode != 0;
+		return o;
+	}
+
+	host_result scan_host(const std::uint8_t* h_data, std::size_t n, hipStream_t s = nullptr,
+						  bool want_pairs = false) const {
+		pars_host_result r {};
+		check(pars_hip_scan_host(h_, h_data, n, s, want_pairs, &r));
+		host_result o;
+		o.structural.assign(reinterpret_cast<const std::int32_t*>(r.structural),
+							reinterpret_cast<const std::int32_t*>(r.structural) + r.n_struct);
+		if (want_pairs && r.pair_pos)
+			o.pair_pos.assign(r.pair_pos, r.pair_pos + r.n_struct);
+		o.balanced		= r.balanced != 0;
+		o.well_formed	= r.well_formed != 0;
+		o.max_depth		= r.max_depth;
+		o.unclosed_mode = r.unclosed_mode != 0;
+		return o;
+	}
+
+	void reserve_struct(std::size_t cap) const { check(pars_hip_reserve_struct(h_, cap)); }
+
+	chunk_carry scan_chunk(const std::uint8_t* d_chunk, std::size_t n, chunk_carry in, hipStream_t s = nullptr,
+						   std::size_t byte_base = 0) const {
+		chunk_carry out {};
+		check(pars_hip_scan_chunk(h_, d_chunk, n, &in, s, byte_base, &out));
+		return out;
+	}
+
+	const std::int32_t* struct_ptr() const { return reinterpret_cast<const std::int32_t*>(pars_hip_struct_ptr(h_)); }
+
+	pars_scanner* handle() const noexcept { return h_; }
+
+private:
+	pars_scanner* h_ = nullptr;
+};
+
+// gpu::scanner<S> for a compile-time plan S (a pars::scan_plan or pars_plan).
+template<auto S>
+class scanner : public runtime_scanner {
+	static_assert(sizeof(S) == PARS_PLAN_BYTES, "S must be a scan_plan (3177 bytes)");
+
+public:
+	scanner() : runtime_scanner(*reinterpret_cast<const pars_plan*>(&S)) {}
+};
+
+template<auto S>
+device_result scan_device(const std::uint8_t* d_data, std::size_t n, hipStream_t s = nullptr,
+						  bool want_pairs = false) {
+	static thread_local scanner<S> sc;
+	return sc.scan_device(d_data, n, s, want_pairs);
+}
+
+template<auto S>
+host_result scan_host(const std::uint8_t* h_data, std::size_t n, bool want_pairs = false) {
+	static thread_local scanner<S> sc;
+	return sc.scan_host(h_data, n, nullptr, want_pairs);
+}
+
+template<auto S>
+chunk_carry scan_chunk(const std::uint8_t* d_chunk, std::size_t n, chunk_carry in, hipStream_t s = nullptr,
+					   std::size_t byte_base = 0) {
+	static thread_local scanner<S> sc;
+	return sc.scan_chunk(d_chunk, n, in, s, byte_base);
+}
+
+}  // namespace pars::hip

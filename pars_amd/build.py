@@ -80,10 +80,22 @@ def build_oracle() -> None:
         _run([make, "-s", "-C", os.path.join(ROOT, "oracle"), "ref"])
 
 
+def build_facade_check() -> str:
+    """tests/cpp/facade_check: the C++ facade (include/pars_hip.hpp) driven like
+    the reference's gpu::scanner<S>; linked against the in-tree library."""
+    src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "facade_check")
+    _run(["g++", "-std=c++20", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lpars_hip",
+          "-Wl,-rpath,$ORIGIN/../../pars_amd", "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_oracle()
     build_hip()
+    build_facade_check()
 
 
 if __name__ == "__main__":

@@ -252,6 +252,13 @@ void fill_result(pars_scanner* sc, uint64_t n, bool want_pairs, bool run_pairs, 
 		out->well_formed = out->balanced && (h.n_brackets % 2 == 0);
 }
 
+// R1-R5 through the fastest evaluator that is exact for the plan.
+hipError_t launch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
+	if (sc->path == 0 && sc->hplan.fast_kind == kFastEscape)
+		return launch_scan_escape(a, s);
+	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s);
+}
+
 // One full document: reset, R1-R5, optional R6; no host sync inside.
 int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStream_t s, bool want_pairs,
 					 bool run_pairs, bool timed) {
@@ -261,7 +268,7 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 	if (timed && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
 	ScanArgs a = make_scan_args(sc, d_in, n, want_pairs);
-	if ((e = launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s)) != hipSuccess)
+	if ((e = launch_scan(sc, a, s)) != hipSuccess)
 		return hip_fail(e, "scan kernel launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
@@ -480,7 +487,7 @@ int pars_hip_scan_chunk(pars_scanner* sc, const uint8_t* d_chunk, size_t n, cons
 	hipError_t e  = launch_summary_reset(a.sum, si, s);
 	if (e != hipSuccess)
 		return hip_fail(e, "summary reset");
-	if ((e = launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s)) != hipSuccess)
+	if ((e = launch_scan(sc, a, s)) != hipSuccess)
 		return hip_fail(e, "scan kernel launch");
 	if ((r = read_summary(sc, s)))
 		return r;

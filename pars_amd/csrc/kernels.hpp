@@ -65,6 +65,8 @@ struct ScanArgs {
 
 // R1-R5: one pass over the input (plus summary reset), general descriptor path.
 hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
+// R1-R5 for plans whose one descriptor is an affine escape automaton (JSON).
+hipError_t launch_scan_escape(const ScanArgs& a, hipStream_t s);
 // Initial summary values: a fresh document starts all at 0; a chunk of a
 // longer document starts from the running values its carry holds.
 struct SummaryInit {

@@ -129,6 +129,13 @@ int lower_plan(const pars_plan& S, DevPlan& P) {
 	P.fast_kind = kFastNone;
 	if (S.n_descs == 1 && is_escape_desc(S.descs[0], P.fk_delim, P.fk_escape, P.fk_state))
 		P.fast_kind = kFastEscape;
+	for (int j = 0; j < 8; ++j)
+		for (int c = 0; c < 256; ++c) {
+			const uint32_t q = P.fast_kind == kFastEscape && c == P.fk_delim;
+			const uint32_t e = P.fast_kind == kFastEscape && c == P.fk_escape;
+			P.cls8[j][c][0]	 = P.cls[j][c] | (q << (24 + j));
+			P.cls8[j][c][1]	 = e << j;
+		}
 	return 0;
 }
 

@@ -21,6 +21,9 @@ constexpr int kMaxDescs = 8;
 // open|close, byte 2 = open.  cls[j][c] = (S(c) << j) | (OC(c) << 8+j) | (O(c) << 16+j).
 struct DevPlan {
 	uint32_t cls[8][256];
+	// The same with two more class bytes for the escape fast path: entry [0] =
+	// bytes {structural, open|close, open, delimiter}, [1] = byte {escape}.
+	uint32_t cls8[8][256][2];
 	// Per descriptor d and byte c, the step taken on c, as maps over the 8 state
 	// slots: {next-state map lo (states 0-3), hi (4-7), masked-bit map lo, hi}.
 	// Unused slots map to themselves and never mask.

@@ -1,194 +1,17 @@
-// scan_kernels.hip — R1-R5 in one pass over the input, for any plan.
+// scan_kernels.hip — lane evaluators plugged into the tile kernel (scan_core.hpp).
 //
-// Replaces the reference's classify_kernel, desc_wordt64_kernel,
-// thrust::inclusive_scan(TCompose64), desc_replay64_kernel,
-// structural_out_kernel, popcount/exclusive_scan x2, pack_counts_kernel and
-// scatter_kernel (include/pars/backend/cuda/pipeline.cuh:203-265, 292-317,
-// 394-443, 503-511, 784-930) and their two blocking host syncs.
-//
-// Per 16 KiB tile (256 lanes x 64 contiguous bytes):
-//   1. each lane walks its 64 bytes once: class bits from pre-shifted LDS
-//      tables, and for every descriptor the composed state map (v_perm) plus,
-//      for every possible lane-entry state, the bit "byte masked" — so no
-//      second walk is needed once the entry state is known;
-//   2. block scan of the lane maps; decoupled look-back over tiles on tagged
-//      64-bit words gives the tile-entry states (chain 1);
-//   3. masks -> unmasked structural / bracket bits; counts and the +-1 depth
-//      delta; block scan + a second look-back (chain 2) give output offsets;
-//   4. scatter of uint32 positions (and chars, bitmap, bracket keys for R6).
-#include "dev_common.hpp"
-#include "kernels.hpp"
-
-#include <limits.h>
-#include <type_traits>
+//   GeneralEval<ND, WIDE>  any plan: every descriptor stepped byte by byte as
+//                          a byte-map composition (v_perm), with the masked bit
+//                          kept for every possible lane-entry state.
+//   EscapeEval             one affine descriptor with a prefix escape (JSON
+//                          strings): bit-parallel over the lane's 64 bytes, exact
+//                          for every input including a backslash outside a string
+//                          (which does NOT escape there, model/scan.hpp semantics).
+#include "scan_core.hpp"
 
 namespace pars_hip {
 
 namespace {
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, kWave);
-	return v;
-}
-
-// Chain 1 look-back, run by one full wave of tile `tile` > 0.  Returns the
-// packed per-descriptor states entering the tile.
-// A look-back waits only on lower tiles, which were all dispatched earlier and
-// publish before they wait, so it always ends; the spin bound is a guard so a
-// defect can never hang the device: past it the kernel flags the summary and
-// returns.
-constexpr uint32_t kSpinLimit = 1u << 21;
-constexpr uint32_t kErrLookback = 0x80000000u;
-
-template<int ND, bool WIDE>
-__device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_t epoch, uint32_t in_states,
-									DevSummary* sum) {
-	using M	  = MapT<WIDE>;
-	using Map = typename M::type;
-	const int lane = lane_id();
-	Map		  acc[ND];
-#pragma unroll
-	for (int d = 0; d < ND; ++d) acc[d] = M::id();
-	int64_t	 jb	   = (int64_t)tile - 1;
-	uint32_t spins = 0;
-	while (true) {
-		const int64_t j		 = jb - lane;
-		uint32_t	  status = 0, ival = 0;
-		Map			  agg[ND];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) agg[d] = M::id();
-		if (j < 0) {
-			status = kIncl;
-			ival   = in_states;
-		} else {
-			const uint64_t* r = recs + (uint64_t)j * kRecWords;
-			const uint64_t	w = peek(r + kRecSIncl);
-			if (word_is(w, epoch, kIncl)) {
-				status = kIncl;
-				ival   = (uint32_t)w;
-			} else {
-				status = kAgg;
-#pragma unroll
-				for (int d = 0; d < ND; ++d) {
-					const uint64_t wa = peek(r + kRecSAgg + d);
-					if (!word_is(wa, epoch, kAgg))
-						status = 0;
-					else
-						agg[d] = M::unpack((uint32_t)wa);
-				}
-			}
-		}
-		const uint64_t b_incl = __ballot(status == kIncl);
-		const uint64_t b_none = __ballot(status == 0);
-		const int	   k	  = b_incl ? (__ffsll((long long)b_incl) - 1) : 64;
-		const uint64_t need	  = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-		if (b_none & need) {
-			if (++spins > kSpinLimit) {
-				if (lane == 0)
-					atomicOr(&sum->overflow, kErrLookback);
-				return in_states;
-			}
-			__builtin_amdgcn_s_sleep(1);
-			continue;
-		}
-		// window map: agg_0 . agg_1 . ... . agg_{k-1} (lane 0 = newest tile, outermost)
-		Map win[ND];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			Map m = lane < k ? agg[d] : M::id();
-#pragma unroll
-			for (int off = 1; off < kWave; off <<= 1) {
-				Map o = M::shfl_up(m, off);
-				if (lane >= off)
-					m = M::compose(o, m);
-			}
-			win[d] = k == 0 ? M::id() : M::shfl(m, k - 1);
-		}
-		if (k < 64) {
-			const uint32_t S   = (uint32_t)__shfl((int)ival, k, kWave);
-			uint32_t	   out = 0;
-#pragma unroll
-			for (int d = 0; d < ND; ++d) {
-				uint32_t sd = (S >> (3 * d)) & 7u;
-				sd			= M::apply(win[d], sd);
-				sd			= M::apply(acc[d], sd);
-				out |= sd << (3 * d);
-			}
-			return out;
-		}
-#pragma unroll
-		for (int d = 0; d < ND; ++d) acc[d] = M::compose(acc[d], win[d]);
-		jb -= 64;
-	}
-}
-
-// Chain 2 look-back (counts), run by one full wave of tile `tile` > 0.
-__device__ void lookback_counts(
-	const uint64_t* recs, uint32_t tile, uint32_t epoch, const uint32_t base[3], uint32_t out[3], DevSummary* sum
-) {
-	const int lane	 = lane_id();
-	uint32_t  acc[3] = {0u, 0u, 0u};
-	int64_t	  jb	 = (int64_t)tile - 1;
-	uint32_t  spins	 = 0;
-	while (true) {
-		const int64_t j		 = jb - lane;
-		uint32_t	  status = 0, v[3] = {0u, 0u, 0u};
-		if (j < 0) {
-			status = kIncl;
-			v[0]   = base[0];
-			v[1]   = base[1];
-			v[2]   = base[2];
-		} else {
-			const uint64_t* r	= recs + (uint64_t)j * kRecWords;
-			bool			inc = true;
-#pragma unroll
-			for (int f = 0; f < 3; ++f) {
-				const uint64_t w = peek(r + kRecCIncl + f);
-				if (!word_is(w, epoch, kIncl))
-					inc = false;
-				v[f] = (uint32_t)w;
-			}
-			if (inc) {
-				status = kIncl;
-			} else {
-				status = kAgg;
-#pragma unroll
-				for (int f = 0; f < 3; ++f) {
-					const uint64_t w = peek(r + kRecCAgg + f);
-					if (!word_is(w, epoch, kAgg))
-						status = 0;
-					v[f] = (uint32_t)w;
-				}
-			}
-		}
-		const uint64_t b_incl = __ballot(status == kIncl);
-		const uint64_t b_none = __ballot(status == 0);
-		const int	   k	  = b_incl ? (__ffsll((long long)b_incl) - 1) : 64;
-		const uint64_t need	  = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-		if (b_none & need) {
-			if (++spins > kSpinLimit) {
-				if (lane == 0)
-					atomicOr(&sum->overflow, kErrLookback);
-				out[0] = base[0];
-				out[1] = base[1];
-				out[2] = base[2];
-				return;
-			}
-			__builtin_amdgcn_s_sleep(1);
-			continue;
-		}
-#pragma unroll
-		for (int f = 0; f < 3; ++f) acc[f] += wave_sum_u32(lane <= k ? v[f] : 0u);
-		if (k < 64) {
-			out[0] = acc[0];
-			out[1] = acc[1];
-			out[2] = acc[2];
-			return;
-		}
-		jb -= 64;
-	}
-}
 
 template<bool WIDE>
 struct DTEntry;
@@ -203,304 +26,266 @@ struct DTEntry<true> {
 	using type = uint4;	 // {next lo, next hi, masked lo, masked hi}
 };
 
+// Class bytes of eight accumulated groups -> three 64-bit masks
+// (byte 0 structural, 1 open|close, 2 open).
+__device__ __forceinline__ void gather_classes(const uint32_t (&c)[8], uint64_t& S, uint64_t& OC, uint64_t& OP,
+											   uint64_t& B3) {
+	uint32_t s0, o0, p0, q0, s1, o1, p1, q1;
+	transpose4(c[0], c[1], c[2], c[3], s0, o0, p0, q0);
+	transpose4(c[4], c[5], c[6], c[7], s1, o1, p1, q1);
+	S  = ((uint64_t)s1 << 32) | s0;
+	OC = ((uint64_t)o1 << 32) | o0;
+	OP = ((uint64_t)p1 << 32) | p0;
+	B3 = ((uint64_t)q1 << 32) | q0;
+}
+
+// ---------------------------------------------------------------------------
 template<int ND, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_scan_general(ScanArgs a) {
-	using M	  = MapT<WIDE>;
-	using Map = typename M::type;
-	using DT  = typename DTEntry<WIDE>::type;
-	constexpr int NDA = ND > 0 ? ND : 1;
+struct GeneralEval {
+	static constexpr int  kND	= ND;
+	static constexpr bool kWide = WIDE;
+	static constexpr int  NDA	= ND > 0 ? ND : 1;
+	using M						= MapT<WIDE>;
+	using Map					= typename M::type;
+	using DT					= typename DTEntry<WIDE>::type;
 
-	__shared__ uint32_t s_cls[8 * 256];
-	__shared__ DT		s_dt[NDA * 256];
-	__shared__ Map		s_wagg[4][NDA];
-	__shared__ uint32_t s_misc[8];
-	__shared__ uint32_t s_wcnt[4][3];
-	__shared__ int32_t	s_mm[4][2];
+	struct Lds {
+		uint32_t cls[8 * 256];
+		DT		 dt[NDA * 256];
+	};
 
-	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-	if (tid == 0) {
-		const uint32_t t = atomicAdd(a.ticket, 1u);
-		if (t == a.n_tiles - 1)
-			atomicExch(a.ticket, 0u);  // every block of this launch has drawn its ticket
-		s_misc[0] = t;
-	}
-	const DevPlan* P = a.plan;
-	for (int i = tid; i < 8 * 256; i += kBlock) s_cls[i] = (&P->cls[0][0])[i];
-	for (int i = tid; i < ND * 256; i += kBlock) {
-		const uint32_t* e = P->dt[i >> 8][i & 255];
-		if constexpr (WIDE)
-			s_dt[i] = make_uint4(e[0], e[1], e[2], e[3]);
-		else
-			s_dt[i] = make_uint2(e[0], e[2]);
-	}
-	__syncthreads();
-
-	const uint32_t tile	 = s_misc[0];
-	const uint64_t base	 = (uint64_t)tile * kTile + (uint64_t)tid * kBPL;
-	const uint32_t valid = base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
-	uint64_t*	   rec	 = a.recs + (uint64_t)tile * kRecWords;
-
-	// ---- load 64 bytes (the tail lane reads only what exists) ----
-	uint32_t   x[16];
-	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
-	if (valid == kBPL && aligned) {
-		const uint4* p = reinterpret_cast<const uint4*>(a.data + base);
-#pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			const uint4 v = p[k];
-			x[4 * k + 0]  = v.x;
-			x[4 * k + 1]  = v.y;
-			x[4 * k + 2]  = v.z;
-			x[4 * k + 3]  = v.w;
-		}
-	} else {
-#pragma unroll
-		for (int k = 0; k < 16; ++k) {
-			uint32_t w = 0;
-#pragma unroll
-			for (int b = 0; b < 4; ++b)
-				if ((uint32_t)(4 * k + b) < valid)
-					w |= (uint32_t)a.data[base + 4 * k + b] << (8 * b);
-			x[k] = w;
+	__device__ static void load(Lds& L, const DevPlan* P) {
+		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) L.cls[i] = (&P->cls[0][0])[i];
+		for (int i = threadIdx.x; i < ND * 256; i += kBlock) {
+			const uint32_t* e = P->dt[i >> 8][i & 255];
+			if constexpr (WIDE)
+				L.dt[i] = make_uint4(e[0], e[1], e[2], e[3]);
+			else
+				L.dt[i] = make_uint2(e[0], e[2]);
 		}
 	}
 
-	// ---- 1. one walk: class bits, lane maps, masked bit per entry state ----
 	Map		 T[NDA];
 	uint32_t accl[NDA][8];
 	uint32_t acch[WIDE ? NDA : 1][8];
 	uint32_t cacc[8];
-#pragma unroll
-	for (int g = 0; g < 8; ++g) {
-		cacc[g] = 0;
-#pragma unroll
-		for (int d = 0; d < NDA; ++d) {
-			accl[d][g] = 0;
-			if constexpr (WIDE)
-				acch[d][g] = 0;
-		}
-	}
-#pragma unroll
-	for (int d = 0; d < NDA; ++d) T[d] = M::id();
 
-	auto step = [&](int j) {
-		const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-		cacc[j >> 3] |= s_cls[(j & 7) * 256 + c];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			const DT e = s_dt[d * 256 + c];
-			if constexpr (WIDE) {
-				accl[d][j >> 3] |= perm(e.w, e.z, T[d].lo) << (j & 7);
-				acch[d][j >> 3] |= perm(e.w, e.z, T[d].hi) << (j & 7);
-				T[d] = M::compose(Map {e.x, e.y}, T[d]);
-			} else {
-				accl[d][j >> 3] |= perm(0u, e.y, T[d].lo) << (j & 7);
-				T[d] = M::compose(Map {e.x}, T[d]);
-			}
-		}
-	};
-	if (valid == kBPL) {
-#pragma unroll
-		for (int j = 0; j < kBPL; ++j) step(j);
-	} else {
-#pragma unroll
-		for (int j = 0; j < kBPL; ++j)
-			if ((uint32_t)j < valid)
-				step(j);
-	}
-
-	// ---- 2. chain 1: entry states ----
-	uint32_t tile_in = a.in_states;
-	Map		 lpre[NDA];
-	if constexpr (ND > 0) {
-		Map inc[ND], exc[ND];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			inc[d] = wave_scan_maps<WIDE>(T[d]);
-			exc[d] = M::shfl_up(inc[d], 1);
-			if (lane == 0)
-				exc[d] = M::id();
-		}
-		if (lane == 63)
-#pragma unroll
-			for (int d = 0; d < ND; ++d) s_wagg[wid][d] = inc[d];
-		__syncthreads();
-		Map tagg[ND];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			Map wp = M::id();
-#pragma unroll
-			for (int w2 = 0; w2 < 4; ++w2) {
-				if (w2 == wid)
-					lpre[d] = M::compose(exc[d], wp);
-				wp = M::compose(s_wagg[w2][d], wp);
-			}
-			tagg[d] = wp;
-		}
-		if (tile != 0) {
-			if (tid == 0)
-#pragma unroll
-				for (int d = 0; d < ND; ++d)
-					publish(rec + kRecSAgg + d, make_word(a.epoch, kAgg, M::pack(tagg[d])));
-			if (wid == 0) {
-				const uint32_t st = lookback_states<ND, WIDE>(a.recs, tile, a.epoch, a.in_states, a.sum);
-				if (lane == 0)
-					s_misc[1] = st;
-			}
-			__syncthreads();
-			tile_in = s_misc[1];
-		}
-		if (tid == 0) {
-			uint32_t out = 0;
-#pragma unroll
-			for (int d = 0; d < ND; ++d) out |= M::apply(tagg[d], (tile_in >> (3 * d)) & 7u) << (3 * d);
-			publish(rec + kRecSIncl, make_word(a.epoch, kIncl, out));
-			if (tile == a.n_tiles - 1)
-				a.sum->final_states = out;
-		}
-	} else {
-		if (tid == 0 && tile == a.n_tiles - 1)
-			a.sum->final_states = 0;
-	}
-
-	// ---- 3. masks and counts ----
-	uint64_t masked = 0;
-#pragma unroll
-	for (int d = 0; d < ND; ++d) {
-		const uint32_t s0 = M::apply(lpre[d], (tile_in >> (3 * d)) & 7u);
+	__device__ void walk(const Lds& L, const DevPlan*, const uint32_t (&x)[16], uint32_t valid) {
 #pragma unroll
 		for (int g = 0; g < 8; ++g) {
-			uint32_t byte;
-			if constexpr (WIDE)
-				byte = ((s0 < 4 ? accl[d][g] : acch[d][g]) >> (8 * (s0 & 3))) & 0xFFu;
-			else
-				byte = (accl[d][g] >> (8 * s0)) & 0xFFu;
-			masked |= (uint64_t)byte << (8 * g);
-		}
-	}
-	uint64_t S = 0, OC = 0, OP = 0;
+			cacc[g] = 0;
 #pragma unroll
-	for (int g = 0; g < 8; ++g) {
-		S |= (uint64_t)(cacc[g] & 0xFFu) << (8 * g);
-		OC |= (uint64_t)((cacc[g] >> 8) & 0xFFu) << (8 * g);
-		OP |= (uint64_t)((cacc[g] >> 16) & 0xFFu) << (8 * g);
-	}
-	const uint64_t sm = S & ~masked, om = OC & ~masked, opm = OP & ~masked;
-	if (a.bitmap && valid)
-		a.bitmap[base >> 6] = sm;
-
-	const uint32_t cnt[3] = {(uint32_t)__popcll(sm), (uint32_t)__popcll(om),
-							 2u * (uint32_t)__popcll(opm) - (uint32_t)__popcll(om)};
-	uint32_t	   cinc[3];
-#pragma unroll
-	for (int f = 0; f < 3; ++f) cinc[f] = wave_incl_sum(cnt[f]);
-	if (lane == 63)
-#pragma unroll
-		for (int f = 0; f < 3; ++f) s_wcnt[wid][f] = cinc[f];
-	__syncthreads();
-	uint32_t cpre[3], ctot[3];
-#pragma unroll
-	for (int f = 0; f < 3; ++f) {
-		uint32_t p = 0, t = 0;
-#pragma unroll
-		for (int w2 = 0; w2 < 4; ++w2) {
-			const uint32_t v = s_wcnt[w2][f];
-			if (w2 < wid)
-				p += v;
-			t += v;
-		}
-		cpre[f] = p + cinc[f] - cnt[f];
-		ctot[f] = t;
-	}
-
-	// chain 2: tile output offsets
-	const uint32_t cbase[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
-	uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
-	if (tile != 0) {
-		if (tid == 0)
-#pragma unroll
-			for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
-		if (wid == 0) {
-			uint32_t o[3];
-			lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
-			if (lane == 0) {
-				s_misc[2] = o[0];
-				s_misc[3] = o[1];
-				s_misc[4] = o[2];
+			for (int d = 0; d < NDA; ++d) {
+				accl[d][g] = 0;
+				if constexpr (WIDE)
+					acch[d][g] = 0;
 			}
 		}
-		__syncthreads();
-		cex[0] = s_misc[2];
-		cex[1] = s_misc[3];
-		cex[2] = s_misc[4];
-	}
-	if (tid == 0) {
 #pragma unroll
-		for (int f = 0; f < 3; ++f) publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
-		if (tile == a.n_tiles - 1) {
-			a.sum->n_struct	  = cex[0] + ctot[0];
-			a.sum->n_brackets = cex[1] + ctot[1];
-			a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
-		}
-	}
-
-	// ---- 4. scatter ----
-	uint32_t	   si = cex[0] + cpre[0], bi = cex[1] + cpre[1];
-	int32_t		   bal	= (int32_t)(cex[2] + cpre[2]);
-	int32_t		   lmin = INT_MAX, lmax = INT_MIN;
-	const uint64_t pbase = a.byte_base + base;
+		for (int d = 0; d < NDA; ++d) T[d] = M::id();
+		auto step = [&](int j) {
+			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+			cacc[j >> 3] |= L.cls[(j & 7) * 256 + c];
 #pragma unroll
-	for (int q = 0; q < 16; ++q) {
-		uint32_t m = (uint32_t)(sm >> (4 * q)) & 0xFu;
-		while (m) {
-			const int	   b   = __builtin_ctz(m);
-			const int	   p   = 4 * q + b;
-			const uint32_t pos = (uint32_t)(pbase + (uint64_t)p);
-			const uint8_t  ch  = (uint8_t)(x[q] >> (8 * b));
-			if (si < a.cap_struct) {
-				a.positions[si] = pos;
-				if (a.chars)
-					a.chars[si] = ch;
-				if (a.pair_pos)
-					a.pair_pos[si] = -1;
-			}
-			if ((om >> p) & 1ull) {
-				const bool op = (opm >> p) & 1ull;
-				bal += op ? 1 : -1;
-				lmin = min(lmin, bal);
-				lmax = max(lmax, bal);
-				if (bi < a.cap_br) {
-					a.bpos[bi]	= pos;
-					a.bsidx[bi] = si;
-					a.bkey[bi]	= op ? bal - 1 : bal;
-					a.bchar[bi] = ch;
+			for (int d = 0; d < ND; ++d) {
+				const DT e = L.dt[d * 256 + c];
+				if constexpr (WIDE) {
+					accl[d][j >> 3] |= perm(e.w, e.z, T[d].lo) << (j & 7);
+					acch[d][j >> 3] |= perm(e.w, e.z, T[d].hi) << (j & 7);
+					T[d] = M::compose(Map {e.x, e.y}, T[d]);
+				} else {
+					accl[d][j >> 3] |= perm(0u, e.y, T[d].lo) << (j & 7);
+					T[d] = M::compose(Map {e.x}, T[d]);
 				}
-				++bi;
 			}
-			++si;
-			m &= m - 1;
-		}
-	}
-	lmin = wave_min(lmin);
-	lmax = wave_max(lmax);
-	if (lane == 0) {
-		s_mm[wid][0] = lmin;
-		s_mm[wid][1] = lmax;
-	}
-	__syncthreads();
-	if (tid == 0) {
-		int32_t mn = s_mm[0][0], mx = s_mm[0][1];
+		};
+		if (valid == kBPL) {
 #pragma unroll
-		for (int w2 = 1; w2 < 4; ++w2) {
-			mn = min(mn, s_mm[w2][0]);
-			mx = max(mx, s_mm[w2][1]);
+			for (int j = 0; j < kBPL; ++j) step(j);
+		} else {
+#pragma unroll
+			for (int j = 0; j < kBPL; ++j)
+				if ((uint32_t)j < valid)
+					step(j);
 		}
-		if (mn != INT_MAX)
-			atomicMin(&a.sum->min_bal, mn);
-		if (mx != INT_MIN)
-			atomicMax(&a.sum->max_bal, mx);
 	}
+
+	__device__ uint64_t masked(const uint32_t (&s0)[NDA]) const {
+		uint64_t m = 0;
+#pragma unroll
+		for (int d = 0; d < ND; ++d) {
+			const uint32_t s = s0[d];
+#pragma unroll
+			for (int g = 0; g < 8; ++g) {
+				uint32_t byte;
+				if constexpr (WIDE)
+					byte = ((s < 4 ? accl[d][g] : acch[d][g]) >> (8 * (s & 3))) & 0xFFu;
+				else
+					byte = (accl[d][g] >> (8 * s)) & 0xFFu;
+				m |= (uint64_t)byte << (8 * g);
+			}
+		}
+		return m;
+	}
+
+	__device__ void classes(uint64_t& S, uint64_t& OC, uint64_t& OP) const {
+		uint64_t unused;
+		gather_classes(cacc, S, OC, OP, unused);
+	}
+};
+
+// ---------------------------------------------------------------------------
+// EscapeEval: the automaton out -delim-> in, in -delim-> closed, in -escape->
+// esc, esc -any-> in, closed behaving as out; masked exactly in `in`/`esc`.
+// A lane evaluates it bit-parallel for each possible entry state:
+//   escapers  = backslashes not themselves escaped (odd positions within a
+//               run's even/odd parity class, one carry-add per word);
+//   U / X     = delimiters not escaped / escaped;
+//   in-string = exclusive prefix-xor of the real toggles, where U always
+//               toggles and X toggles only when outside (an escape outside a
+//               string is inert); X events are resolved in order, rarely more
+//               than one per lane, none on valid JSON for the true entry state.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t prefix_xor(uint64_t x) {
+	x ^= x << 1;
+	x ^= x << 2;
+	x ^= x << 4;
+	x ^= x << 8;
+	x ^= x << 16;
+	x ^= x << 32;
+	return x;
 }
+
+__device__ __forceinline__ uint64_t escapers(uint64_t bs, uint64_t carry) {
+	constexpr uint64_t EVEN		 = 0x5555555555555555ull;
+	const uint64_t	   b		 = bs & ~carry;	 // an escaped first byte escapes nothing
+	const uint64_t	   starts	 = b & ~(b << 1);
+	const uint64_t	   even_runs = b & ~(b + (starts & EVEN));
+	const uint64_t	   odd_runs	 = b & ~even_runs;
+	return (even_runs & EVEN) | (odd_runs & ~EVEN);
+}
+
+enum : uint32_t { kCodeOut = 0, kCodeIn = 1, kCodeClosed = 2, kCodeEsc = 3 };
+
+struct EscHyp {
+	uint64_t P;		   // bytes masked (state before the byte is in or esc)
+	uint64_t R;		   // delimiters that toggle
+	uint64_t escaped;  // bytes consumed by an escape
+};
+
+__device__ __forceinline__ EscHyp esc_eval(uint64_t Q, uint64_t BS, bool in0, bool esc0) {
+	const uint64_t carry   = esc0 ? 1ull : 0ull;
+	const uint64_t escaped = (escapers(BS, carry) << 1) | carry;
+	const uint64_t U = Q & ~escaped, X = Q & escaped;
+	uint64_t	   R = U;
+	uint64_t	   P = prefix_xor(U) ^ U;
+	if (in0)
+		P = ~P;
+	uint64_t pend = X & ~P;
+	while (pend) {
+		const uint64_t b	 = pend & (~pend + 1ull);
+		const uint64_t above = ~((b << 1) - 1ull);
+		P ^= above;
+		R |= b;
+		pend = X & ~P & above;
+	}
+	return EscHyp {P, R, escaped};
+}
+
+__device__ __forceinline__ uint32_t esc_code(const EscHyp& h, uint64_t BS, uint64_t last) {
+	const bool pl = (h.P & last) != 0, rl = (h.R & last) != 0;
+	if (pl == rl)
+		return pl ? kCodeClosed : kCodeOut;
+	return (pl && (BS & last) && !(h.escaped & last)) ? kCodeEsc : kCodeIn;
+}
+
+struct EscapeEval {
+	static constexpr int  kND	= 1;
+	static constexpr bool kWide = false;
+	using Map					= Map4;
+
+	struct Lds {
+		uint2 cls[8 * 256];	 // {S|OC|OP|delim bytes, escape byte}, pre-shifted by position
+	};
+
+	__device__ static void load(Lds& L, const DevPlan* P) {
+		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) {
+			const uint32_t* e = P->cls8[i >> 8][i & 255];
+			L.cls[i]		  = make_uint2(e[0], e[1]);
+		}
+	}
+
+	Map		 T[1];
+	uint64_t S, OC, OP;
+	uint64_t mO, mI, mE;
+	uint32_t st_in, st_esc;
+
+	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
+		uint32_t clo[8], chi[8];
+#pragma unroll
+		for (int g = 0; g < 8; ++g) clo[g] = chi[g] = 0;
+		auto step = [&](int j) {
+			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+			const uint2	   e = L.cls[(j & 7) * 256 + c];
+			clo[j >> 3] |= e.x;
+			chi[j >> 3] |= e.y;
+		};
+		if (valid == kBPL) {
+#pragma unroll
+			for (int j = 0; j < kBPL; ++j) step(j);
+		} else {
+#pragma unroll
+			for (int j = 0; j < kBPL; ++j)
+				if ((uint32_t)j < valid)
+					step(j);
+		}
+		uint64_t Q;
+		gather_classes(clo, S, OC, OP, Q);
+		const uint32_t bs_lo = perm(chi[1], chi[0], 0x0C0C0400u) | perm(chi[3], chi[2], 0x04000C0Cu);
+		const uint32_t bs_hi = perm(chi[5], chi[4], 0x0C0C0400u) | perm(chi[7], chi[6], 0x04000C0Cu);
+		const uint64_t BS	 = ((uint64_t)bs_hi << 32) | bs_lo;
+
+		const uint32_t f0 = plan->fk_state[0], f1 = plan->fk_state[1], f2 = plan->fk_state[2],
+					   f3 = plan->fk_state[3];
+		st_in			  = f1;
+		st_esc			  = f3;
+		if (valid == 0) {
+			T[0].lo = kIdLo;
+			mO = mI = mE = 0;
+			return;
+		}
+		const uint64_t last = 1ull << (valid - 1);
+		const EscHyp   hI	= esc_eval(Q, BS, true, false);
+		EscHyp		   hO, hE;
+		if ((Q & hI.escaped) == 0) {  // no escaped delimiter: entry out is entry in, inverted
+			hO = EscHyp {~hI.P, hI.R, hI.escaped};
+		} else {
+			hO = esc_eval(Q, BS, false, false);
+		}
+		if (((Q | BS) & 1ull) == 0) {  // first byte plain: entry esc only consumes it
+			hE = EscHyp {hI.P, hI.R, hI.escaped | 1ull};
+		} else {
+			hE = esc_eval(Q, BS, true, true);
+		}
+		mO = hO.P;
+		mI = hI.P;
+		mE = hE.P;
+		const uint32_t fk[4] = {f0, f1, f2, f3};
+		const uint32_t tO = fk[esc_code(hO, BS, last)], tI = fk[esc_code(hI, BS, last)],
+					   tE = fk[esc_code(hE, BS, last)];
+		T[0].lo = (tO << (8 * f0)) | (tI << (8 * f1)) | (tO << (8 * f2)) | (tE << (8 * f3));
+	}
+
+	__device__ uint64_t masked(const uint32_t (&s0)[1]) const {
+		return s0[0] == st_in ? mI : (s0[0] == st_esc ? mE : mO);
+	}
+
+	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {
+		s  = S;
+		oc = OC;
+		op = OP;
+	}
+};
 
 __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 	if (threadIdx.x == 0) {
@@ -518,9 +303,9 @@ __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 	}
 }
 
-template<int ND, bool WIDE>
-hipError_t launch_nd(const ScanArgs& a, hipStream_t s) {
-	hipLaunchKernelGGL((k_scan_general<ND, WIDE>), dim3(a.n_tiles), dim3(kBlock), 0, s, a);
+template<class E>
+hipError_t launch_eval(const ScanArgs& a, hipStream_t s) {
+	hipLaunchKernelGGL((k_scan<E>), dim3(a.n_tiles), dim3(kBlock), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -531,31 +316,37 @@ hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStrea
 	return hipGetLastError();
 }
 
+hipError_t launch_scan_escape(const ScanArgs& a, hipStream_t s) {
+	if (a.n_tiles == 0)
+		return hipSuccess;
+	return launch_eval<EscapeEval>(a, s);
+}
+
 hipError_t launch_scan_general(const ScanArgs& a, int nd, bool wide, hipStream_t s) {
 	if (a.n_tiles == 0)
 		return hipSuccess;
 	if (!wide) {
 		switch (nd) {
-		case 0: return launch_nd<0, false>(a, s);
-		case 1: return launch_nd<1, false>(a, s);
-		case 2: return launch_nd<2, false>(a, s);
-		case 3: return launch_nd<3, false>(a, s);
-		case 4: return launch_nd<4, false>(a, s);
-		case 5: return launch_nd<5, false>(a, s);
-		case 6: return launch_nd<6, false>(a, s);
-		case 7: return launch_nd<7, false>(a, s);
-		case 8: return launch_nd<8, false>(a, s);
+		case 0: return launch_eval<GeneralEval<0, false>>(a, s);
+		case 1: return launch_eval<GeneralEval<1, false>>(a, s);
+		case 2: return launch_eval<GeneralEval<2, false>>(a, s);
+		case 3: return launch_eval<GeneralEval<3, false>>(a, s);
+		case 4: return launch_eval<GeneralEval<4, false>>(a, s);
+		case 5: return launch_eval<GeneralEval<5, false>>(a, s);
+		case 6: return launch_eval<GeneralEval<6, false>>(a, s);
+		case 7: return launch_eval<GeneralEval<7, false>>(a, s);
+		case 8: return launch_eval<GeneralEval<8, false>>(a, s);
 		}
 	} else {
 		switch (nd) {
-		case 1: return launch_nd<1, true>(a, s);
-		case 2: return launch_nd<2, true>(a, s);
-		case 3: return launch_nd<3, true>(a, s);
-		case 4: return launch_nd<4, true>(a, s);
-		case 5: return launch_nd<5, true>(a, s);
-		case 6: return launch_nd<6, true>(a, s);
-		case 7: return launch_nd<7, true>(a, s);
-		case 8: return launch_nd<8, true>(a, s);
+		case 1: return launch_eval<GeneralEval<1, true>>(a, s);
+		case 2: return launch_eval<GeneralEval<2, true>>(a, s);
+		case 3: return launch_eval<GeneralEval<3, true>>(a, s);
+		case 4: return launch_eval<GeneralEval<4, true>>(a, s);
+		case 5: return launch_eval<GeneralEval<5, true>>(a, s);
+		case 6: return launch_eval<GeneralEval<6, true>>(a, s);
+		case 7: return launch_eval<GeneralEval<7, true>>(a, s);
+		case 8: return launch_eval<GeneralEval<8, true>>(a, s);
 		}
 	}
 	return hipErrorInvalidValue;

@@ -67,3 +67,15 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(N, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         N.lib()
+
+
+def test_cpp_facade_builds_and_links():
+    """include/pars_hip.hpp compiles against the C ABI (scanner<S> with a
+    compile-time plan, runtime_scanner with a loaded one) and links to the
+    in-tree library."""
+    exe = os.path.join(ROOT, "tests", "cpp", "facade_check")
+    assert os.access(exe, os.X_OK)
+    import subprocess
+
+    out = subprocess.run(["ldd", exe], stdout=subprocess.PIPE, text=True).stdout
+    assert "libpars_hip.so" in out and "not found" not in out

@@ -221,3 +221,24 @@ def test_clike6_1gib_full_size(torch_cuda):
     assert np.array_equal(got["pair"], exp["pair"])
     assert (got["balanced"], got["max_depth"]) == (exp["balanced"], exp["max_depth"])
     sc.close()
+
+
+def test_cpp_facade_end_to_end(torch_cuda, tmp_path):
+    """The C++ facade, driven like gpu::scanner<S>, equals the oracle."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "facade_check")
+    d = W.json(71, 50_000)
+    inp = tmp_path / "in.json"
+    d.tofile(inp)
+    say("[gpu] C++ facade")
+    r = subprocess.run([exe, os.path.join(os.path.dirname(os.path.dirname(__file__)), "pars_amd", "plans",
+                                          "json.plan"), str(inp)], stdout=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0
+    lines = r.stdout.split()
+    k, bal, wf, md = (int(v) for v in lines[:4])
+    exp = O.full(PLANS["json"].blob, d)
+    assert k == exp["positions"].size and (bool(bal), bool(wf), md) == (exp["balanced"], exp["well_formed"],
+                                                                       exp["max_depth"])
+    assert np.array_equal(np.array(lines[4:], dtype=np.int64), exp["positions"].astype(np.int64))

@@ -102,7 +102,8 @@ typedef struct {
 typedef enum {
 	PARS_OPT_WANT_CHARS	 = 1, /* also emit structural bytes (default 0) */
 	PARS_OPT_WANT_BITMAP = 2, /* also emit the structural bitmap (default 1) */
-	PARS_OPT_PATH		 = 3  /* 0 = auto, 1 = force the general descriptor path */
+	PARS_OPT_PATH		 = 3, /* 0 = auto, 1 = force the general descriptor path */
+	PARS_OPT_ROWS_PER_TILE = 4 /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 */
 } pars_option;
 
 /* ---- scanner lifetime (reference: gpu::scanner<S> constructor/destructor) ---- */

@@ -24,6 +24,7 @@ STATUS_NAMES = {
 PARS_OPT_WANT_CHARS = 1
 PARS_OPT_WANT_BITMAP = 2
 PARS_OPT_PATH = 3
+PARS_OPT_ROWS_PER_TILE = 4
 
 
 class StageTimes(C.Structure):

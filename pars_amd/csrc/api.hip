@@ -89,10 +89,10 @@ struct pars_scanner {
 	DevBuf	  dplan;
 	DevBuf	  recs, ticket, summary;
 	DevBuf	  positions, chars, bitmap, pair_pos;
-	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, pair, hist, hist_part;
+	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	uint32_t  epoch		 = 0;
-	int		  want_chars = 0, want_bitmap = 1, path = 0;
+	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
 	DevSummary* h_sum	 = nullptr;	 // pinned
 	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
 	// scan_host staging
@@ -106,7 +106,7 @@ struct pars_scanner {
 
 	~pars_scanner() {
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
-						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &pair, &hist, &hist_part, &d_in})
+						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in})
 			b->release();
 		if (h_sum)
 			(void)hipHostFree(h_sum);
@@ -159,7 +159,8 @@ int ensure_outputs(pars_scanner* sc, uint64_t n_tiles, uint64_t cap_struct, uint
 		return r;
 	if (want_pairs && (r = sc->pair_pos.ensure(cap_struct * 4)))
 		return r;
-	for (DevBuf* b : {&sc->bpos, &sc->bsidx, &sc->bkey, &sc->ukeys0, &sc->ukeys1, &sc->order0, &sc->order1, &sc->pair})
+	for (DevBuf* b : {&sc->bpos, &sc->bsidx, &sc->bkey, &sc->ukeys0, &sc->ukeys1, &sc->order0, &sc->order1, &sc->rank,
+					  &sc->pair})
 		if ((r = b->ensure(cap_br * 4)))
 			return r;
 	if ((r = sc->bchar.ensure(cap_br)))
@@ -183,6 +184,7 @@ ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool 
 	a.plan		 = sc->dplan.as<DevPlan>();
 	a.recs		 = sc->recs.as<uint64_t>();
 	a.ticket	 = sc->ticket.as<uint32_t>();
+	a.done		 = sc->ticket.as<uint32_t>() + 1;
 	a.sum		 = sc->summary.as<DevSummary>();
 	a.positions	 = sc->positions.as<uint32_t>();
 	a.chars		 = sc->want_chars ? sc->chars.as<uint8_t>() : nullptr;
@@ -210,6 +212,7 @@ PairArgs make_pair_args(pars_scanner* sc, bool want_pairs) {
 	p.ukeys[1]	 = sc->ukeys1.as<uint32_t>();
 	p.order[0]	 = sc->order0.as<uint32_t>();
 	p.order[1]	 = sc->order1.as<uint32_t>();
+	p.rank		 = sc->rank.as<uint32_t>();
 	p.hist		 = sc->hist.as<uint32_t>();
 	p.hist_part	 = sc->hist_part.as<uint32_t>();
 	p.max_chunks = pairing_max_chunks(sc->cap_br);
@@ -255,8 +258,10 @@ void fill_result(pars_scanner* sc, uint64_t n, bool want_pairs, bool run_pairs, 
 // R1-R5 through the fastest evaluator that is exact for the plan.
 hipError_t launch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastEscape)
-		return launch_scan_escape(a, s);
-	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, s);
+		return launch_scan_escape(a, sc->rows, s);
+	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
+		return launch_scan_parity(a, sc->rows, s);
+	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 
 // One full document: reset, R1-R5, optional R6; no host sync inside.
@@ -340,6 +345,11 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 		if (value < 0 || value > 1)
 			return fail(PARS_ERR_ARGUMENT, "path must be 0 (auto) or 1 (general)");
 		sc->path = (int)value;
+		return PARS_OK;
+	case PARS_OPT_ROWS_PER_TILE:
+		if (value != 0 && value != 1 && value != 2 && value != 4)
+			return fail(PARS_ERR_ARGUMENT, "rows per tile must be 0 (default), 1, 2 or 4");
+		sc->rows = (int)value;
 		return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);

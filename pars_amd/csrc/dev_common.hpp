@@ -53,6 +53,10 @@ struct MapT<false> {
 
 	__device__ static Map4 shfl(Map4 m, int src) { return Map4 {(uint32_t)__shfl((int)m.lo, src, kWave)}; }
 
+	__device__ static Map4 shfl_down(Map4 m, int off) {
+		return Map4 {(uint32_t)__shfl_down((int)m.lo, off, kWave)};
+	}
+
 	// 3-bit packed form used in look-back words (24 bits for 8 states)
 	__device__ static uint32_t pack(Map4 m) {
 		uint32_t r = 0;
@@ -91,6 +95,10 @@ struct MapT<true> {
 
 	__device__ static Map8 shfl(Map8 m, int src) {
 		return Map8 {(uint32_t)__shfl((int)m.lo, src, kWave), (uint32_t)__shfl((int)m.hi, src, kWave)};
+	}
+
+	__device__ static Map8 shfl_down(Map8 m, int off) {
+		return Map8 {(uint32_t)__shfl_down((int)m.lo, off, kWave), (uint32_t)__shfl_down((int)m.hi, off, kWave)};
 	}
 
 	__device__ static uint32_t pack(Map8 m) {

@@ -10,8 +10,8 @@
 namespace pars_hip {
 
 constexpr int kBlock = 256;			   // threads per scan block (4 waves)
-constexpr int kBPL	 = 64;			   // bytes per lane
-constexpr int kTile	 = kBlock * kBPL;  // bytes per scan tile (16 KiB)
+constexpr int kBPL	 = 64;			   // bytes per lane and row
+constexpr int kTile	 = kBlock * kBPL;  // bytes per tile row (16 KiB); a tile holds 1..4 rows
 constexpr int kRecWords = 16;		   // look-back words per tile record
 
 // Record layout (u64 words): [0..7] state-chain aggregates, one per
@@ -34,17 +34,19 @@ struct DevSummary {
 	int32_t	 kmin;		   // smallest pairing key
 	uint32_t n_chunks;	   // pairing sort chunks
 	uint32_t overflow;	   // an output exceeded its capacity
-	uint32_t pad[5];
+	uint32_t digit_bits;   // radix digit width of the pairing sort
+	uint32_t pad[4];
 };
 
 struct ScanArgs {
 	const uint8_t*	data;
 	uint64_t		n;
-	uint32_t		n_tiles;
+	uint32_t		n_tiles;  // set by the launcher from the evaluator's rows per tile
 	uint32_t		epoch;
 	const DevPlan*	plan;
-	uint64_t*		recs;
-	uint32_t*		ticket;
+	uint64_t*		recs;	  // one record per tile; sized for one-row tiles
+	uint32_t*		ticket;	  // [0] next tile, [1] blocks finished; both 0 between launches
+	uint32_t*		done;
 	DevSummary*		sum;
 	uint32_t*		positions;
 	uint8_t*		chars;	   // optional
@@ -63,10 +65,13 @@ struct ScanArgs {
 	uint32_t		in_states;	 // packed per-descriptor states before the first byte
 };
 
-// R1-R5: one pass over the input (plus summary reset), general descriptor path.
-hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
+// R1-R5: one pass over the input, general descriptor path.  `rows` = 16 KiB
+// rows per tile (1, 2 or 4; 0 = the default); tuned per plan.
+hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 // R1-R5 for plans whose one descriptor is an affine escape automaton (JSON).
-hipError_t launch_scan_escape(const ScanArgs& a, hipStream_t s);
+hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s);
+// R1-R5 for plans whose one descriptor is quote parity (CSV doubled quotes).
+hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
 // Initial summary values: a fresh document starts all at 0; a chunk of a
 // longer document starts from the running values its carry holds.
 struct SummaryInit {
@@ -90,6 +95,7 @@ struct PairArgs {
 	uint64_t	   cap_br;
 	uint32_t*	   ukeys[2];  // radix ping-pong keys
 	uint32_t*	   order[2];  // radix ping-pong values (bracket indices)
+	uint32_t*	   rank;	  // sorted position of each bracket (last pass)
 	uint32_t*	   hist;	  // 256 * max_chunks
 	uint32_t*	   hist_part; // scan partials
 	uint64_t	   max_chunks;

@@ -9,11 +9,14 @@
 // sorted stably, so the two agree on every input.
 //
 // Layout: the scan kernel leaves bkey/bchar/bsidx per bracket.  The radix sort
-// runs 8-bit LSD passes over u = key - kmin; the pass count (1..4) comes from
-// the key range found on device, so no host round trip sizes anything.  Each
-// pass: per-chunk digit histogram (LDS atomics; counts are order-free), one
-// device-wide exclusive scan of the digit-major histogram, then a stable
-// scatter that ranks equal digits in input order with wave ballots.
+// runs LSD passes over u = key - kmin with a digit width fitted to the key
+// range found on device (one 6-bit pass for every document up to depth 63,
+// at most 8 bits per pass, 1..4 passes): per-chunk digit histograms (LDS
+// atomics — counts are order-free), one device-wide exclusive scan of the
+// digit-major histogram, then a stable scatter that ranks equal digits in
+// input order with wave ballots.  The last pass also records each bracket's
+// sorted rank, so pairing walks brackets in document order, gathers the
+// sorted neighbour, and writes every output sequentially.
 #include "dev_common.hpp"
 #include "kernels.hpp"
 
@@ -24,7 +27,7 @@ namespace pars_hip {
 namespace {
 
 constexpr int kSortBlock = 256;
-constexpr int kSortRows	 = 8;
+constexpr int kSortRows	 = 16;
 constexpr int kChunk	 = kSortBlock * kSortRows;	// brackets per sort chunk
 constexpr int kScanGrid	 = 256;						// blocks of the histogram scan
 
@@ -41,10 +44,11 @@ __global__ void k_sort_prep(PairArgs a) {
 	const int32_t  kmin	 = s->min_bal - 1;	// open keys are depth-1, depths >= min_bal
 	const int32_t  kmax	 = s->max_bal;
 	const uint32_t range = (uint32_t)(kmax - kmin);
-	const uint32_t bits	 = range ? 32u - (uint32_t)__clz(range) : 0u;
-	uint32_t	   p	 = (bits + 7u) / 8u;
+	const uint32_t bits	 = range ? 32u - (uint32_t)__clz(range) : 1u;
+	const uint32_t p	 = (bits + 7u) / 8u;
 	s->kmin				 = kmin;
-	s->sort_passes		 = p < 1u ? 1u : p;
+	s->sort_passes		 = p;
+	s->digit_bits		 = (bits + p - 1) / p;
 	s->n_chunks			 = (nb + kChunk - 1) / kChunk;
 	if ((uint64_t)s->n_chunks > a.max_chunks)
 		s->overflow |= 2u;
@@ -61,23 +65,25 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(PairArgs a) {
 	__shared__ uint32_t h[256];
 	const uint32_t		nb = n_brackets(a), nc = a.sum->n_chunks;
 	const int32_t		kmin = a.sum->kmin;
-	const int			tid	 = threadIdx.x;
+	const uint32_t		w = a.sum->digit_bits, B = 1u << w, mask = B - 1u, shift = PASS * w;
+	const int			tid = threadIdx.x;
 	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
 		h[tid] = 0;
 		__syncthreads();
-#pragma unroll
+#pragma unroll 4
 		for (int r = 0; r < kSortRows; ++r) {
 			const uint32_t i = c * kChunk + r * kSortBlock + tid;
 			if (i < nb)
-				atomicAdd(&h[(key_at(a, PASS, i, kmin) >> (8 * PASS)) & 255u], 1u);
+				atomicAdd(&h[(key_at(a, PASS, i, kmin) >> shift) & mask], 1u);
 		}
 		__syncthreads();
-		a.hist[(uint64_t)tid * nc + c] = h[tid];
+		if ((uint32_t)tid < B)
+			a.hist[(uint64_t)tid * nc + c] = h[tid];
 		__syncthreads();
 	}
 }
 
-// Device-wide exclusive scan of hist[0 .. 256*nc), three phases, fixed grid.
+// Device-wide exclusive scan of hist[0 .. B*nc), three phases, fixed grid.
 __device__ __forceinline__ void scan_range(uint32_t len, uint32_t b, uint32_t& lo, uint32_t& hi) {
 	const uint64_t per = ((uint64_t)len + kScanGrid - 1) / kScanGrid;
 	lo				   = (uint32_t)min<uint64_t>((uint64_t)b * per, len);
@@ -102,14 +108,17 @@ __device__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t& total) {
 	return p + inc - v;
 }
 
+__device__ __forceinline__ uint32_t hist_len(const PairArgs& a) {
+	return (1u << a.sum->digit_bits) * a.sum->n_chunks;
+}
+
 template<int PASS>
 __global__ __launch_bounds__(kSortBlock) void k_scan_partial(PairArgs a) {
 	if ((uint32_t)PASS >= a.sum->sort_passes)
 		return;
 	__shared__ uint32_t s_w[4];
-	const uint32_t		len = 256u * a.sum->n_chunks;
 	uint32_t			lo, hi;
-	scan_range(len, blockIdx.x, lo, hi);
+	scan_range(hist_len(a), blockIdx.x, lo, hi);
 	uint32_t acc = 0;
 	for (uint32_t i = lo + threadIdx.x; i < hi; i += kSortBlock) acc += a.hist[i];
 	uint32_t tot;
@@ -133,9 +142,8 @@ __global__ __launch_bounds__(kSortBlock) void k_scan_apply(PairArgs a) {
 	if ((uint32_t)PASS >= a.sum->sort_passes)
 		return;
 	__shared__ uint32_t s_w[4];
-	const uint32_t		len = 256u * a.sum->n_chunks;
 	uint32_t			lo, hi;
-	scan_range(len, blockIdx.x, lo, hi);
+	scan_range(hist_len(a), blockIdx.x, lo, hi);
 	uint32_t run = a.hist_part[blockIdx.x];
 	for (uint32_t i0 = lo; i0 < hi; i0 += kSortBlock) {
 		const uint32_t i = i0 + threadIdx.x;
@@ -158,14 +166,15 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
 	__shared__ uint32_t s_wo[4][256];
 	const uint32_t		nb = n_brackets(a), nc = a.sum->n_chunks;
 	const int32_t		kmin = a.sum->kmin;
+	const uint32_t		w = a.sum->digit_bits, B = 1u << w, mask = B - 1u, shift = PASS * w;
 	const bool			last = (uint32_t)PASS + 1 == passes;
-	const int			tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	const int			tid = threadIdx.x, wid = tid >> 6;
 	uint32_t*			okey = a.ukeys[PASS & 1];
 	uint32_t*			oval = a.order[PASS & 1];
 	for (uint32_t c = blockIdx.x; c < nc; c += gridDim.x) {
-		s_base[tid] = a.hist[(uint64_t)tid * nc + c];
+		s_base[tid] = (uint32_t)tid < B ? a.hist[(uint64_t)tid * nc + c] : 0u;
 #pragma unroll
-		for (int w = 0; w < 4; ++w) s_wc[w][tid] = 0;
+		for (int q = 0; q < 4; ++q) s_wc[q][tid] = 0;
 		__syncthreads();
 		for (int r = 0; r < kSortRows; ++r) {
 			const uint32_t i	 = c * kChunk + r * kSortBlock + tid;
@@ -174,12 +183,11 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
 			if (valid) {
 				u = key_at(a, PASS, i, kmin);
 				v = PASS == 0 ? i : a.order[(PASS - 1) & 1][i];
-				d = (u >> (8 * PASS)) & 255u;
+				d = (u >> shift) & mask;
 			}
 			// lanes holding the same digit, by ballots over the digit's bits
 			uint64_t peers = __ballot(valid);
-#pragma unroll
-			for (int b = 0; b < 8; ++b) {
+			for (uint32_t b = 0; b < w; ++b) {
 				const bool	   bit = (d >> b) & 1u;
 				const uint64_t bb  = __ballot(valid && bit);
 				peers &= bit ? bb : ~bb;
@@ -188,23 +196,25 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
 			if (valid && wrank == 0)
 				s_wc[wid][d] = (uint32_t)__popcll(peers);
 			__syncthreads();
-			{
+			if ((uint32_t)tid < B) {
 				uint32_t run = s_base[tid];
 #pragma unroll
-				for (int w = 0; w < 4; ++w) {
-					const uint32_t t = s_wc[w][tid];
-					s_wo[w][tid]	 = run;
+				for (int q = 0; q < 4; ++q) {
+					const uint32_t t = s_wc[q][tid];
+					s_wo[q][tid]	 = run;
 					run += t;
-					s_wc[w][tid] = 0;
+					s_wc[q][tid] = 0;
 				}
 				s_base[tid] = run;
 			}
 			__syncthreads();
 			if (valid) {
 				const uint32_t dst = s_wo[wid][d] + wrank;
-				if (!last)
+				oval[dst]		   = v;
+				if (last)
+					a.rank[v] = dst;
+				else
 					okey[dst] = u;
-				oval[dst] = v;
 			}
 		}
 		__syncthreads();
@@ -212,35 +222,31 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
 }
 
 // nest.hpp:78-90: sorted neighbours (2i, 2i+1) pair if open/close and matching.
+// Walked in document order: bracket i sits at sorted position rank[i]; its
+// neighbour is at rank[i] ^ 1.
 __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 	const uint32_t	nb	  = n_brackets(a);
 	const uint32_t	pass  = a.sum->sort_passes;
 	const uint32_t* order = a.order[(pass - 1) & 1];
 	const DevPlan*	P	  = a.plan;
 	bool			bad	  = false;
-	for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; 2ull * g < nb; g += gridDim.x * blockDim.x) {
-		const uint32_t s  = 2u * g;
-		const uint32_t ia = order[s];
-		if (s + 1 < nb) {
-			const uint32_t ib = order[s + 1];
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
+		const uint32_t p = a.rank[i], q = p ^ 1u;
+		int32_t		   partner = -1;
+		if (q < nb) {
+			const uint32_t j  = order[q];
+			const uint32_t ia = (p & 1u) ? j : i, ib = (p & 1u) ? i : j;
 			const uint8_t  ca = a.bchar[ia], cb = a.bchar[ib];
 			const bool ok = (P->br_flags[ca] & 1u) && (P->br_flags[cb] & 2u) && P->br_close_of[ca] == cb;
-			if (a.pair) {
-				a.pair[ia] = ok ? (int32_t)ib : -1;
-				a.pair[ib] = ok ? (int32_t)ia : -1;
-			}
-			if (a.pair_pos) {
-				const uint32_t sa = a.bsidx[ia], sb = a.bsidx[ib];
-				a.pair_pos[sa]	  = ok ? (int32_t)sb : -1;
-				a.pair_pos[sb]	  = ok ? (int32_t)sa : -1;
-			}
-			bad |= !ok;
-		} else {
-			if (a.pair)
-				a.pair[ia] = -1;
-			if (a.pair_pos)
-				a.pair_pos[a.bsidx[ia]] = -1;
+			if (ok)
+				partner = (int32_t)j;
+			else if (!(p & 1u))
+				bad = true;	 // one report per sorted pair
 		}
+		if (a.pair)
+			a.pair[i] = partner;
+		if (a.pair_pos)
+			a.pair_pos[a.bsidx[i]] = partner >= 0 ? (int32_t)a.bsidx[partner] : -1;
 	}
 	if (__ballot(bad) && lane_id() == 0)
 		atomicOr(&a.sum->mismatched, 1u);
@@ -262,7 +268,7 @@ hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
 	if (a.cap_br == 0)
 		return hipSuccess;
 	const uint64_t chunks = (a.cap_br + kChunk - 1) / kChunk;
-	const unsigned grid	  = (unsigned)(chunks < 2048 ? (chunks ? chunks : 1) : 2048);
+	const unsigned grid	  = (unsigned)std::min<uint64_t>(std::max<uint64_t>(chunks, 1), 2048);
 	hipLaunchKernelGGL(k_sort_prep, dim3(1), dim3(64), 0, s, a);
 	hipError_t e;
 	if ((e = launch_pass<0>(a, grid, s)) != hipSuccess)
@@ -273,8 +279,7 @@ hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
 		return e;
 	if ((e = launch_pass<3>(a, grid, s)) != hipSuccess)
 		return e;
-	const uint64_t pairs = (a.cap_br + 1) / 2;
-	const unsigned pgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((pairs + 255) / 256, 1), 4096);
+	const unsigned pgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>((a.cap_br + 255) / 256, 1), 4096);
 	hipLaunchKernelGGL(k_pair, dim3(pgrid), dim3(256), 0, s, a);
 	return hipGetLastError();
 }

@@ -66,6 +66,46 @@ bool is_escape_desc(const pars_lex_desc& d, uint8_t& dl, uint8_t& es, uint8_t st
 	return true;
 }
 
+// One descriptor that is quote parity, exactly the three-state automaton the
+// compiler yields for a doubled delimiter (`"` ("" | [^"])* `"`, CSV) or an
+// escape-less one (`'` [^']* `'`):
+//   out --delim--> in --delim--> closed --delim--> in, closed --other--> out,
+//   in --other--> in; masked in `in`, and in `closed` on a delimiter when doubled.
+bool is_parity_desc(const pars_lex_desc& d, uint8_t& dl, uint8_t st[4], uint8_t& doubled) {
+	if (d.n_states != 3 || d.n_classes != 2)
+		return false;
+	int			  nq = 0, qb = -1;
+	const uint8_t c0 = d.rep_of[0];
+	for (int b = 0; b < 256; ++b)
+		if (d.rep_of[b] != c0) {
+			++nq;
+			qb = b;
+		}
+	if (nq != 1) {	// maybe byte 0 itself is the delimiter
+		if (nq != 255)
+			return false;
+		qb = 0;
+	}
+	const uint8_t q = d.rep_of[qb], o = q == 0 ? 1 : 0;
+	if (d.rep_of[qb] == d.rep_of[(qb + 1) & 255])
+		return false;
+	const uint8_t s_out = d.start, s_in = d.trans[s_out][q], s_cl = d.trans[s_in][q];
+	if (s_out == s_in || s_out == s_cl || s_in == s_cl)
+		return false;
+	const bool ok = d.trans[s_out][o] == s_out && d.masked[s_out][o] == 0 && d.masked[s_out][q] == 0
+				 && d.trans[s_in][o] == s_in && d.masked[s_in][o] == 1 && d.masked[s_in][q] == 1
+				 && d.trans[s_cl][o] == s_out && d.trans[s_cl][q] == s_in && d.masked[s_cl][o] == 0;
+	if (!ok)
+		return false;
+	dl		= (uint8_t)qb;
+	st[0]	= s_out;
+	st[1]	= s_in;
+	st[2]	= s_cl;
+	st[3]	= s_cl;
+	doubled = d.masked[s_cl][q];
+	return true;
+}
+
 }  // namespace
 
 int lower_plan(const pars_plan& S, DevPlan& P) {
@@ -129,9 +169,11 @@ int lower_plan(const pars_plan& S, DevPlan& P) {
 	P.fast_kind = kFastNone;
 	if (S.n_descs == 1 && is_escape_desc(S.descs[0], P.fk_delim, P.fk_escape, P.fk_state))
 		P.fast_kind = kFastEscape;
+	else if (S.n_descs == 1 && is_parity_desc(S.descs[0], P.fk_delim, P.fk_state, P.fk_doubled))
+		P.fast_kind = kFastParity;
 	for (int j = 0; j < 8; ++j)
 		for (int c = 0; c < 256; ++c) {
-			const uint32_t q = P.fast_kind == kFastEscape && c == P.fk_delim;
+			const uint32_t q = P.fast_kind != kFastNone && c == P.fk_delim;
 			const uint32_t e = P.fast_kind == kFastEscape && c == P.fk_escape;
 			P.cls8[j][c][0]	 = P.cls[j][c] | (q << (24 + j));
 			P.cls8[j][c][1]	 = e << j;

@@ -40,7 +40,8 @@ struct DevPlan {
 	// affine-with-escape / parity fast paths: the descriptor's class of each
 	// event byte and its state numbering (filled when fast_kind != 0)
 	uint8_t fk_delim, fk_escape;
-	uint8_t fk_state[4];
+	uint8_t fk_state[4];  // escape: out, in, closed, esc; parity: out, in, closed, closed
+	uint8_t fk_doubled;	  // parity: a delimiter right after a closing one is masked (CSV "")
 };
 
 enum FastKind : uint8_t {

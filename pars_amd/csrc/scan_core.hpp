@@ -1,15 +1,17 @@
 // scan_core.hpp — the single-pass R1-R5 tile kernel, shared by every lane
-// evaluator (general descriptor walk, JSON-style escape fast path).
+// evaluator (general descriptor walk, JSON-style escape and quote-parity fast
+// paths).
 //
-// Per 16 KiB tile (256 lanes x 64 contiguous bytes):
-//   1. the evaluator walks the lane's bytes once and yields, per descriptor, the
-//      lane's state map, plus enough to produce the lane's masked bits for any
-//      lane-entry state without a second walk;
-//   2. block scan of the lane maps; decoupled look-back over tiles on tagged
-//      64-bit words gives the tile-entry states (chain 1);
+// Per tile (W rows of 256 lanes x 64 contiguous bytes):
+//   1. the evaluator walks each lane's bytes once and yields, per descriptor,
+//      the lane's state map, plus enough to produce the lane's masked bits for
+//      any lane-entry state without a second walk;
+//   2. block scans of the lane maps per row; decoupled look-back over tiles on
+//      tagged 64-bit words gives the tile-entry states (chain 1);
 //   3. masks -> unmasked structural / bracket bits; counts and the +-1 depth
-//      delta; block scan + a second look-back (chain 2) give output offsets;
-//   4. scatter of uint32 positions (and chars, bitmap, bracket keys for R6).
+//      delta; block scans + a second look-back (chain 2) give output offsets;
+//   4. uint32 positions (and bytes, bracket keys for R6) staged in LDS per row
+//      and written with coalesced stores; the bitmap word per lane.
 //
 // Replaces the reference's classify_kernel, desc_wordt64_kernel,
 // thrust::inclusive_scan(TCompose64), desc_replay64_kernel,
@@ -72,8 +74,12 @@ __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_
 			status = kIncl;
 			ival   = in_states;
 		} else {
+			// one round trip: the inclusive and every aggregate word together
 			const uint64_t* r = recs + (uint64_t)j * kRecWords;
 			const uint64_t	w = peek(r + kRecSIncl);
+			uint64_t		wa[ND];
+#pragma unroll
+			for (int d = 0; d < ND; ++d) wa[d] = peek(r + kRecSAgg + d);
 			if (word_is(w, epoch, kIncl)) {
 				status = kIncl;
 				ival   = (uint32_t)w;
@@ -81,11 +87,10 @@ __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_
 				status = kAgg;
 #pragma unroll
 				for (int d = 0; d < ND; ++d) {
-					const uint64_t wa = peek(r + kRecSAgg + d);
-					if (!word_is(wa, epoch, kAgg))
+					if (!word_is(wa[d], epoch, kAgg))
 						status = 0;
 					else
-						agg[d] = M::unpack((uint32_t)wa);
+						agg[d] = M::unpack((uint32_t)wa[d]);
 				}
 			}
 		}
@@ -149,27 +154,23 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 			v[1]   = base[1];
 			v[2]   = base[2];
 		} else {
-			const uint64_t* r	= recs + (uint64_t)j * kRecWords;
-			bool			inc = true;
+			// one round trip: the inclusive and aggregate words together
+			const uint64_t* r = recs + (uint64_t)j * kRecWords;
+			uint64_t		wi[3], wa[3];
 #pragma unroll
 			for (int f = 0; f < 3; ++f) {
-				const uint64_t w = peek(r + kRecCIncl + f);
-				if (!word_is(w, epoch, kIncl))
-					inc = false;
-				v[f] = (uint32_t)w;
+				wi[f] = peek(r + kRecCIncl + f);
+				wa[f] = peek(r + kRecCAgg + f);
 			}
-			if (inc) {
-				status = kIncl;
-			} else {
-				status = kAgg;
+			bool inc = true, agg = true;
 #pragma unroll
-				for (int f = 0; f < 3; ++f) {
-					const uint64_t w = peek(r + kRecCAgg + f);
-					if (!word_is(w, epoch, kAgg))
-						status = 0;
-					v[f] = (uint32_t)w;
-				}
+			for (int f = 0; f < 3; ++f) {
+				inc = inc && word_is(wi[f], epoch, kIncl);
+				agg = agg && word_is(wa[f], epoch, kAgg);
 			}
+			status = inc ? kIncl : (agg ? kAgg : 0u);
+#pragma unroll
+			for (int f = 0; f < 3; ++f) v[f] = (uint32_t)(inc ? wi[f] : wa[f]);
 		}
 		const uint64_t b_incl = __ballot(status == kIncl);
 		const uint64_t b_none = __ballot(status == 0);
@@ -200,246 +201,334 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 }
 
 // ---------------------------------------------------------------------------
-// The tile kernel.  E (the lane evaluator) provides:
+// The tile kernel.  Persistent: every block loads its LDS tables once, then
+// draws tiles from an atomic ticket in dispatch order until none is left (so a
+// tile only ever waits on tiles held by running blocks).  A tile is W rows of
+// 16 KiB; in row r, lane l owns bytes [r*16K + l*64, +64), so every row's
+// outputs are one contiguous run: they are staged in LDS and written back with
+// coalesced stores (scattered 4-byte stores amplify HBM writes several-fold).
+//
+// E (the lane evaluator) provides:
 //   kND, kWide, Map        descriptors and their map type
 //   Lds, load(Lds&, plan)  LDS tables (cooperative load, all threads)
-//   walk(Lds, plan, x, valid)         one pass over the lane's bytes
-//   T[d]                    lane maps after walk
+//   walk(Lds, plan, x, valid)   one pass over a lane's 64 bytes
+//   T[d]                    the lane's map after walk
 //   masked(s0[])            masked bits for lane-entry states s0
 //   classes(S, OC, OP)      structural / bracket / open bits
 // ---------------------------------------------------------------------------
-template<class E>
+constexpr uint32_t kRowBytes = kBlock * kBPL;  // 16 KiB
+constexpr int	   kCapP	 = 3072;		   // staged structural outputs per row
+constexpr int	   kCapB	 = 768;			   // staged bracket outputs per row
+
+template<class E, int W>
 __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 	constexpr int ND  = E::kND;
 	constexpr int NDA = ND > 0 ? ND : 1;
 	using M			  = MapT<E::kWide>;
 	using Map		  = typename M::type;
+	constexpr uint64_t kTileBytes = (uint64_t)W * kRowBytes;
 
 	__shared__ typename E::Lds s_lds;
-	__shared__ Map			   s_wagg[4][NDA];
+	__shared__ Map			   s_wagg[W][4][NDA];
 	__shared__ uint32_t		   s_misc[8];
-	__shared__ uint32_t		   s_wcnt[4][3];
+	__shared__ uint32_t		   s_wcnt[W][4][3];
 	__shared__ int32_t		   s_mm[4][2];
+	__shared__ uint32_t		   s_pos[kCapP];
+	__shared__ uint32_t		   s_bpos[kCapB];
+	__shared__ uint32_t		   s_bsidx[kCapB];
+	__shared__ int32_t		   s_bkey[kCapB];
+	__shared__ uint8_t		   s_bch[kCapB];
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-	if (tid == 0) {
-		const uint32_t t = atomicAdd(a.ticket, 1u);
-		if (t == a.n_tiles - 1)
-			atomicExch(a.ticket, 0u);  // every block of this launch has drawn its ticket
-		s_misc[0] = t;
-	}
 	E::load(s_lds, a.plan);
-	__syncthreads();
-
-	const uint32_t tile	 = s_misc[0];
-	const uint64_t base	 = (uint64_t)tile * kTile + (uint64_t)tid * kBPL;
-	const uint32_t valid = base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
-	uint64_t*	   rec	 = a.recs + (uint64_t)tile * kRecWords;
-
-	// ---- load 64 bytes (the tail lane reads only what exists) ----
-	uint32_t   x[16];
 	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
-	if (valid == kBPL && aligned) {
-		const uint4* p = reinterpret_cast<const uint4*>(a.data + base);
-#pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			const uint4 v = p[k];
-			x[4 * k + 0]  = v.x;
-			x[4 * k + 1]  = v.y;
-			x[4 * k + 2]  = v.z;
-			x[4 * k + 3]  = v.w;
-		}
-	} else {
-#pragma unroll
-		for (int k = 0; k < 16; ++k) {
-			uint32_t w = 0;
-#pragma unroll
-			for (int b = 0; b < 4; ++b)
-				if ((uint32_t)(4 * k + b) < valid)
-					w |= (uint32_t)a.data[base + 4 * k + b] << (8 * b);
-			x[k] = w;
-		}
-	}
 
-	// ---- 1. one walk ----
-	E ev;
-	ev.walk(s_lds, a.plan, x, valid);
-
-	// ---- 2. chain 1: entry states ----
-	uint32_t tile_in = a.in_states;
-	Map		 lpre[NDA];
-	if constexpr (ND > 0) {
-		Map inc[ND], exc[ND];
-#pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			inc[d] = wave_scan_maps<E::kWide>(ev.T[d]);
-			exc[d] = M::shfl_up(inc[d], 1);
-			if (lane == 0)
-				exc[d] = M::id();
-		}
-		if (lane == 63)
-#pragma unroll
-			for (int d = 0; d < ND; ++d) s_wagg[wid][d] = inc[d];
+	while (true) {
+		__syncthreads();  // LDS of the previous tile is free
+		if (tid == 0)
+			s_misc[0] = atomicAdd(a.ticket, 1u);
 		__syncthreads();
-		Map tagg[ND];
+		const uint32_t tile = s_misc[0];
+		if (tile >= a.n_tiles)
+			break;
+		const uint64_t tbase = (uint64_t)tile * kTileBytes;
+		uint64_t*	   rec	 = a.recs + (uint64_t)tile * kRecWords;
+
+		// ---- 1. one walk per row ----
+		E		 ev[W];
+		uint32_t valid[W];
 #pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			Map wp = M::id();
+		for (int r = 0; r < W; ++r) {
+			const uint64_t base = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
+			valid[r]			= base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
+			uint32_t x[16];
+			if (valid[r] == kBPL && aligned) {
+				const uint4* p = reinterpret_cast<const uint4*>(a.data + base);
 #pragma unroll
-			for (int w2 = 0; w2 < 4; ++w2) {
-				if (w2 == wid)
-					lpre[d] = M::compose(exc[d], wp);
-				wp = M::compose(s_wagg[w2][d], wp);
+				for (int k = 0; k < 4; ++k) {
+					const uint4 v = p[k];
+					x[4 * k + 0]  = v.x;
+					x[4 * k + 1]  = v.y;
+					x[4 * k + 2]  = v.z;
+					x[4 * k + 3]  = v.w;
+				}
+			} else {
+#pragma unroll
+				for (int k = 0; k < 16; ++k) {
+					uint32_t w = 0;
+#pragma unroll
+					for (int b = 0; b < 4; ++b)
+						if ((uint32_t)(4 * k + b) < valid[r])
+							w |= (uint32_t)a.data[base + 4 * k + b] << (8 * b);
+					x[k] = w;
+				}
 			}
-			tagg[d] = wp;
+			ev[r].walk(s_lds, a.plan, x, valid[r]);
 		}
+
+		// ---- 2. chain 1: entry states ----
+		uint32_t tile_in = a.in_states;
+		Map		 lpre[W][NDA];
+		uint32_t row_in[W];
+		if constexpr (ND > 0) {
+			Map exc[W][ND];
+#pragma unroll
+			for (int r = 0; r < W; ++r)
+#pragma unroll
+				for (int d = 0; d < ND; ++d) {
+					const Map inc = wave_scan_maps<E::kWide>(ev[r].T[d]);
+					exc[r][d]	  = M::shfl_up(inc, 1);
+					if (lane == 0)
+						exc[r][d] = M::id();
+					if (lane == 63)
+						s_wagg[r][wid][d] = inc;
+				}
+			__syncthreads();
+			Map ragg[W][ND], tagg[ND];
+#pragma unroll
+			for (int d = 0; d < ND; ++d) {
+				tagg[d] = M::id();
+#pragma unroll
+				for (int r = 0; r < W; ++r) {
+					Map wp = M::id();
+#pragma unroll
+					for (int w2 = 0; w2 < 4; ++w2) {
+						if (w2 == wid)
+							lpre[r][d] = M::compose(exc[r][d], wp);
+						wp = M::compose(s_wagg[r][w2][d], wp);
+					}
+					ragg[r][d] = wp;
+					tagg[d]	   = M::compose(wp, tagg[d]);
+				}
+			}
+			if (tile != 0) {
+				if (tid == 0)
+#pragma unroll
+					for (int d = 0; d < ND; ++d)
+						publish(rec + kRecSAgg + d, make_word(a.epoch, kAgg, M::pack(tagg[d])));
+				if (wid == 0) {
+					const uint32_t st = lookback_states<ND, E::kWide>(a.recs, tile, a.epoch, a.in_states, a.sum);
+					if (lane == 0)
+						s_misc[1] = st;
+				}
+				__syncthreads();
+				tile_in = s_misc[1];
+			}
+			uint32_t cur = tile_in;
+#pragma unroll
+			for (int r = 0; r < W; ++r) {
+				row_in[r]	 = cur;
+				uint32_t nxt = 0;
+#pragma unroll
+				for (int d = 0; d < ND; ++d) nxt |= M::apply(ragg[r][d], (cur >> (3 * d)) & 7u) << (3 * d);
+				cur = nxt;
+			}
+			if (tid == 0) {
+				publish(rec + kRecSIncl, make_word(a.epoch, kIncl, cur));
+				if (tile == a.n_tiles - 1)
+					a.sum->final_states = cur;
+			}
+		} else {
+#pragma unroll
+			for (int r = 0; r < W; ++r) row_in[r] = 0;
+			if (tid == 0 && tile == a.n_tiles - 1)
+				a.sum->final_states = 0;
+		}
+
+		// ---- 3. masks and counts ----
+		uint64_t sm[W], om[W], opm[W];
+		uint32_t cnt[W][3];
+#pragma unroll
+		for (int r = 0; r < W; ++r) {
+			uint32_t s0[NDA];
+#pragma unroll
+			for (int d = 0; d < ND; ++d) s0[d] = M::apply(lpre[r][d], (row_in[r] >> (3 * d)) & 7u);
+			const uint64_t masked = ND > 0 ? ev[r].masked(s0) : 0ull;
+			uint64_t	   S, OC, OP;
+			ev[r].classes(S, OC, OP);
+			sm[r]  = S & ~masked;
+			om[r]  = OC & ~masked;
+			opm[r] = OP & ~masked;
+			if (a.bitmap && valid[r])
+				a.bitmap[(tbase + (uint64_t)r * kRowBytes) / 64 + tid] = sm[r];
+			cnt[r][0] = (uint32_t)__popcll(sm[r]);
+			cnt[r][1] = (uint32_t)__popcll(om[r]);
+			cnt[r][2] = 2u * (uint32_t)__popcll(opm[r]) - cnt[r][1];
+		}
+		uint32_t cpre[W][3], rtot[W][3], ctot[3] = {0u, 0u, 0u};
+		{
+			uint32_t cinc[W][3];
+#pragma unroll
+			for (int r = 0; r < W; ++r)
+#pragma unroll
+				for (int f = 0; f < 3; ++f) {
+					cinc[r][f] = wave_incl_sum(cnt[r][f]);
+					if (lane == 63)
+						s_wcnt[r][wid][f] = cinc[r][f];
+				}
+			__syncthreads();
+#pragma unroll
+			for (int r = 0; r < W; ++r)
+#pragma unroll
+				for (int f = 0; f < 3; ++f) {
+					uint32_t p = 0, t = 0;
+#pragma unroll
+					for (int w2 = 0; w2 < 4; ++w2) {
+						const uint32_t v = s_wcnt[r][w2][f];
+						if (w2 < wid)
+							p += v;
+						t += v;
+					}
+					cpre[r][f] = p + cinc[r][f] - cnt[r][f];
+					rtot[r][f] = t;
+					ctot[f] += t;
+				}
+		}
+
+		// chain 2: tile output offsets
+		const uint32_t cbase[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
+		uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
 		if (tile != 0) {
 			if (tid == 0)
 #pragma unroll
-				for (int d = 0; d < ND; ++d)
-					publish(rec + kRecSAgg + d, make_word(a.epoch, kAgg, M::pack(tagg[d])));
+				for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
 			if (wid == 0) {
-				const uint32_t st = lookback_states<ND, E::kWide>(a.recs, tile, a.epoch, a.in_states, a.sum);
-				if (lane == 0)
-					s_misc[1] = st;
+				uint32_t o[3];
+				lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
+				if (lane == 0) {
+					s_misc[2] = o[0];
+					s_misc[3] = o[1];
+					s_misc[4] = o[2];
+				}
 			}
 			__syncthreads();
-			tile_in = s_misc[1];
+			cex[0] = s_misc[2];
+			cex[1] = s_misc[3];
+			cex[2] = s_misc[4];
 		}
 		if (tid == 0) {
-			uint32_t out = 0;
 #pragma unroll
-			for (int d = 0; d < ND; ++d) out |= M::apply(tagg[d], (tile_in >> (3 * d)) & 7u) << (3 * d);
-			publish(rec + kRecSIncl, make_word(a.epoch, kIncl, out));
-			if (tile == a.n_tiles - 1)
-				a.sum->final_states = out;
-		}
-	} else {
-		if (tid == 0 && tile == a.n_tiles - 1)
-			a.sum->final_states = 0;
-	}
-
-	// ---- 3. masks and counts ----
-	uint32_t s0[NDA];
-#pragma unroll
-	for (int d = 0; d < ND; ++d) s0[d] = M::apply(lpre[d], (tile_in >> (3 * d)) & 7u);
-	const uint64_t masked = ND > 0 ? ev.masked(s0) : 0ull;
-	uint64_t	   S, OC, OP;
-	ev.classes(S, OC, OP);
-	const uint64_t sm = S & ~masked, om = OC & ~masked, opm = OP & ~masked;
-	if (a.bitmap && valid)
-		a.bitmap[base >> 6] = sm;
-
-	const uint32_t cnt[3] = {(uint32_t)__popcll(sm), (uint32_t)__popcll(om),
-							 2u * (uint32_t)__popcll(opm) - (uint32_t)__popcll(om)};
-	uint32_t	   cinc[3];
-#pragma unroll
-	for (int f = 0; f < 3; ++f) cinc[f] = wave_incl_sum(cnt[f]);
-	if (lane == 63)
-#pragma unroll
-		for (int f = 0; f < 3; ++f) s_wcnt[wid][f] = cinc[f];
-	__syncthreads();
-	uint32_t cpre[3], ctot[3];
-#pragma unroll
-	for (int f = 0; f < 3; ++f) {
-		uint32_t p = 0, t = 0;
-#pragma unroll
-		for (int w2 = 0; w2 < 4; ++w2) {
-			const uint32_t v = s_wcnt[w2][f];
-			if (w2 < wid)
-				p += v;
-			t += v;
-		}
-		cpre[f] = p + cinc[f] - cnt[f];
-		ctot[f] = t;
-	}
-
-	// chain 2: tile output offsets
-	const uint32_t cbase[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
-	uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
-	if (tile != 0) {
-		if (tid == 0)
-#pragma unroll
-			for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
-		if (wid == 0) {
-			uint32_t o[3];
-			lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
-			if (lane == 0) {
-				s_misc[2] = o[0];
-				s_misc[3] = o[1];
-				s_misc[4] = o[2];
+			for (int f = 0; f < 3; ++f) publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
+			if (tile == a.n_tiles - 1) {
+				a.sum->n_struct	  = cex[0] + ctot[0];
+				a.sum->n_brackets = cex[1] + ctot[1];
+				a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
 			}
+		}
+		if (a.pair_pos)	 // -1 for every structural index of the tile; R6 overwrites brackets
+			for (uint32_t i = tid; i < ctot[0]; i += kBlock)
+				if ((uint64_t)cex[0] + i < a.cap_struct)
+					a.pair_pos[cex[0] + i] = -1;
+
+		// ---- 4. scatter, row by row ----
+		uint32_t rb[3] = {cex[0], cex[1], cex[2]};
+		int32_t	 lmin = INT_MAX, lmax = INT_MIN;
+#pragma unroll
+		for (int r = 0; r < W; ++r) {
+			const uint64_t rowb	  = tbase + (uint64_t)r * kRowBytes;
+			const uint64_t lbase  = rowb + (uint64_t)tid * kBPL;
+			const bool	   staged = rtot[r][0] <= (uint32_t)kCapP && rtot[r][1] <= (uint32_t)kCapB;
+			uint32_t	   li = cpre[r][0], lb = cpre[r][1];
+			int32_t		   bal	= (int32_t)(rb[2] + cpre[r][2]);
+			const uint64_t pbase = a.byte_base + lbase;
+			uint64_t	   m	 = sm[r];
+			while (m) {
+				const int	   p   = __builtin_ctzll(m);
+				const uint32_t pos = (uint32_t)(pbase + (uint64_t)p);
+				const bool	   br  = (om[r] >> p) & 1ull;
+				uint8_t		   ch  = 0;
+				if (br || a.chars)
+					ch = a.data[lbase + p];
+				if (a.chars && (uint64_t)rb[0] + li < a.cap_struct)
+					a.chars[rb[0] + li] = ch;
+				if (staged)
+					s_pos[li] = pos;
+				else if ((uint64_t)rb[0] + li < a.cap_struct)
+					a.positions[rb[0] + li] = pos;
+				if (br) {
+					const bool op = (opm[r] >> p) & 1ull;
+					bal += op ? 1 : -1;
+					lmin			  = min(lmin, bal);
+					lmax			  = max(lmax, bal);
+					const int32_t key = op ? bal - 1 : bal;
+					if (staged) {
+						s_bpos[lb]	= pos;
+						s_bsidx[lb] = rb[0] + li;
+						s_bkey[lb]	= key;
+						s_bch[lb]	= ch;
+					} else if ((uint64_t)rb[1] + lb < a.cap_br) {
+						const uint32_t g = rb[1] + lb;
+						a.bpos[g]		 = pos;
+						a.bsidx[g]		 = rb[0] + li;
+						a.bkey[g]		 = key;
+						a.bchar[g]		 = ch;
+					}
+					++lb;
+				}
+				++li;
+				m &= m - 1;
+			}
+			if (staged) {  // block-uniform
+				__syncthreads();
+				const uint32_t np = rtot[r][0], nb = rtot[r][1];
+				const uint64_t lim_p = a.cap_struct > rb[0] ? a.cap_struct - rb[0] : 0;
+				const uint64_t lim_b = a.cap_br > rb[1] ? a.cap_br - rb[1] : 0;
+				for (uint32_t i = tid; i < np && i < lim_p; i += kBlock) a.positions[rb[0] + i] = s_pos[i];
+				for (uint32_t i = tid; i < nb && i < lim_b; i += kBlock) {
+					a.bpos[rb[1] + i]  = s_bpos[i];
+					a.bsidx[rb[1] + i] = s_bsidx[i];
+					a.bkey[rb[1] + i]  = s_bkey[i];
+					a.bchar[rb[1] + i] = s_bch[i];
+				}
+				__syncthreads();
+			}
+#pragma unroll
+			for (int f = 0; f < 3; ++f) rb[f] += rtot[r][f];
+		}
+		lmin = wave_min(lmin);
+		lmax = wave_max(lmax);
+		if (lane == 0) {
+			s_mm[wid][0] = lmin;
+			s_mm[wid][1] = lmax;
 		}
 		__syncthreads();
-		cex[0] = s_misc[2];
-		cex[1] = s_misc[3];
-		cex[2] = s_misc[4];
-	}
-	if (tid == 0) {
+		if (tid == 0) {
+			int32_t mn = s_mm[0][0], mx = s_mm[0][1];
 #pragma unroll
-		for (int f = 0; f < 3; ++f) publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
-		if (tile == a.n_tiles - 1) {
-			a.sum->n_struct	  = cex[0] + ctot[0];
-			a.sum->n_brackets = cex[1] + ctot[1];
-			a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
-		}
-	}
-
-	// ---- 4. scatter ----
-	uint32_t	   si = cex[0] + cpre[0], bi = cex[1] + cpre[1];
-	int32_t		   bal	= (int32_t)(cex[2] + cpre[2]);
-	int32_t		   lmin = INT_MAX, lmax = INT_MIN;
-	const uint64_t pbase = a.byte_base + base;
-#pragma unroll
-	for (int q = 0; q < 16; ++q) {
-		uint32_t m = (uint32_t)(sm >> (4 * q)) & 0xFu;
-		while (m) {
-			const int	   b   = __builtin_ctz(m);
-			const int	   p   = 4 * q + b;
-			const uint32_t pos = (uint32_t)(pbase + (uint64_t)p);
-			const uint8_t  ch  = (uint8_t)(x[q] >> (8 * b));
-			if (si < a.cap_struct) {
-				a.positions[si] = pos;
-				if (a.chars)
-					a.chars[si] = ch;
-				if (a.pair_pos)
-					a.pair_pos[si] = -1;
+			for (int w2 = 1; w2 < 4; ++w2) {
+				mn = min(mn, s_mm[w2][0]);
+				mx = max(mx, s_mm[w2][1]);
 			}
-			if ((om >> p) & 1ull) {
-				const bool op = (opm >> p) & 1ull;
-				bal += op ? 1 : -1;
-				lmin = min(lmin, bal);
-				lmax = max(lmax, bal);
-				if (bi < a.cap_br) {
-					a.bpos[bi]	= pos;
-					a.bsidx[bi] = si;
-					a.bkey[bi]	= op ? bal - 1 : bal;
-					a.bchar[bi] = ch;
-				}
-				++bi;
-			}
-			++si;
-			m &= m - 1;
+			if (mn != INT_MAX)
+				atomicMin(&a.sum->min_bal, mn);
+			if (mx != INT_MIN)
+				atomicMax(&a.sum->max_bal, mx);
 		}
 	}
-	lmin = wave_min(lmin);
-	lmax = wave_max(lmax);
-	if (lane == 0) {
-		s_mm[wid][0] = lmin;
-		s_mm[wid][1] = lmax;
-	}
-	__syncthreads();
-	if (tid == 0) {
-		int32_t mn = s_mm[0][0], mx = s_mm[0][1];
-#pragma unroll
-		for (int w2 = 1; w2 < 4; ++w2) {
-			mn = min(mn, s_mm[w2][0]);
-			mx = max(mx, s_mm[w2][1]);
-		}
-		if (mn != INT_MAX)
-			atomicMin(&a.sum->min_bal, mn);
-		if (mx != INT_MIN)
-			atomicMax(&a.sum->max_bal, mx);
+	// every block has drawn a ticket past the end: the last one out resets
+	if (tid == 0 && atomicAdd(a.done, 1u) == gridDim.x - 1) {
+		atomicExch(a.ticket, 0u);
+		atomicExch(a.done, 0u);
 	}
 }
 

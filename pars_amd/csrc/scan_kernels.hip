@@ -270,14 +270,85 @@ struct EscapeEval {
 		mO = hO.P;
 		mI = hI.P;
 		mE = hE.P;
-		const uint32_t fk[4] = {f0, f1, f2, f3};
-		const uint32_t tO = fk[esc_code(hO, BS, last)], tI = fk[esc_code(hI, BS, last)],
-					   tE = fk[esc_code(hE, BS, last)];
+		const uint32_t fk = f0 | (f1 << 8) | (f2 << 16) | (f3 << 24);  // code -> plan state
+		const uint32_t tO = (fk >> (8 * esc_code(hO, BS, last))) & 0xFFu;
+		const uint32_t tI = (fk >> (8 * esc_code(hI, BS, last))) & 0xFFu;
+		const uint32_t tE = (fk >> (8 * esc_code(hE, BS, last))) & 0xFFu;
 		T[0].lo = (tO << (8 * f0)) | (tI << (8 * f1)) | (tO << (8 * f2)) | (tE << (8 * f3));
 	}
 
 	__device__ uint64_t masked(const uint32_t (&s0)[1]) const {
-		return s0[0] == st_in ? mI : (s0[0] == st_esc ? mE : mO);
+		// blend with masks, not a select (a select of members lowers to scratch)
+		const uint64_t kI = 0ull - (uint64_t)(s0[0] == st_in), kE = 0ull - (uint64_t)(s0[0] == st_esc);
+		return (mO & ~(kI | kE)) | (mI & kI) | (mE & kE);
+	}
+
+	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {
+		s  = S;
+		oc = OC;
+		op = OP;
+	}
+};
+
+// ---------------------------------------------------------------------------
+// ParityEval: the three-state quote-parity automaton (CSV doubled quotes,
+// escape-less strings).  The state before byte i is `in` exactly when an odd
+// number of delimiters precede it; `closed` when the parity is even and byte
+// i-1 is a delimiter.  One prefix-xor ladder serves every entry state.
+// ---------------------------------------------------------------------------
+struct ParityEval {
+	static constexpr int  kND	= 1;
+	static constexpr bool kWide = false;
+	using Map					= Map4;
+	using Lds					= EscapeEval::Lds;
+
+	__device__ static void load(Lds& L, const DevPlan* P) { EscapeEval::load(L, P); }
+
+	Map		 T[1];
+	uint64_t S, OC, OP, Q, Pe;
+	uint32_t f_in, f_cl, doubled;
+
+	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
+		uint32_t clo[8];
+#pragma unroll
+		for (int g = 0; g < 8; ++g) clo[g] = 0;
+		auto step = [&](int j) {
+			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+			clo[j >> 3] |= L.cls[(j & 7) * 256 + c].x;
+		};
+		if (valid == kBPL) {
+#pragma unroll
+			for (int j = 0; j < kBPL; ++j) step(j);
+		} else {
+#pragma unroll
+			for (int j = 0; j < kBPL; ++j)
+				if ((uint32_t)j < valid)
+					step(j);
+		}
+		gather_classes(clo, S, OC, OP, Q);
+		const uint32_t f_out = plan->fk_state[0];
+		f_in				 = plan->fk_state[1];
+		f_cl				 = plan->fk_state[2];
+		doubled				 = plan->fk_doubled;
+		Pe					 = prefix_xor(Q) ^ Q;  // parity before each byte, entered even
+		if (valid == 0) {
+			T[0].lo = kIdLo;
+			return;
+		}
+		const uint64_t last	  = 1ull << (valid - 1);
+		const bool	   ql	  = (Q & last) != 0;
+		const bool	   odd_ev = ((Pe & last) != 0) != ql;  // parity after the lane, entered even
+		const uint32_t fe	  = odd_ev ? f_in : (ql ? f_cl : f_out);
+		const uint32_t fo	  = !odd_ev ? f_in : (ql ? f_cl : f_out);
+		T[0].lo = (fe << (8 * f_out)) | (fo << (8 * f_in)) | (fe << (8 * f_cl)) | (3u << 24);
+	}
+
+	__device__ uint64_t masked(const uint32_t (&s0)[1]) const {
+		const uint64_t odd = s0[0] == f_in ? ~Pe : Pe;
+		if (!doubled)
+			return odd;
+		const uint64_t prevq = (Q << 1) | (s0[0] == f_cl ? 1ull : 0ull);
+		return odd | (Q & prevq);
 	}
 
 	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {
@@ -303,9 +374,29 @@ __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 	}
 }
 
-template<class E>
-hipError_t launch_eval(const ScanArgs& a, hipStream_t s) {
-	hipLaunchKernelGGL((k_scan<E>), dim3(a.n_tiles), dim3(kBlock), 0, s, a);
+// Persistent launch: as many blocks as fit on the device at once (never more
+// than there are tiles); each draws tiles until none is left.
+template<class E, int W>
+hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
+	static int resident = 0;  // blocks resident at once, per kernel instance
+	if (!resident) {
+		int dev = 0, cus = 0, per_cu = 0;
+		hipError_t e = hipGetDevice(&dev);
+		if (e == hipSuccess)
+			e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+		if (e == hipSuccess)
+			e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<E, W>, kBlock, 0);
+		if (e != hipSuccess)
+			return e;
+		resident = (cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+	}
+	ScanArgs a			  = in;
+	const uint64_t tbytes = (uint64_t)W * kTile;
+	a.n_tiles			  = (uint32_t)((a.n + tbytes - 1) / tbytes);
+	if (a.n_tiles == 0)
+		return hipSuccess;
+	const uint32_t grid = a.n_tiles < (uint32_t)resident ? a.n_tiles : (uint32_t)resident;
+	hipLaunchKernelGGL((k_scan<E, W>), dim3(grid), dim3(kBlock), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -316,37 +407,54 @@ hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStrea
 	return hipGetLastError();
 }
 
-hipError_t launch_scan_escape(const ScanArgs& a, hipStream_t s) {
-	if (a.n_tiles == 0)
-		return hipSuccess;
-	return launch_eval<EscapeEval>(a, s);
+template<class E>
+hipError_t launch_rows(const ScanArgs& a, int rows, hipStream_t s) {
+	switch (rows) {
+	case 1: return launch_eval<E, 1>(a, s);
+	case 4: return launch_eval<E, 4>(a, s);
+	default: return launch_eval<E, 2>(a, s);
+	}
 }
 
-hipError_t launch_scan_general(const ScanArgs& a, int nd, bool wide, hipStream_t s) {
-	if (a.n_tiles == 0)
+hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s) {
+	if (a.n == 0)
 		return hipSuccess;
+	return launch_rows<EscapeEval>(a, rows, s);
+}
+
+hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s) {
+	if (a.n == 0)
+		return hipSuccess;
+	return launch_rows<ParityEval>(a, rows, s);
+}
+
+hipError_t launch_scan_general(const ScanArgs& a, int nd, bool wide, int rows, hipStream_t s) {
+	if (a.n == 0)
+		return hipSuccess;
+	if (!wide && nd == 1)
+		return launch_rows<GeneralEval<1, false>>(a, rows, s);
 	if (!wide) {
 		switch (nd) {
-		case 0: return launch_eval<GeneralEval<0, false>>(a, s);
-		case 1: return launch_eval<GeneralEval<1, false>>(a, s);
-		case 2: return launch_eval<GeneralEval<2, false>>(a, s);
-		case 3: return launch_eval<GeneralEval<3, false>>(a, s);
-		case 4: return launch_eval<GeneralEval<4, false>>(a, s);
-		case 5: return launch_eval<GeneralEval<5, false>>(a, s);
-		case 6: return launch_eval<GeneralEval<6, false>>(a, s);
-		case 7: return launch_eval<GeneralEval<7, false>>(a, s);
-		case 8: return launch_eval<GeneralEval<8, false>>(a, s);
+		case 0: return launch_eval<GeneralEval<0, false>, 2>(a, s);
+		case 1: return launch_eval<GeneralEval<1, false>, 2>(a, s);
+		case 2: return launch_eval<GeneralEval<2, false>, 1>(a, s);
+		case 3: return launch_eval<GeneralEval<3, false>, 1>(a, s);
+		case 4: return launch_eval<GeneralEval<4, false>, 1>(a, s);
+		case 5: return launch_eval<GeneralEval<5, false>, 1>(a, s);
+		case 6: return launch_eval<GeneralEval<6, false>, 1>(a, s);
+		case 7: return launch_eval<GeneralEval<7, false>, 1>(a, s);
+		case 8: return launch_eval<GeneralEval<8, false>, 1>(a, s);
 		}
 	} else {
 		switch (nd) {
-		case 1: return launch_eval<GeneralEval<1, true>>(a, s);
-		case 2: return launch_eval<GeneralEval<2, true>>(a, s);
-		case 3: return launch_eval<GeneralEval<3, true>>(a, s);
-		case 4: return launch_eval<GeneralEval<4, true>>(a, s);
-		case 5: return launch_eval<GeneralEval<5, true>>(a, s);
-		case 6: return launch_eval<GeneralEval<6, true>>(a, s);
-		case 7: return launch_eval<GeneralEval<7, true>>(a, s);
-		case 8: return launch_eval<GeneralEval<8, true>>(a, s);
+		case 1: return launch_eval<GeneralEval<1, true>, 1>(a, s);
+		case 2: return launch_eval<GeneralEval<2, true>, 1>(a, s);
+		case 3: return launch_eval<GeneralEval<3, true>, 1>(a, s);
+		case 4: return launch_eval<GeneralEval<4, true>, 1>(a, s);
+		case 5: return launch_eval<GeneralEval<5, true>, 1>(a, s);
+		case 6: return launch_eval<GeneralEval<6, true>, 1>(a, s);
+		case 7: return launch_eval<GeneralEval<7, true>, 1>(a, s);
+		case 8: return launch_eval<GeneralEval<8, true>, 1>(a, s);
 		}
 	}
 	return hipErrorInvalidValue;

@@ -91,6 +91,23 @@ def test_generated_documents_multi_tile(torch_cuda, g, gen):
         compare(g, d, gpu_full(torch_cuda, scanner(torch_cuda, g), d), f"n={n}")
 
 
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("g", ["json", "csv", "clike6"])
+def test_rows_per_tile_and_paths(torch_cuda, g, rows, path):
+    """Every tile shape and both evaluators (fast path, general walk) agree with the oracle."""
+    sc = Scanner(PLANS[g])
+    sc.set_option(N.PARS_OPT_WANT_CHARS, 1)
+    sc.set_option(N.PARS_OPT_ROWS_PER_TILE, rows)
+    sc.set_option(N.PARS_OPT_PATH, path)
+    gen = {"json": lambda s, n: W.json(s, n, 12, 0.3), "csv": W.csv, "clike6": W.clike}[g]
+    for n in (100, 3 * 16384 + 5, (1 << 20) + 77, 3 << 20):
+        for d in (gen(n, n), W.alpha(n + 1, n, W.ALPHABETS[g])):
+            say(f"[gpu] rows={rows} path={path} {g} n={n}")
+            compare(g, d, gpu_full(torch_cuda, sc, d), f"rows={rows} path={path} n={n}")
+    sc.close()
+
+
 def test_unaligned_inputs(torch_cuda):
     d = W.json(31, 200_000)
     for off in (1, 3, 5, 8, 13):

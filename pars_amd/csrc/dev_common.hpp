@@ -9,6 +9,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 namespace pars_hip {
@@ -23,6 +24,22 @@ constexpr int kWave = 64;
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel) {
 	return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// DPP lane moves (GFX9 encodings): row_shr:n = 0x110 + n (within 16-lane
+// rows), wave_shr:1 = 0x138, row_bcast:15 = 0x142 (lane 15 of each row to the
+// next row), row_bcast:31 = 0x143 (lane 31 to rows 2 and 3).  Lanes whose source
+// is outside the row, or whose row is masked off, keep `old`.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppWaveShr1 = 0x138, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template<int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t src) {
+	return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, CTRL, ROW_MASK, 0xF, false);
+}
+
+__device__ __forceinline__ uint32_t readlane(uint32_t v, int lane) {
+	return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 
 struct Map4 {
@@ -53,9 +70,12 @@ struct MapT<false> {
 
 	__device__ static Map4 shfl(Map4 m, int src) { return Map4 {(uint32_t)__shfl((int)m.lo, src, kWave)}; }
 
-	__device__ static Map4 shfl_down(Map4 m, int off) {
-		return Map4 {(uint32_t)__shfl_down((int)m.lo, off, kWave)};
+	template<int CTRL, int ROW_MASK = 0xF>
+	__device__ static Map4 dpp(Map4 m) {
+		return Map4 {pars_hip::dpp<CTRL, ROW_MASK>(kIdLo, m.lo)};
 	}
+
+	__device__ static Map4 readlane(Map4 m, int lane) { return Map4 {pars_hip::readlane(m.lo, lane)}; }
 
 	// 3-bit packed form used in look-back words (24 bits for 8 states)
 	__device__ static uint32_t pack(Map4 m) {
@@ -97,8 +117,13 @@ struct MapT<true> {
 		return Map8 {(uint32_t)__shfl((int)m.lo, src, kWave), (uint32_t)__shfl((int)m.hi, src, kWave)};
 	}
 
-	__device__ static Map8 shfl_down(Map8 m, int off) {
-		return Map8 {(uint32_t)__shfl_down((int)m.lo, off, kWave), (uint32_t)__shfl_down((int)m.hi, off, kWave)};
+	template<int CTRL, int ROW_MASK = 0xF>
+	__device__ static Map8 dpp(Map8 m) {
+		return Map8 {pars_hip::dpp<CTRL, ROW_MASK>(kIdLo, m.lo), pars_hip::dpp<CTRL, ROW_MASK>(kIdHi, m.hi)};
+	}
+
+	__device__ static Map8 readlane(Map8 m, int lane) {
+		return Map8 {pars_hip::readlane(m.lo, lane), pars_hip::readlane(m.hi, lane)};
 	}
 
 	__device__ static uint32_t pack(Map8 m) {
@@ -159,43 +184,64 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 	return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-template<class T>
-__device__ __forceinline__ T wave_incl_sum(T v) {
-	const int l = lane_id();
-#pragma unroll
-	for (int off = 1; off < kWave; off <<= 1) {
-		T o = __shfl_up(v, off, kWave);
-		if (l >= off)
-			v += o;
-	}
+// Inclusive wave scans on DPP: four row_shr steps scan each 16-lane row, two
+// row broadcasts carry rows 0..2 into the rows above.  `old` of every move is
+// the operation's identity, so lanes without a source combine with it.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+	v += dpp<kDppRowShr1>(0u, v);
+	v += dpp<kDppRowShr2>(0u, v);
+	v += dpp<kDppRowShr4>(0u, v);
+	v += dpp<kDppRowShr8>(0u, v);
+	v += dpp<kDppBcast15, 0xA>(0u, v);
+	v += dpp<kDppBcast31, 0xC>(0u, v);
 	return v;
+}
+
+// Sum over the wave, the same value in every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+	return readlane(wave_incl_sum(v), kWave - 1);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
-	return v;
+	uint32_t u = (uint32_t)v;
+	u = (uint32_t)min((int)u, (int)dpp<kDppRowShr1>((uint32_t)INT_MAX, u));
+	u = (uint32_t)min((int)u, (int)dpp<kDppRowShr2>((uint32_t)INT_MAX, u));
+	u = (uint32_t)min((int)u, (int)dpp<kDppRowShr4>((uint32_t)INT_MAX, u));
+	u = (uint32_t)min((int)u, (int)dpp<kDppRowShr8>((uint32_t)INT_MAX, u));
+	u = (uint32_t)min((int)u, (int)dpp<kDppBcast15, 0xA>((uint32_t)INT_MAX, u));
+	u = (uint32_t)min((int)u, (int)dpp<kDppBcast31, 0xC>((uint32_t)INT_MAX, u));
+	return (int)readlane(u, kWave - 1);
 }
 
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
-	return v;
+	uint32_t u = (uint32_t)v;
+	u = (uint32_t)max((int)u, (int)dpp<kDppRowShr1>((uint32_t)INT_MIN, u));
+	u = (uint32_t)max((int)u, (int)dpp<kDppRowShr2>((uint32_t)INT_MIN, u));
+	u = (uint32_t)max((int)u, (int)dpp<kDppRowShr4>((uint32_t)INT_MIN, u));
+	u = (uint32_t)max((int)u, (int)dpp<kDppRowShr8>((uint32_t)INT_MIN, u));
+	u = (uint32_t)max((int)u, (int)dpp<kDppBcast15, 0xA>((uint32_t)INT_MIN, u));
+	u = (uint32_t)max((int)u, (int)dpp<kDppBcast31, 0xC>((uint32_t)INT_MIN, u));
+	return (int)readlane(u, kWave - 1);
 }
 
 // Inclusive wave scan of maps in lane order: lane i ends with
-// m_i . m_{i-1} . ... . m_0 (the earliest lane is applied first).
+// m_i . m_{i-1} . ... . m_0 (the lowest lane is applied first).
 template<bool WIDE>
 __device__ __forceinline__ typename MapT<WIDE>::type wave_scan_maps(typename MapT<WIDE>::type m) {
-	using M		= MapT<WIDE>;
-	const int l = lane_id();
-#pragma unroll
-	for (int off = 1; off < kWave; off <<= 1) {
-		auto o = M::shfl_up(m, off);
-		if (l >= off)
-			m = M::compose(m, o);
-	}
+	using M = MapT<WIDE>;
+	m		= M::compose(m, M::template dpp<kDppRowShr1>(m));
+	m		= M::compose(m, M::template dpp<kDppRowShr2>(m));
+	m		= M::compose(m, M::template dpp<kDppRowShr4>(m));
+	m		= M::compose(m, M::template dpp<kDppRowShr8>(m));
+	m		= M::compose(m, M::template dpp<kDppBcast15, 0xA>(m));
+	m		= M::compose(m, M::template dpp<kDppBcast31, 0xC>(m));
 	return m;
+}
+
+// The previous lane's value (identity in lane 0).
+template<bool WIDE>
+__device__ __forceinline__ typename MapT<WIDE>::type wave_shift_maps(typename MapT<WIDE>::type m) {
+	return MapT<WIDE>::template dpp<kDppWaveShr1>(m);
 }
 
 }  // namespace pars_hip

@@ -12,13 +12,17 @@ namespace pars_hip {
 constexpr int kBlock = 256;			   // threads per scan block (4 waves)
 constexpr int kBPL	 = 64;			   // bytes per lane and row
 constexpr int kTile	 = kBlock * kBPL;  // bytes per tile row (16 KiB); a tile holds 1..4 rows
-constexpr int kRecWords = 16;		   // look-back words per tile record
+constexpr int kRecWords = 32;		   // look-back words per tile record (256 bytes)
 
-// Record layout (u64 words): [0..7] state-chain aggregates, one per
-// descriptor (3-bit packed map); [8] state-chain inclusive (packed states);
-// [9..11] count aggregates (structural, brackets, depth); [12..14] count
-// inclusives.
+// Record layout, two-chain mode (u64 tagged words): [0..7] state-chain
+// aggregates, one per descriptor (3-bit packed map); [8] state-chain inclusive
+// (packed states); [9..11] count aggregates (structural, brackets, depth);
+// [12..14] count inclusives.
 enum : int { kRecSAgg = 0, kRecSIncl = 8, kRecCAgg = 9, kRecCIncl = 12 };
+// One-chain mode (one descriptor): [0] aggregate map; [1 + 3e + f] count f of
+// the tile entered in state e (e < 8); [28] inclusive state; [29..31]
+// inclusive counts.
+enum : int { kRecFMap = 0, kRecFCnt = 1, kRecFIncl = 28 };
 
 // Device-side result summary of one scan (written by the kernels, read back by
 // the host once at the end of a call).

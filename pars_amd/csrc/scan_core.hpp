@@ -7,11 +7,18 @@
 //      the lane's state map, plus enough to produce the lane's masked bits for
 //      any lane-entry state without a second walk;
 //   2. block scans of the lane maps per row; decoupled look-back over tiles on
-//      tagged 64-bit words gives the tile-entry states (chain 1);
+//      tagged 64-bit words gives the tile-entry states and output offsets;
 //   3. masks -> unmasked structural / bracket bits; counts and the +-1 depth
-//      delta; block scans + a second look-back (chain 2) give output offsets;
-//   4. uint32 positions (and bytes, bracket keys for R6) staged in LDS per row
-//      and written with coalesced stores; the bitmap word per lane.
+//      delta; block scans give each lane's output offsets inside the tile;
+//   4. row-local positions and bracket records staged in LDS per row and
+//      written with coalesced stores; the bitmap word per lane.
+//
+// Chains.  A plan with one descriptor (JSON, CSV, every single-automaton
+// grammar) runs ONE chain: a tile publishes its state map together with its
+// (structural, bracket, depth) counts for every possible tile-entry state, so
+// a single look-back yields both the entry state and the output offsets.
+// Plans with several descriptors have too many joint entry states for that and
+// run two chains: states first, then counts.
 //
 // Replaces the reference's classify_kernel, desc_wordt64_kernel,
 // thrust::inclusive_scan(TCompose64), desc_replay64_kernel,
@@ -31,14 +38,11 @@ namespace pars_hip {
 // publish before they wait, so it always ends; the spin bound is a guard so a
 // defect can never hang the device: past it the kernel flags the summary and
 // returns.
-constexpr uint32_t kSpinLimit	= 1u << 21;
+constexpr uint32_t kSpinLimit	= 1u << 20;
 constexpr uint32_t kErrLookback = 0x80000000u;
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-	for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, kWave);
-	return v;
-}
+// Predecessor records read per look-back round trip (the newest lanes of the
+// wave); the nearest inclusive prefix is typically 10-20 tiles back.
+constexpr int kLbWindow = 32;
 
 // 4x4 byte transpose: out[k] byte g = in[g] byte k.
 __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t& t0,
@@ -51,110 +55,143 @@ __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2
 	t3				  = perm(u3, u1, 0x07060302u);
 }
 
-// Chain 1 look-back, run by one full wave of tile `tile` > 0.  Returns the
-// packed per-descriptor states entering the tile.
+__device__ __forceinline__ uint32_t popc64(uint64_t x) {
+	return (uint32_t)__popcll(x);
+}
+
+// Set bits of x strictly below bit p (p < 64).
+__device__ __forceinline__ uint32_t popc_below(uint64_t x, int p) {
+	return popc64(x & ((1ull << p) - 1ull));
+}
+
+// v[s] for s < N, as an OR of masked values: a chain of selects over an
+// array is turned into a dynamically indexed load from scratch memory.
+template<int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&v)[N], uint32_t s) {
+	uint32_t r = 0;
+#pragma unroll
+	for (int q = 0; q < N; ++q) r |= v[q] & (0u - (uint32_t)(s == (uint32_t)q));
+	return r;
+}
+
+// ---------------------------------------------------------------------------
+// Look-backs run in one full wave.  Lane L >= 64 - kLbWindow reads the record
+// of tile jb - 63 + L, so lane 63 holds the nearest predecessor and folding a
+// window is a forward wave scan (lanes below the window are neutral); the
+// newest lane holding an inclusive prefix (kp) ends the walk.  While a
+// predecessor has not published, only the missing records are read again.
+// ---------------------------------------------------------------------------
+struct Window {
+	int		 kp;	  // newest lane with an inclusive prefix, -1 if none
+	uint64_t b_none;  // lanes whose record is not published yet
+};
+
+__device__ __forceinline__ Window window_extent(bool incl, bool none) {
+	const uint64_t b_incl = __ballot(incl);
+	Window		   w;
+	w.kp	 = b_incl ? 63 - (int)__clzll((long long)b_incl) : -1;
+	w.b_none = __ballot(none);
+	return w;
+}
+
+// Every lane newer than kp must have published before the window can fold.
+__device__ __forceinline__ bool window_ready(const Window& w) {
+	const uint64_t newer = w.kp >= 63 ? 0ull : (~0ull << (w.kp + 1));
+	return (w.b_none & newer) == 0;
+}
+
+// Two-chain mode, chain 1: packed per-descriptor states entering tile > 0.
 template<int ND, bool WIDE>
 __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_t epoch, uint32_t in_states,
 									DevSummary* sum) {
 	using M	  = MapT<WIDE>;
 	using Map = typename M::type;
-	const int lane = lane_id();
-	Map		  acc[ND];
+	const int  lane	  = lane_id();
+	const bool active = lane >= kWave - kLbWindow;
+	Map		   acc[ND];	 // windows folded so far: their entry -> the tile's entry
 #pragma unroll
 	for (int d = 0; d < ND; ++d) acc[d] = M::id();
 	int64_t	 jb	   = (int64_t)tile - 1;
 	uint32_t spins = 0;
+	bool	 load  = true, incl = false, none = false;
+	uint32_t ival  = 0;
+	Map		 agg[ND];
 	while (true) {
-		const int64_t j		 = jb - lane;
-		uint32_t	  status = 0, ival = 0;
-		Map			  agg[ND];
+		const int64_t j = jb - 63 + lane;
+		if (!active || j < 0) {	 // outside the window: neutral; before the input: the carry
+			incl = active;
+			none = false;
+			ival = in_states;
 #pragma unroll
-		for (int d = 0; d < ND; ++d) agg[d] = M::id();
-		if (j < 0) {
-			status = kIncl;
-			ival   = in_states;
-		} else {
+			for (int d = 0; d < ND; ++d) agg[d] = M::id();
+		} else if (load) {
 			// one round trip: the inclusive and every aggregate word together
 			const uint64_t* r = recs + (uint64_t)j * kRecWords;
 			const uint64_t	w = peek(r + kRecSIncl);
 			uint64_t		wa[ND];
 #pragma unroll
 			for (int d = 0; d < ND; ++d) wa[d] = peek(r + kRecSAgg + d);
-			if (word_is(w, epoch, kIncl)) {
-				status = kIncl;
-				ival   = (uint32_t)w;
-			} else {
-				status = kAgg;
+			incl = word_is(w, epoch, kIncl);
+			none = false;
+			ival = (uint32_t)w;
 #pragma unroll
-				for (int d = 0; d < ND; ++d) {
-					if (!word_is(wa[d], epoch, kAgg))
-						status = 0;
-					else
-						agg[d] = M::unpack((uint32_t)wa[d]);
-				}
+			for (int d = 0; d < ND; ++d) {
+				if (!word_is(wa[d], epoch, kAgg))
+					none = !incl;
+				agg[d] = M::unpack((uint32_t)wa[d]);
 			}
 		}
-		const uint64_t b_incl = __ballot(status == kIncl);
-		const uint64_t b_none = __ballot(status == 0);
-		const int	   k	  = b_incl ? (__ffsll((long long)b_incl) - 1) : 64;
-		const uint64_t need	  = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-		if (b_none & need) {
+		const Window w = window_extent(incl, none);
+		if (!window_ready(w)) {
 			if (++spins > kSpinLimit) {
 				if (lane == 0)
 					atomicOr(&sum->overflow, kErrLookback);
 				return in_states;
 			}
 			__builtin_amdgcn_s_sleep(1);
+			load = none;  // only records still missing are read again
 			continue;
 		}
-		// window map: agg_0 . agg_1 . ... . agg_{k-1} (lane 0 = newest tile, outermost)
-		Map win[ND];
+		Map win[ND];  // m_63 . m_62 . ... . m_{kp+1}
 #pragma unroll
-		for (int d = 0; d < ND; ++d) {
-			Map m = lane < k ? agg[d] : M::id();
-#pragma unroll
-			for (int off = 1; off < kWave; off <<= 1) {
-				Map o = M::shfl_up(m, off);
-				if (lane >= off)
-					m = M::compose(o, m);
-			}
-			win[d] = k == 0 ? M::id() : M::shfl(m, k - 1);
-		}
-		if (k < 64) {
-			const uint32_t S   = (uint32_t)__shfl((int)ival, k, kWave);
+		for (int d = 0; d < ND; ++d)
+			win[d] = M::readlane(wave_scan_maps<WIDE>(lane > w.kp ? agg[d] : M::id()), 63);
+		if (w.kp >= 0) {
+			const uint32_t S   = readlane(ival, w.kp);
 			uint32_t	   out = 0;
 #pragma unroll
 			for (int d = 0; d < ND; ++d) {
 				uint32_t sd = (S >> (3 * d)) & 7u;
-				sd			= M::apply(win[d], sd);
-				sd			= M::apply(acc[d], sd);
+				sd			= M::apply(acc[d], M::apply(win[d], sd));
 				out |= sd << (3 * d);
 			}
 			return out;
 		}
 #pragma unroll
 		for (int d = 0; d < ND; ++d) acc[d] = M::compose(acc[d], win[d]);
-		jb -= 64;
+		jb -= kLbWindow;
+		load = true;
 	}
 }
 
-// Chain 2 look-back (counts), run by one full wave of tile `tile` > 0.
+// Two-chain mode, chain 2: (structural, bracket, depth) offsets entering tile > 0.
 __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint32_t epoch, const uint32_t base[3],
 									   uint32_t out[3], DevSummary* sum) {
-	const int lane	 = lane_id();
-	uint32_t  acc[3] = {0u, 0u, 0u};
-	int64_t	  jb	 = (int64_t)tile - 1;
-	uint32_t  spins	 = 0;
+	const int  lane	  = lane_id();
+	const bool active = lane >= kWave - kLbWindow;
+	uint32_t   acc[3] = {0u, 0u, 0u};
+	int64_t	   jb	  = (int64_t)tile - 1;
+	uint32_t   spins  = 0;
+	bool	   load = true, incl = false, none = false;
+	uint32_t   v[3] = {0u, 0u, 0u};
 	while (true) {
-		const int64_t j		 = jb - lane;
-		uint32_t	  status = 0, v[3] = {0u, 0u, 0u};
-		if (j < 0) {
-			status = kIncl;
-			v[0]   = base[0];
-			v[1]   = base[1];
-			v[2]   = base[2];
-		} else {
-			// one round trip: the inclusive and aggregate words together
+		const int64_t j = jb - 63 + lane;
+		if (!active || j < 0) {
+			incl = active;
+			none = false;
+#pragma unroll
+			for (int f = 0; f < 3; ++f) v[f] = active ? base[f] : 0u;
+		} else if (load) {
 			const uint64_t* r = recs + (uint64_t)j * kRecWords;
 			uint64_t		wi[3], wa[3];
 #pragma unroll
@@ -168,15 +205,13 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 				inc = inc && word_is(wi[f], epoch, kIncl);
 				agg = agg && word_is(wa[f], epoch, kAgg);
 			}
-			status = inc ? kIncl : (agg ? kAgg : 0u);
+			incl = inc;
+			none = !inc && !agg;
 #pragma unroll
 			for (int f = 0; f < 3; ++f) v[f] = (uint32_t)(inc ? wi[f] : wa[f]);
 		}
-		const uint64_t b_incl = __ballot(status == kIncl);
-		const uint64_t b_none = __ballot(status == 0);
-		const int	   k	  = b_incl ? (__ffsll((long long)b_incl) - 1) : 64;
-		const uint64_t need	  = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-		if (b_none & need) {
+		const Window w = window_extent(incl, none);
+		if (!window_ready(w)) {
 			if (++spins > kSpinLimit) {
 				if (lane == 0)
 					atomicOr(&sum->overflow, kErrLookback);
@@ -186,17 +221,143 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 				return;
 			}
 			__builtin_amdgcn_s_sleep(1);
+			load = none;
 			continue;
 		}
 #pragma unroll
-		for (int f = 0; f < 3; ++f) acc[f] += wave_sum_u32(lane <= k ? v[f] : 0u);
-		if (k < 64) {
+		for (int f = 0; f < 3; ++f) acc[f] += wave_sum(lane >= w.kp ? v[f] : 0u);
+		if (w.kp >= 0) {
 			out[0] = acc[0];
 			out[1] = acc[1];
 			out[2] = acc[2];
 			return;
 		}
-		jb -= 64;
+		jb -= kLbWindow;
+		load = true;
+	}
+}
+
+// One-chain mode: the state entering tile > 0 (`st`) and the counts before it
+// (`cnt`).  A tile's aggregate is its map m plus v[f][e], its counts when
+// entered in state e.  Folding the aggregates newer than an inclusive (S, C):
+// lane L enters in s_L = (m_{L-1} . ... . m_{kp+1})(S) and adds v_L[f][s_L].
+// A window without an inclusive is folded into `acc` as a function of its
+// entry state (for every entry state) and the walk moves on.
+template<bool WIDE, int NS>
+__device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epoch, uint32_t in_states,
+							   const uint32_t base[3], uint32_t& st, uint32_t cnt[3], DevSummary* sum) {
+	using M	  = MapT<WIDE>;
+	using Map = typename M::type;
+	const int  lane	  = lane_id();
+	const bool active = lane >= kWave - kLbWindow;
+	uint32_t   accS[NS], accC[NS][3];  // entry state e of the folded windows -> (tile entry state, counts)
+#pragma unroll
+	for (int e = 0; e < NS; ++e) {
+		accS[e]	   = (uint32_t)e;
+		accC[e][0] = accC[e][1] = accC[e][2] = 0u;
+	}
+	int64_t	 jb	   = (int64_t)tile - 1;
+	uint32_t spins = 0;
+	bool	 load = true, incl = false, none = false;
+	uint32_t iS = 0, iC[3] = {0u, 0u, 0u};
+	Map		 m = M::id();
+	uint32_t v[3][NS];
+	while (true) {
+		const int64_t j = jb - 63 + lane;
+		if (!active || j < 0) {	 // outside the window: neutral; before the input: the carry
+			incl = active;
+			none = false;
+			iS	 = in_states;
+			m	 = M::id();
+#pragma unroll
+			for (int f = 0; f < 3; ++f) {
+				iC[f] = active ? base[f] : 0u;
+#pragma unroll
+				for (int e = 0; e < NS; ++e) v[f][e] = 0u;
+			}
+		} else if (load) {
+			// one round trip: every word of the record
+			const uint64_t* r = recs + (uint64_t)j * kRecWords;
+			uint64_t		wi[4], wv[3 * NS];
+#pragma unroll
+			for (int q = 0; q < 4; ++q) wi[q] = peek(r + kRecFIncl + q);
+			const uint64_t wm = peek(r + kRecFMap);
+#pragma unroll
+			for (int q = 0; q < 3 * NS; ++q) wv[q] = peek(r + kRecFCnt + q);
+			bool inc = true, agg = word_is(wm, epoch, kAgg);
+#pragma unroll
+			for (int q = 0; q < 4; ++q) inc = inc && word_is(wi[q], epoch, kIncl);
+#pragma unroll
+			for (int q = 0; q < 3 * NS; ++q) agg = agg && word_is(wv[q], epoch, kAgg);
+			incl  = inc;
+			none  = !inc && !agg;
+			iS	  = (uint32_t)wi[0];
+			iC[0] = (uint32_t)wi[1];
+			iC[1] = (uint32_t)wi[2];
+			iC[2] = (uint32_t)wi[3];
+			m	  = M::unpack((uint32_t)wm);
+#pragma unroll
+			for (int e = 0; e < NS; ++e)
+#pragma unroll
+				for (int f = 0; f < 3; ++f) v[f][e] = (uint32_t)wv[3 * e + f];
+		}
+		const Window w = window_extent(incl, none);
+		if (!window_ready(w)) {
+			if (++spins > kSpinLimit) {
+				if (lane == 0)
+					atomicOr(&sum->overflow, kErrLookback);
+				st	   = in_states;
+				cnt[0] = base[0];
+				cnt[1] = base[1];
+				cnt[2] = base[2];
+				return;
+			}
+			__builtin_amdgcn_s_sleep(1);
+			load = none;  // only records still missing are read again
+			continue;
+		}
+		const bool agg_lane = lane > w.kp;
+		const Map  inc		= wave_scan_maps<WIDE>(agg_lane ? m : M::id());
+		const Map  exc		= wave_shift_maps<WIDE>(inc);
+		const Map  wmap		= M::readlane(inc, 63);
+		if (w.kp >= 0) {
+			const uint32_t S   = readlane(iS, w.kp) & 7u;
+			const uint32_t s_l = M::apply(exc, S);
+			const uint32_t so  = M::apply(wmap, S);
+			st				   = pick(accS, so);
+#pragma unroll
+			for (int f = 0; f < 3; ++f) {
+				const uint32_t own = agg_lane ? pick(v[f], s_l) : (lane == w.kp ? iC[f] : 0u);
+				uint32_t	   a[NS];
+#pragma unroll
+				for (int e = 0; e < NS; ++e) a[e] = accC[e][f];
+				cnt[f] = wave_sum(own) + pick(a, so);
+			}
+			return;
+		}
+		// no inclusive in this window: fold its function of the entry state
+		uint32_t nS[NS], nC[NS][3];
+#pragma unroll
+		for (int e = 0; e < NS; ++e) {
+			const uint32_t s_l = M::apply(exc, (uint32_t)e);
+			const uint32_t we  = M::apply(wmap, (uint32_t)e);
+			nS[e]			   = pick(accS, we);
+#pragma unroll
+			for (int f = 0; f < 3; ++f) {
+				uint32_t a[NS];
+#pragma unroll
+				for (int q = 0; q < NS; ++q) a[q] = accC[q][f];
+				nC[e][f] = wave_sum(pick(v[f], s_l)) + pick(a, we);
+			}
+		}
+#pragma unroll
+		for (int e = 0; e < NS; ++e) {
+			accS[e] = nS[e];
+#pragma unroll
+			for (int f = 0; f < 3; ++f) accC[e][f] = nC[e][f];
+		}
+		jb -= kLbWindow;
+		load = true;
 	}
 }
 
@@ -205,8 +366,9 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 // draws tiles from an atomic ticket in dispatch order until none is left (so a
 // tile only ever waits on tiles held by running blocks).  A tile is W rows of
 // 16 KiB; in row r, lane l owns bytes [r*16K + l*64, +64), so every row's
-// outputs are one contiguous run: they are staged in LDS and written back with
-// coalesced stores (scattered 4-byte stores amplify HBM writes several-fold).
+// outputs are one contiguous run: they are staged in LDS (row-local 16-bit
+// offsets) and written back with coalesced stores (scattered 4-byte stores
+// amplify HBM writes several-fold).
 //
 // E (the lane evaluator) provides:
 //   kND, kWide, Map        descriptors and their map type
@@ -214,35 +376,42 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 //   walk(Lds, plan, x, valid)   one pass over a lane's 64 bytes
 //   T[d]                    the lane's map after walk
 //   masked(s0[])            masked bits for lane-entry states s0
+//   masked_all(m[])         (one descriptor) masked bits for every entry state
 //   classes(S, OC, OP)      structural / bracket / open bits
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRowBytes = kBlock * kBPL;  // 16 KiB
-constexpr int	   kCapP	 = 3072;		   // staged structural outputs per row
-constexpr int	   kCapB	 = 768;			   // staged bracket outputs per row
+constexpr int	   kCapP	 = 4096;		   // staged structural outputs per row
+constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
 
 template<class E, int W>
 __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
-	constexpr int ND  = E::kND;
-	constexpr int NDA = ND > 0 ? ND : 1;
-	using M			  = MapT<E::kWide>;
-	using Map		  = typename M::type;
+	constexpr int  ND	 = E::kND;
+	constexpr int  NDA	 = ND > 0 ? ND : 1;
+	constexpr bool FUSED = ND == 1;
+	constexpr int  NS	 = E::kWide ? 8 : 4;  // entry states a one-descriptor tile is summarised for
+	constexpr int  NPK	 = NS / 4;			  // count dwords per field, four states each
+	using M				 = MapT<E::kWide>;
+	using Map			 = typename M::type;
 	constexpr uint64_t kTileBytes = (uint64_t)W * kRowBytes;
 
 	__shared__ typename E::Lds s_lds;
 	__shared__ Map			   s_wagg[W][4][NDA];
 	__shared__ uint32_t		   s_misc[8];
 	__shared__ uint32_t		   s_wcnt[W][4][3];
+	__shared__ uint32_t		   s_vsum[4][3][2 * NPK];  // one-chain: per-wave 16-bit-pair count sums
+	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
 	__shared__ int32_t		   s_mm[4][2];
-	__shared__ uint32_t		   s_pos[kCapP];
-	__shared__ uint32_t		   s_bpos[kCapB];
-	__shared__ uint32_t		   s_bsidx[kCapB];
-	__shared__ int32_t		   s_bkey[kCapB];
-	__shared__ uint8_t		   s_bch[kCapB];
+	__shared__ uint16_t		   s_pos[kCapP];	// row-local byte offset
+	__shared__ uint16_t		   s_bpos[kCapB];	// row-local byte offset
+	__shared__ uint16_t		   s_bsidx[kCapB];	// row-local structural index
+	__shared__ int16_t		   s_bkey[kCapB];	// pairing key relative to the row's entry depth
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	E::load(s_lds, a.plan);
 	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
 
+	// A block takes its next tile only when it is ready to start it: a ticket
+	// drawn early leaves a hole that every later tile's look-back waits on.
 	while (true) {
 		__syncthreads();  // LDS of the previous tile is free
 		if (tid == 0)
@@ -286,10 +455,12 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 			ev[r].walk(s_lds, a.plan, x, valid[r]);
 		}
 
-		// ---- 2. chain 1: entry states ----
-		uint32_t tile_in = a.in_states;
-		Map		 lpre[W][NDA];
-		uint32_t row_in[W];
+		// ---- 2. entry states (and, one-chain, output offsets) ----
+		uint32_t	   tile_in	= a.in_states;
+		const uint32_t cbase[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
+		uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
+		Map			   lpre[W][NDA];
+		uint32_t	   row_in[W];
 		if constexpr (ND > 0) {
 			Map exc[W][ND];
 #pragma unroll
@@ -297,9 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 #pragma unroll
 				for (int d = 0; d < ND; ++d) {
 					const Map inc = wave_scan_maps<E::kWide>(ev[r].T[d]);
-					exc[r][d]	  = M::shfl_up(inc, 1);
-					if (lane == 0)
-						exc[r][d] = M::id();
+					exc[r][d]	  = wave_shift_maps<E::kWide>(inc);
 					if (lane == 63)
 						s_wagg[r][wid][d] = inc;
 				}
@@ -321,7 +490,98 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 					tagg[d]	   = M::compose(wp, tagg[d]);
 				}
 			}
-			if (tile != 0) {
+
+			if constexpr (FUSED) {
+				// The lane's counts for every lane-entry state, one byte per state
+				// (depth delta biased by +64), gathered per tile-entry state through
+				// the lane's entry map, then summed as 16-bit pairs over rows and
+				// lanes (at most 4 x 128 x 64 = 32768 per field per wave).
+				uint32_t acc16[3][2 * NPK];
+#pragma unroll
+				for (int f = 0; f < 3; ++f)
+#pragma unroll
+					for (int q = 0; q < 2 * NPK; ++q) acc16[f][q] = 0u;
+				Map rowpre = M::id();  // tile entry -> row entry
+#pragma unroll
+				for (int r = 0; r < W; ++r) {
+					uint64_t ms[NS];
+					ev[r].masked_all(ms);
+					uint64_t S, OC, OP;
+					ev[r].classes(S, OC, OP);
+					uint32_t pk[3][NPK];
+#pragma unroll
+					for (int f = 0; f < 3; ++f)
+#pragma unroll
+						for (int h = 0; h < NPK; ++h) pk[f][h] = 0u;
+#pragma unroll
+					for (int s = 0; s < NS; ++s) {
+						const uint32_t ns = popc64(S & ~ms[s]), nb = popc64(OC & ~ms[s]);
+						const uint32_t dd = 2u * popc64(OP & ~ms[s]) - nb + 64u;
+						pk[0][s >> 2] |= ns << (8 * (s & 3));
+						pk[1][s >> 2] |= nb << (8 * (s & 3));
+						pk[2][s >> 2] |= dd << (8 * (s & 3));
+					}
+					const Map lmap = M::compose(lpre[r][0], rowpre);  // tile entry -> lane entry
+#pragma unroll
+					for (int f = 0; f < 3; ++f) {
+						uint32_t g[NPK];
+						if constexpr (E::kWide) {
+							g[0]	   = perm(pk[f][NPK - 1], pk[f][0], lmap.lo);
+							g[NPK - 1] = perm(pk[f][NPK - 1], pk[f][0], lmap.hi);
+						} else {
+							g[0] = perm(0u, pk[f][0], lmap.lo);
+						}
+#pragma unroll
+						for (int h = 0; h < NPK; ++h) {
+							acc16[f][2 * h] += g[h] & 0x00FF00FFu;
+							acc16[f][2 * h + 1] += (g[h] >> 8) & 0x00FF00FFu;
+						}
+					}
+					rowpre = M::compose(ragg[r][0], rowpre);
+				}
+#pragma unroll
+				for (int f = 0; f < 3; ++f)
+#pragma unroll
+					for (int q = 0; q < 2 * NPK; ++q) {
+						const uint32_t t = wave_sum(acc16[f][q]);
+						if (lane == 0)
+							s_vsum[wid][f][q] = t;
+					}
+				__syncthreads();
+				if (tid < 3 * NS) {	 // thread (f, e): field e of the four wave sums
+					const int f = tid / NS, e = tid % NS;
+					const int q = 2 * (e >> 2) + (e & 1), sh = (e & 2) ? 16 : 0;
+					uint32_t  t = 0;
+#pragma unroll
+					for (int w2 = 0; w2 < 4; ++w2) t += (s_vsum[w2][f][q] >> sh) & 0xFFFFu;
+					if (f == 2)
+						t -= 64u * (uint32_t)(kBlock * W);
+					s_v[f][e] = t;
+					if (tile != 0)
+						publish(rec + kRecFCnt + 3 * e + f, make_word(a.epoch, kAgg, t));
+				}
+				if (tile != 0) {
+					if (tid == 0)
+						publish(rec + kRecFMap, make_word(a.epoch, kAgg, M::pack(tagg[0])));
+					if (wid == 0) {
+						uint32_t st, o[3];
+						lookback_fused<E::kWide, NS>(a.recs, tile, a.epoch, a.in_states, cbase, st, o, a.sum);
+						if (lane == 0) {
+							s_misc[1] = st;
+							s_misc[2] = o[0];
+							s_misc[3] = o[1];
+							s_misc[4] = o[2];
+						}
+					}
+				}
+				__syncthreads();
+				if (tile != 0) {
+					tile_in = s_misc[1];
+					cex[0]	= s_misc[2];
+					cex[1]	= s_misc[3];
+					cex[2]	= s_misc[4];
+				}
+			} else if (tile != 0) {
 				if (tid == 0)
 #pragma unroll
 					for (int d = 0; d < ND; ++d)
@@ -344,7 +604,15 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 				cur = nxt;
 			}
 			if (tid == 0) {
-				publish(rec + kRecSIncl, make_word(a.epoch, kIncl, cur));
+				if constexpr (FUSED) {
+					const uint32_t e = tile_in & 7u;
+					publish(rec + kRecFIncl, make_word(a.epoch, kIncl, cur));
+#pragma unroll
+					for (int f = 0; f < 3; ++f)
+						publish(rec + kRecFIncl + 1 + f, make_word(a.epoch, kIncl, cex[f] + s_v[f][e]));
+				} else {
+					publish(rec + kRecSIncl, make_word(a.epoch, kIncl, cur));
+				}
 				if (tile == a.n_tiles - 1)
 					a.sum->final_states = cur;
 			}
@@ -355,7 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 				a.sum->final_states = 0;
 		}
 
-		// ---- 3. masks and counts ----
+		// ---- 3. masks and in-tile offsets ----
 		uint64_t sm[W], om[W], opm[W];
 		uint32_t cnt[W][3];
 #pragma unroll
@@ -371,9 +639,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 			opm[r] = OP & ~masked;
 			if (a.bitmap && valid[r])
 				a.bitmap[(tbase + (uint64_t)r * kRowBytes) / 64 + tid] = sm[r];
-			cnt[r][0] = (uint32_t)__popcll(sm[r]);
-			cnt[r][1] = (uint32_t)__popcll(om[r]);
-			cnt[r][2] = 2u * (uint32_t)__popcll(opm[r]) - cnt[r][1];
+			cnt[r][0] = popc64(sm[r]);
+			cnt[r][1] = popc64(om[r]);
+			cnt[r][2] = 2u * popc64(opm[r]) - cnt[r][1];
 		}
 		uint32_t cpre[W][3], rtot[W][3], ctot[3] = {0u, 0u, 0u};
 		{
@@ -405,35 +673,35 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 				}
 		}
 
-		// chain 2: tile output offsets
-		const uint32_t cbase[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
-		uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
-		if (tile != 0) {
+		// two-chain mode, chain 2: tile output offsets
+		if constexpr (!FUSED) {
+			if (tile != 0) {
+				if (tid == 0)
+#pragma unroll
+					for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
+				if (wid == 0) {
+					uint32_t o[3];
+					lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
+					if (lane == 0) {
+						s_misc[2] = o[0];
+						s_misc[3] = o[1];
+						s_misc[4] = o[2];
+					}
+				}
+				__syncthreads();
+				cex[0] = s_misc[2];
+				cex[1] = s_misc[3];
+				cex[2] = s_misc[4];
+			}
 			if (tid == 0)
 #pragma unroll
-				for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
-			if (wid == 0) {
-				uint32_t o[3];
-				lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
-				if (lane == 0) {
-					s_misc[2] = o[0];
-					s_misc[3] = o[1];
-					s_misc[4] = o[2];
-				}
-			}
-			__syncthreads();
-			cex[0] = s_misc[2];
-			cex[1] = s_misc[3];
-			cex[2] = s_misc[4];
+				for (int f = 0; f < 3; ++f)
+					publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
 		}
-		if (tid == 0) {
-#pragma unroll
-			for (int f = 0; f < 3; ++f) publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
-			if (tile == a.n_tiles - 1) {
-				a.sum->n_struct	  = cex[0] + ctot[0];
-				a.sum->n_brackets = cex[1] + ctot[1];
-				a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
-			}
+		if (tid == 0 && tile == a.n_tiles - 1) {
+			a.sum->n_struct	  = cex[0] + ctot[0];
+			a.sum->n_brackets = cex[1] + ctot[1];
+			a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
 		}
 		if (a.pair_pos)	 // -1 for every structural index of the tile; R6 overwrites brackets
 			for (uint32_t i = tid; i < ctot[0]; i += kBlock)
@@ -441,67 +709,90 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 					a.pair_pos[cex[0] + i] = -1;
 
 		// ---- 4. scatter, row by row ----
+		// Depths below are relative to the tile's entry depth until the end.
 		uint32_t rb[3] = {cex[0], cex[1], cex[2]};
+		int32_t	 rowd = 0;	// the row's entry depth
 		int32_t	 lmin = INT_MAX, lmax = INT_MIN;
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
-			const uint64_t rowb	  = tbase + (uint64_t)r * kRowBytes;
-			const uint64_t lbase  = rowb + (uint64_t)tid * kBPL;
+			const uint64_t rowb	  = tbase + (uint64_t)r * kRowBytes;  // data index of the row
+			const uint32_t loff	  = (uint32_t)tid * kBPL;			  // lane's offset in the row
+			const uint64_t pbase  = a.byte_base + rowb;
+			const int32_t  dl	  = rowd + (int32_t)cpre[r][2];  // the lane's entry depth
 			const bool	   staged = rtot[r][0] <= (uint32_t)kCapP && rtot[r][1] <= (uint32_t)kCapB;
-			uint32_t	   li = cpre[r][0], lb = cpre[r][1];
-			int32_t		   bal	= (int32_t)(rb[2] + cpre[r][2]);
-			const uint64_t pbase = a.byte_base + lbase;
-			uint64_t	   m	 = sm[r];
-			while (m) {
-				const int	   p   = __builtin_ctzll(m);
-				const uint32_t pos = (uint32_t)(pbase + (uint64_t)p);
-				const bool	   br  = (om[r] >> p) & 1ull;
-				uint8_t		   ch  = 0;
-				if (br || a.chars)
-					ch = a.data[lbase + p];
-				if (a.chars && (uint64_t)rb[0] + li < a.cap_struct)
-					a.chars[rb[0] + li] = ch;
-				if (staged)
-					s_pos[li] = pos;
-				else if ((uint64_t)rb[0] + li < a.cap_struct)
-					a.positions[rb[0] + li] = pos;
-				if (br) {
-					const bool op = (opm[r] >> p) & 1ull;
-					bal += op ? 1 : -1;
-					lmin			  = min(lmin, bal);
-					lmax			  = max(lmax, bal);
-					const int32_t key = op ? bal - 1 : bal;
-					if (staged) {
-						s_bpos[lb]	= pos;
-						s_bsidx[lb] = rb[0] + li;
-						s_bkey[lb]	= key;
-						s_bch[lb]	= ch;
-					} else if ((uint64_t)rb[1] + lb < a.cap_br) {
-						const uint32_t g = rb[1] + lb;
-						a.bpos[g]		 = pos;
-						a.bsidx[g]		 = rb[0] + li;
-						a.bkey[g]		 = key;
-						a.bchar[g]		 = ch;
-					}
-					++lb;
-				}
-				++li;
-				m &= m - 1;
-			}
 			if (staged) {  // block-uniform
+				uint64_t m	= sm[r];
+				uint32_t li = cpre[r][0];
+				while (m) {
+					s_pos[li++] = (uint16_t)(loff + (uint32_t)__builtin_ctzll(m));
+					m &= m - 1;
+				}
+				uint64_t bm = om[r];
+				uint32_t lb = cpre[r][1];
+				while (bm) {
+					const int	  p	   = __builtin_ctzll(bm);
+					const bool	  op   = (opm[r] >> p) & 1ull;
+					const int32_t bal0 = dl + 2 * (int32_t)popc_below(opm[r], p) - (int32_t)popc_below(om[r], p);
+					const int32_t bal1 = bal0 + (op ? 1 : -1);
+					lmin			   = min(lmin, bal1);
+					lmax			   = max(lmax, bal1);
+					s_bpos[lb]		   = (uint16_t)(loff + (uint32_t)p);
+					s_bsidx[lb]		   = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
+					s_bkey[lb]		   = (int16_t)((op ? bal0 : bal1) - rowd);
+					++lb;
+					bm &= bm - 1;
+				}
 				__syncthreads();
 				const uint32_t np = rtot[r][0], nb = rtot[r][1];
 				const uint64_t lim_p = a.cap_struct > rb[0] ? a.cap_struct - rb[0] : 0;
 				const uint64_t lim_b = a.cap_br > rb[1] ? a.cap_br - rb[1] : 0;
-				for (uint32_t i = tid; i < np && i < lim_p; i += kBlock) a.positions[rb[0] + i] = s_pos[i];
+				for (uint32_t i = tid; i < np && i < lim_p; i += kBlock) {
+					const uint32_t off	   = s_pos[i];
+					a.positions[rb[0] + i] = (uint32_t)(pbase + off);
+					if (a.chars)
+						a.chars[rb[0] + i] = a.data[rowb + off];
+				}
+				const int32_t kb = (int32_t)rb[2];
 				for (uint32_t i = tid; i < nb && i < lim_b; i += kBlock) {
-					a.bpos[rb[1] + i]  = s_bpos[i];
-					a.bsidx[rb[1] + i] = s_bsidx[i];
-					a.bkey[rb[1] + i]  = s_bkey[i];
-					a.bchar[rb[1] + i] = s_bch[i];
+					const uint32_t off = s_bpos[i];
+					a.bpos[rb[1] + i]  = (uint32_t)(pbase + off);
+					a.bsidx[rb[1] + i] = rb[0] + s_bsidx[i];
+					a.bkey[rb[1] + i]  = kb + (int32_t)s_bkey[i];
+					a.bchar[rb[1] + i] = a.data[rowb + off];
 				}
 				__syncthreads();
+			} else {  // a row too dense to stage: direct stores
+				uint64_t m	= sm[r];
+				uint32_t li = cpre[r][0], lb = cpre[r][1];
+				int32_t	 bal = dl;
+				while (m) {
+					const int	   p   = __builtin_ctzll(m);
+					const uint32_t pos = (uint32_t)(pbase + loff + (uint64_t)p);
+					const uint8_t  ch  = a.data[rowb + loff + p];
+					if ((uint64_t)rb[0] + li < a.cap_struct) {
+						a.positions[rb[0] + li] = pos;
+						if (a.chars)
+							a.chars[rb[0] + li] = ch;
+					}
+					if ((om[r] >> p) & 1ull) {
+						const bool op = (opm[r] >> p) & 1ull;
+						bal += op ? 1 : -1;
+						lmin = min(lmin, bal);
+						lmax = max(lmax, bal);
+						if ((uint64_t)rb[1] + lb < a.cap_br) {
+							const uint32_t g = rb[1] + lb;
+							a.bpos[g]		 = pos;
+							a.bsidx[g]		 = rb[0] + li;
+							a.bkey[g]		 = (int32_t)cex[2] + (op ? bal - 1 : bal);
+							a.bchar[g]		 = ch;
+						}
+						++lb;
+					}
+					++li;
+					m &= m - 1;
+				}
 			}
+			rowd += (int32_t)rtot[r][2];
 #pragma unroll
 			for (int f = 0; f < 3; ++f) rb[f] += rtot[r][f];
 		}
@@ -520,9 +811,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 				mx = max(mx, s_mm[w2][1]);
 			}
 			if (mn != INT_MAX)
-				atomicMin(&a.sum->min_bal, mn);
+				atomicMin(&a.sum->min_bal, (int32_t)cex[2] + mn);
 			if (mx != INT_MIN)
-				atomicMax(&a.sum->max_bal, mx);
+				atomicMax(&a.sum->max_bal, (int32_t)cex[2] + mx);
 		}
 	}
 	// every block has drawn a ticket past the end: the last one out resets

@@ -128,6 +128,21 @@ struct GeneralEval {
 		return m;
 	}
 
+	// One descriptor: masked bits for every entry state (state s = byte s).
+	__device__ void masked_all(uint64_t (&m)[WIDE ? 8 : 4]) const {
+		uint32_t t0[4], t1[4];
+		transpose4(accl[0][0], accl[0][1], accl[0][2], accl[0][3], t0[0], t0[1], t0[2], t0[3]);
+		transpose4(accl[0][4], accl[0][5], accl[0][6], accl[0][7], t1[0], t1[1], t1[2], t1[3]);
+#pragma unroll
+		for (int s = 0; s < 4; ++s) m[s] = ((uint64_t)t1[s] << 32) | t0[s];
+		if constexpr (WIDE) {
+			transpose4(acch[0][0], acch[0][1], acch[0][2], acch[0][3], t0[0], t0[1], t0[2], t0[3]);
+			transpose4(acch[0][4], acch[0][5], acch[0][6], acch[0][7], t1[0], t1[1], t1[2], t1[3]);
+#pragma unroll
+			for (int s = 0; s < 4; ++s) m[4 + s] = ((uint64_t)t1[s] << 32) | t0[s];
+		}
+	}
+
 	__device__ void classes(uint64_t& S, uint64_t& OC, uint64_t& OP) const {
 		uint64_t unused;
 		gather_classes(cacc, S, OC, OP, unused);
@@ -283,6 +298,14 @@ struct EscapeEval {
 		return (mO & ~(kI | kE)) | (mI & kI) | (mE & kE);
 	}
 
+	__device__ void masked_all(uint64_t (&m)[4]) const {
+#pragma unroll
+		for (int s = 0; s < 4; ++s) {
+			const uint32_t s0[1] = {(uint32_t)s};
+			m[s]				 = masked(s0);
+		}
+	}
+
 	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {
 		s  = S;
 		oc = OC;
@@ -349,6 +372,14 @@ struct ParityEval {
 			return odd;
 		const uint64_t prevq = (Q << 1) | (s0[0] == f_cl ? 1ull : 0ull);
 		return odd | (Q & prevq);
+	}
+
+	__device__ void masked_all(uint64_t (&m)[4]) const {
+#pragma unroll
+		for (int s = 0; s < 4; ++s) {
+			const uint32_t s0[1] = {(uint32_t)s};
+			m[s]				 = masked(s0);
+		}
 	}
 
 	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {

@@ -12,6 +12,9 @@ kernel.  Algorithmic bytes per launch (per document):
     B_alg = n (input) + n/8 (structural bitmap) + 4 k (uint32 positions)
             + 4 k_b (bracket-pair array)
 with k, k_b the document's structural and bracket counts.  peak = 8 TB/s.
+`traffic` is the HBM bytes one scan launch moves as measured by rocprofv3
+counters (FETCH_SIZE + WRITE_SIZE, calibrated on a plain device copy); it is
+read from the committed profile summary of this workload when present.
 
 cpu_baseline: the reference's own CPU backend (model::scanner + pair_brackets,
 compiled from its sources into oracle/_ref/) on a bounded prefix of the same
@@ -35,6 +38,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "input GB/s + % HBM roofline (2 GiB JSON); batched/served vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
+SCAN_KERNEL = "k_scan<EscapeEval, 4> (R1-R5)"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scan_json_2gib.md")
+
+
+def measured_traffic(n: int) -> int | None:
+    """HBM bytes per scan launch from the committed counter summary (same workload size)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            fields = dict(
+                line.strip().lstrip("- ").split(": ", 1) for line in f if line.strip().lstrip("- ").startswith("traffic_")
+            )
+        if int(fields.get("traffic_input_bytes", -1)) != n:
+            return None
+        return int(fields["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def log(msg: str) -> None:
@@ -84,6 +103,8 @@ def main() -> int:
     ap.add_argument("--size-mib", type=int, default=2048)
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of the document (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -113,6 +134,13 @@ def main() -> int:
     d_doc = torch.from_numpy(doc).to("cuda")
     torch.cuda.synchronize()
 
+    if args.copy_calibration:
+        for _ in range(2):
+            d_copy = d_doc.clone()
+            torch.cuda.synchronize()
+            del d_copy
+        log(f"copy calibration: 2 device copies of {n} bytes")
+
     sc = Scanner(Plan.load("json"))
     sc.set_option(N.PARS_OPT_WANT_BITMAP, 1)
     stream = torch.cuda.Stream()
@@ -123,19 +151,20 @@ def main() -> int:
     for i in range(args.warmup):
         r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
         log(f"warmup {i}: {r}")
-    k, kb = r.n_struct, r.n_brackets
 
-    scan_ms, pair_ms = [], []
+    scan_ms, pair_ms, counts = [], [], set()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
         scan_ms.append(st.p1)
         pair_ms.append(st.p4)
+        counts.add((r.n_struct, r.n_brackets, r.balanced, r.well_formed))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
-    assert (r.n_struct, r.n_brackets) == (k, kb)
+    assert len(counts) == 1, f"steps disagree: {counts}"
+    k, kb = r.n_struct, r.n_brackets
     log(f"steps done: {ms_step:.3f} ms/step; scan kernel {np.mean(scan_ms):.3f} ms, pairing {np.mean(pair_ms):.3f} ms")
 
     b_alg = n + n / 8 + 4 * k + 4 * kb
@@ -167,12 +196,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_scan_general<1,false> (R1-R5)",
+            "kernel": SCAN_KERNEL,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(n),
             "b_alg_bytes": int(b_alg),
             "kernel_ms": round(t_scan * 1e3, 4),
         },

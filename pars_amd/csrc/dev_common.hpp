@@ -42,6 +42,11 @@ __device__ __forceinline__ uint32_t readlane(uint32_t v, int lane) {
 	return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
 }
 
+// A value known to be the same in every active lane, moved to a scalar register.
+__device__ __forceinline__ uint32_t readfirstlane(uint32_t v) {
+	return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 struct Map4 {
 	uint32_t lo;
 };

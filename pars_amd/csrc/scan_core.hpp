@@ -383,8 +383,13 @@ constexpr uint32_t kRowBytes = kBlock * kBPL;  // 16 KiB
 constexpr int	   kCapP	 = 4096;		   // staged structural outputs per row
 constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
 
+// Many-descriptor plans with wide descriptors need most of the register file;
+// they are held to two waves per SIMD rather than left to fall to one.
+template<class E>
+constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : 1;
+
 template<class E, int W>
-__global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E>))) void k_scan(ScanArgs a) {
 	constexpr int  ND	 = E::kND;
 	constexpr int  NDA	 = ND > 0 ? ND : 1;
 	constexpr bool FUSED = ND == 1;
@@ -424,13 +429,13 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 		uint64_t*	   rec	 = a.recs + (uint64_t)tile * kRecWords;
 
 		// ---- 1. one walk per row ----
+		// Rows are loaded one ahead of the walk (two row buffers in flight, not W).
 		E		 ev[W];
 		uint32_t valid[W];
-#pragma unroll
-		for (int r = 0; r < W; ++r) {
+		uint32_t xb[2][16];
+		auto	 load_row = [&](int r, uint32_t (&x)[16]) {
 			const uint64_t base = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
 			valid[r]			= base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
-			uint32_t x[16];
 			if (valid[r] == kBPL && aligned) {
 				const uint4* p = reinterpret_cast<const uint4*>(a.data + base);
 #pragma unroll
@@ -452,7 +457,15 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 					x[k] = w;
 				}
 			}
-			ev[r].walk(s_lds, a.plan, x, valid[r]);
+		};
+		load_row(0, xb[0]);
+#pragma unroll
+		for (int r = 0; r < W; ++r) {
+			if (r + 1 < W) {
+				load_row(r + 1, xb[(r + 1) & 1]);
+				__asm__ volatile("" ::: "memory");	// keep later rows' loads behind this walk
+			}
+			ev[r].walk(s_lds, a.plan, xb[r & 1], valid[r]);
 		}
 
 		// ---- 2. entry states (and, one-chain, output offsets) ----
@@ -668,8 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 						t += v;
 					}
 					cpre[r][f] = p + cinc[r][f] - cnt[r][f];
-					rtot[r][f] = t;
-					ctot[f] += t;
+					rtot[r][f] = readfirstlane(t);	// block-uniform
+					ctot[f] += rtot[r][f];
 				}
 		}
 
@@ -721,24 +734,33 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 			const int32_t  dl	  = rowd + (int32_t)cpre[r][2];  // the lane's entry depth
 			const bool	   staged = rtot[r][0] <= (uint32_t)kCapP && rtot[r][1] <= (uint32_t)kCapB;
 			if (staged) {  // block-uniform
-				uint64_t m	= sm[r];
-				uint32_t li = cpre[r][0];
-				while (m) {
-					s_pos[li++] = (uint16_t)(loff + (uint32_t)__builtin_ctzll(m));
-					m &= m - 1;
+				{  // both 32-bit halves of the mask at once: half the iterations
+					uint32_t mlo = (uint32_t)sm[r], mhi = (uint32_t)(sm[r] >> 32);
+					uint32_t ilo = cpre[r][0], ihi = ilo + (uint32_t)__popc(mlo);
+					while (mlo | mhi) {
+						if (mlo) {
+							s_pos[ilo++] = (uint16_t)(loff + (uint32_t)__builtin_ctz(mlo));
+							mlo &= mlo - 1;
+						}
+						if (mhi) {
+							s_pos[ihi++] = (uint16_t)(loff + 32u + (uint32_t)__builtin_ctz(mhi));
+							mhi &= mhi - 1;
+						}
+					}
 				}
 				uint64_t bm = om[r];
 				uint32_t lb = cpre[r][1];
+				int32_t	 bal = dl;	// brackets come in order: a running depth
 				while (bm) {
 					const int	  p	   = __builtin_ctzll(bm);
 					const bool	  op   = (opm[r] >> p) & 1ull;
-					const int32_t bal0 = dl + 2 * (int32_t)popc_below(opm[r], p) - (int32_t)popc_below(om[r], p);
-					const int32_t bal1 = bal0 + (op ? 1 : -1);
-					lmin			   = min(lmin, bal1);
-					lmax			   = max(lmax, bal1);
-					s_bpos[lb]		   = (uint16_t)(loff + (uint32_t)p);
-					s_bsidx[lb]		   = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
-					s_bkey[lb]		   = (int16_t)((op ? bal0 : bal1) - rowd);
+					const int32_t bal0 = bal;
+					bal += op ? 1 : -1;
+					lmin		= min(lmin, bal);
+					lmax		= max(lmax, bal);
+					s_bpos[lb]	= (uint16_t)(loff + (uint32_t)p);
+					s_bsidx[lb] = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
+					s_bkey[lb]	= (int16_t)((op ? bal0 : bal) - rowd);
 					++lb;
 					bm &= bm - 1;
 				}
@@ -746,19 +768,46 @@ __global__ __launch_bounds__(kBlock) void k_scan(ScanArgs a) {
 				const uint32_t np = rtot[r][0], nb = rtot[r][1];
 				const uint64_t lim_p = a.cap_struct > rb[0] ? a.cap_struct - rb[0] : 0;
 				const uint64_t lim_b = a.cap_br > rb[1] ? a.cap_br - rb[1] : 0;
-				for (uint32_t i = tid; i < np && i < lim_p; i += kBlock) {
-					const uint32_t off	   = s_pos[i];
-					a.positions[rb[0] + i] = (uint32_t)(pbase + off);
+				const uint32_t mp = (uint32_t)min<uint64_t>(np, lim_p), mb = (uint32_t)min<uint64_t>(nb, lim_b);
+				// input bytes for chars are gathered four per thread at once, so
+				// their latencies overlap
+				constexpr int U = 4;
+				for (uint32_t i0 = 0; i0 < mp; i0 += U * kBlock) {
+					uint8_t ch[U];
 					if (a.chars)
-						a.chars[rb[0] + i] = a.data[rowb + off];
+#pragma unroll
+						for (int u = 0; u < U; ++u) {
+							const uint32_t i = i0 + u * kBlock + tid;
+							ch[u]			 = i < mp ? a.data[rowb + s_pos[i]] : 0;
+						}
+#pragma unroll
+					for (int u = 0; u < U; ++u) {
+						const uint32_t i = i0 + u * kBlock + tid;
+						if (i < mp) {
+							a.positions[rb[0] + i] = (uint32_t)(pbase + s_pos[i]);
+							if (a.chars)
+								a.chars[rb[0] + i] = ch[u];
+						}
+					}
 				}
 				const int32_t kb = (int32_t)rb[2];
-				for (uint32_t i = tid; i < nb && i < lim_b; i += kBlock) {
-					const uint32_t off = s_bpos[i];
-					a.bpos[rb[1] + i]  = (uint32_t)(pbase + off);
-					a.bsidx[rb[1] + i] = rb[0] + s_bsidx[i];
-					a.bkey[rb[1] + i]  = kb + (int32_t)s_bkey[i];
-					a.bchar[rb[1] + i] = a.data[rowb + off];
+				for (uint32_t i0 = 0; i0 < mb; i0 += U * kBlock) {
+					uint8_t ch[U];
+#pragma unroll
+					for (int u = 0; u < U; ++u) {
+						const uint32_t i = i0 + u * kBlock + tid;
+						ch[u]			 = i < mb ? a.data[rowb + s_bpos[i]] : 0;
+					}
+#pragma unroll
+					for (int u = 0; u < U; ++u) {
+						const uint32_t i = i0 + u * kBlock + tid;
+						if (i < mb) {
+							a.bpos[rb[1] + i]  = (uint32_t)(pbase + s_bpos[i]);
+							a.bsidx[rb[1] + i] = rb[0] + s_bsidx[i];
+							a.bkey[rb[1] + i]  = kb + (int32_t)s_bkey[i];
+							a.bchar[rb[1] + i] = ch[u];
+						}
+					}
 				}
 				__syncthreads();
 			} else {  // a row too dense to stage: direct stores

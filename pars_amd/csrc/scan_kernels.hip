@@ -39,6 +39,45 @@ __device__ __forceinline__ void gather_classes(const uint32_t (&c)[8], uint64_t&
 	B3 = ((uint64_t)q1 << 32) | q0;
 }
 
+// Class lookups of a lane's 64 bytes through the position-shifted two-dword
+// table: group g ORs the entries of bytes 8g..8g+7 (a balanced OR tree, so the
+// compiler forms three-input ORs).
+__device__ __forceinline__ void lookup_groups(const uint2* cls, const uint32_t (&x)[16], uint32_t valid,
+											  uint32_t (&clo)[8], uint32_t (&chi)[8]) {
+	if (valid == kBPL) {
+#pragma unroll
+		for (int g = 0; g < 8; ++g) {
+			uint2 e[8];
+#pragma unroll
+			for (int b = 0; b < 8; ++b) {
+				const int	   j = 8 * g + b;
+				const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+				e[b]			 = cls[b * 256 + c];
+			}
+			clo[g] = ((e[0].x | e[1].x) | (e[2].x | e[3].x)) | ((e[4].x | e[5].x) | (e[6].x | e[7].x));
+			chi[g] = ((e[0].y | e[1].y) | (e[2].y | e[3].y)) | ((e[4].y | e[5].y) | (e[6].y | e[7].y));
+		}
+	} else {
+#pragma unroll
+		for (int g = 0; g < 8; ++g) clo[g] = chi[g] = 0;
+#pragma unroll
+		for (int j = 0; j < kBPL; ++j)
+			if ((uint32_t)j < valid) {
+				const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+				const uint2	   e = cls[(j & 7) * 256 + c];
+				clo[j >> 3] |= e.x;
+				chi[j >> 3] |= e.y;
+			}
+	}
+}
+
+// Byte 0 of eight accumulated groups -> a 64-bit mask (byte g = group g).
+__device__ __forceinline__ uint64_t gather_byte0(const uint32_t (&c)[8]) {
+	const uint32_t lo = perm(c[1], c[0], 0x0C0C0400u) | perm(c[3], c[2], 0x04000C0Cu);
+	const uint32_t hi = perm(c[5], c[4], 0x0C0C0400u) | perm(c[7], c[6], 0x04000C0Cu);
+	return ((uint64_t)hi << 32) | lo;
+}
+
 // ---------------------------------------------------------------------------
 template<int ND, bool WIDE>
 struct GeneralEval {
@@ -237,28 +276,10 @@ struct EscapeEval {
 
 	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
 		uint32_t clo[8], chi[8];
-#pragma unroll
-		for (int g = 0; g < 8; ++g) clo[g] = chi[g] = 0;
-		auto step = [&](int j) {
-			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-			const uint2	   e = L.cls[(j & 7) * 256 + c];
-			clo[j >> 3] |= e.x;
-			chi[j >> 3] |= e.y;
-		};
-		if (valid == kBPL) {
-#pragma unroll
-			for (int j = 0; j < kBPL; ++j) step(j);
-		} else {
-#pragma unroll
-			for (int j = 0; j < kBPL; ++j)
-				if ((uint32_t)j < valid)
-					step(j);
-		}
+		lookup_groups(L.cls, x, valid, clo, chi);
 		uint64_t Q;
 		gather_classes(clo, S, OC, OP, Q);
-		const uint32_t bs_lo = perm(chi[1], chi[0], 0x0C0C0400u) | perm(chi[3], chi[2], 0x04000C0Cu);
-		const uint32_t bs_hi = perm(chi[5], chi[4], 0x0C0C0400u) | perm(chi[7], chi[6], 0x04000C0Cu);
-		const uint64_t BS	 = ((uint64_t)bs_hi << 32) | bs_lo;
+		const uint64_t BS = gather_byte0(chi);
 
 		const uint32_t f0 = plan->fk_state[0], f1 = plan->fk_state[1], f2 = plan->fk_state[2],
 					   f3 = plan->fk_state[3];
@@ -311,6 +332,7 @@ struct EscapeEval {
 		oc = OC;
 		op = OP;
 	}
+
 };
 
 // ---------------------------------------------------------------------------
@@ -332,22 +354,8 @@ struct ParityEval {
 	uint32_t f_in, f_cl, doubled;
 
 	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
-		uint32_t clo[8];
-#pragma unroll
-		for (int g = 0; g < 8; ++g) clo[g] = 0;
-		auto step = [&](int j) {
-			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-			clo[j >> 3] |= L.cls[(j & 7) * 256 + c].x;
-		};
-		if (valid == kBPL) {
-#pragma unroll
-			for (int j = 0; j < kBPL; ++j) step(j);
-		} else {
-#pragma unroll
-			for (int j = 0; j < kBPL; ++j)
-				if ((uint32_t)j < valid)
-					step(j);
-		}
+		uint32_t clo[8], chi[8];
+		lookup_groups(L.cls, x, valid, clo, chi);
 		gather_classes(clo, S, OC, OP, Q);
 		const uint32_t f_out = plan->fk_state[0];
 		f_in				 = plan->fk_state[1];
@@ -387,6 +395,7 @@ struct ParityEval {
 		oc = OC;
 		op = OP;
 	}
+
 };
 
 __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
@@ -438,9 +447,11 @@ hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStrea
 	return hipGetLastError();
 }
 
+// rows = 16 KiB rows per tile; 0 picks `def` (measured per evaluator: the
+// bit-parallel paths amortise the look-back best over 64 KiB tiles).
 template<class E>
-hipError_t launch_rows(const ScanArgs& a, int rows, hipStream_t s) {
-	switch (rows) {
+hipError_t launch_rows(const ScanArgs& a, int rows, int def, hipStream_t s) {
+	switch (rows ? rows : def) {
 	case 1: return launch_eval<E, 1>(a, s);
 	case 4: return launch_eval<E, 4>(a, s);
 	default: return launch_eval<E, 2>(a, s);
@@ -450,20 +461,20 @@ hipError_t launch_rows(const ScanArgs& a, int rows, hipStream_t s) {
 hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s) {
 	if (a.n == 0)
 		return hipSuccess;
-	return launch_rows<EscapeEval>(a, rows, s);
+	return launch_rows<EscapeEval>(a, rows, 4, s);
 }
 
 hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s) {
 	if (a.n == 0)
 		return hipSuccess;
-	return launch_rows<ParityEval>(a, rows, s);
+	return launch_rows<ParityEval>(a, rows, 4, s);
 }
 
 hipError_t launch_scan_general(const ScanArgs& a, int nd, bool wide, int rows, hipStream_t s) {
 	if (a.n == 0)
 		return hipSuccess;
 	if (!wide && nd == 1)
-		return launch_rows<GeneralEval<1, false>>(a, rows, s);
+		return launch_rows<GeneralEval<1, false>>(a, rows, 2, s);
 	if (!wide) {
 		switch (nd) {
 		case 0: return launch_eval<GeneralEval<0, false>, 2>(a, s);

@@ -2,7 +2,8 @@
 
     python -m pars_amd.rocpd_summary <results.db> [more.db ...]
 
-Kernel-trace databases give per-kernel call counts and durations; counter
+Kernel-trace databases give per-kernel call counts and durations (the rocpd
+tables hold microseconds; printed here in milliseconds); counter
 databases (rocprofv3 --pmc) give the per-dispatch average of each collected
 counter (FETCH_SIZE / WRITE_SIZE are in KiB per dispatch).
 """
@@ -23,9 +24,9 @@ def summarize(path: str) -> str:
     out = [f"# {path}"]
     kernels = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
     if kernels:
-        out.append(f"{'kernel':70s} {'calls':>6s} {'avg_us':>10s} {'total_us':>11s} {'pct':>6s}")
+        out.append(f"{'kernel':70s} {'calls':>6s} {'avg_ms':>10s} {'total_ms':>11s} {'pct':>6s}")
         for name, calls, tot, avg, pct in kernels:
-            out.append(f"{short(name):70s} {calls:6d} {avg / 1e3:10.2f} {tot / 1e3:11.1f} {pct:6.2f}")
+            out.append(f"{short(name):70s} {calls:6d} {avg / 1e3:10.4f} {tot / 1e3:11.3f} {pct:6.2f}")
     rows = list(con.execute("select kernel_name, counter_name, value, duration from counters_collection"))
     if rows:
         acc = defaultdict(list)

@@ -103,8 +103,38 @@ typedef enum {
 	PARS_OPT_WANT_CHARS	 = 1, /* also emit structural bytes (default 0) */
 	PARS_OPT_WANT_BITMAP = 2, /* also emit the structural bitmap (default 1) */
 	PARS_OPT_PATH		 = 3, /* 0 = auto, 1 = force the general descriptor path */
-	PARS_OPT_ROWS_PER_TILE = 4 /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 */
+	PARS_OPT_ROWS_PER_TILE = 4, /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 */
+	PARS_OPT_GRAPH		   = 5  /* batches: capture the launch chain once per shape and replay it */
 } pars_option;
+
+/* One document of a batch: its bytes are d_in[offset, offset + size). */
+typedef struct {
+	uint64_t offset; /* a multiple of 64; documents in ascending order, not overlapping */
+	uint64_t size;	 /* >= 1 */
+} pars_doc_span;
+
+/* Results of pars_hip_scan_batch; every pointer is device memory owned by the
+ * scanner and valid until its next call.  Per-document runs are concatenated in
+ * document order: document d's structural positions are
+ * structural[struct_start[d] .. struct_start[d + 1]), byte offsets inside the
+ * document (what scanning the document alone yields); its brackets likewise
+ * through bracket_start, and bracket_pair holds document-local bracket
+ * indices (nest.hpp pair_table convention), -1 when unpaired. */
+typedef struct {
+	uint64_t		n_docs;
+	uint64_t		n_struct;		/* all documents */
+	uint64_t		n_brackets;
+	const uint32_t* structural;
+	const uint32_t* struct_start;	/* [n_docs + 1] */
+	const uint32_t* bracket_pos;	/* document-local byte offsets */
+	const uint8_t*	bracket_char;
+	const uint32_t* bracket_start;	/* [n_docs + 1] */
+	const int32_t*	bracket_pair;	/* with run_pairs; else NULL */
+	const int32_t*	pair_pos;		/* with want_pairs: document-local structural index of the partner, -1 */
+	const uint8_t*	doc_flags;		/* [n_docs] bit0 balanced, bit1 well_formed, bit2 unclosed_mode */
+	const int32_t*	max_depth;		/* [n_docs] */
+	const uint64_t* struct_bitmap;	/* bit per byte of d_in (bytes outside documents are 0), or NULL */
+} pars_batch_result;
 
 /* ---- scanner lifetime (reference: gpu::scanner<S> constructor/destructor) ---- */
 int pars_hip_scanner_create(const pars_plan* plan, pars_scanner** out);
@@ -145,6 +175,18 @@ int pars_hip_scan_chunk(
 int pars_hip_pair_chunks(
 	pars_scanner* sc, const pars_chunk_carry* last, pars_stream stream, int want_pairs,
 	pars_device_result* out
+);
+
+/* Many documents in one persistent launch (no reference counterpart: the
+ * reference runs pipeline.cuh:752-1017 once per document).  Segment heads
+ * reset the state carry and the depth; each document's results equal those of
+ * scanning it alone.  docs: host array of n_docs spans into d_in[0, n), n <
+ * 2^32.  Blocks until results are ready.  With PARS_OPT_GRAPH set and a
+ * non-default stream, the launch chain is captured into a HIP graph the first
+ * time a shape (d_in, n, n_docs, flags) is seen and replayed after that. */
+int pars_hip_scan_batch(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, const pars_doc_span* docs, size_t n_docs, pars_stream stream,
+	int want_pairs, int run_pairs, pars_stage_times* st, pars_batch_result* out
 );
 
 /* pipeline.cuh:1225 scanner::struct_ptr. */

@@ -142,6 +142,50 @@ class HostScan:
         self.unclosed_mode = bool(r.unclosed_mode)
 
 
+class BatchScan:
+    """pars_batch_result: device pointers into the scanner's arena, valid until its next call."""
+
+    def __init__(self, r: N.BatchResult):
+        self.n_docs = r.n_docs
+        self.n_struct = r.n_struct
+        self.n_brackets = r.n_brackets
+        self.structural = r.structural or 0
+        self.struct_start = r.struct_start or 0
+        self.bracket_pos = r.bracket_pos or 0
+        self.bracket_char = r.bracket_char or 0
+        self.bracket_start = r.bracket_start or 0
+        self.bracket_pair = r.bracket_pair or 0
+        self.pair_pos = r.pair_pos or 0
+        self.doc_flags = r.doc_flags or 0
+        self.max_depth = r.max_depth or 0
+        self.struct_bitmap = r.struct_bitmap or 0
+
+    def __repr__(self) -> str:
+        return f"BatchScan(n_docs={self.n_docs}, n_struct={self.n_struct}, n_brackets={self.n_brackets})"
+
+
+def pack_batch(docs, align: int = 64):
+    """Lay documents out for pars_hip_scan_batch: each starts on a 64-byte
+    boundary; the gaps are spaces.  Returns (buffer uint8 array, DocSpan array)."""
+    import numpy as np
+
+    sizes = np.array([len(d) for d in docs], dtype=np.uint64)
+    offs = np.zeros(len(docs), dtype=np.uint64)
+    if len(docs) > 1:
+        padded = (sizes + (align - 1)) // align * align
+        offs[1:] = np.cumsum(padded)[:-1]
+    total = int(offs[-1] + sizes[-1]) if len(docs) else 0
+    total = (total + align - 1) // align * align
+    buf = np.full(max(total, align), 0x20, dtype=np.uint8)
+    for o, d in zip(offs, docs):
+        buf[int(o):int(o) + len(d)] = np.frombuffer(bytes(d), dtype=np.uint8) if not isinstance(d, np.ndarray) else d
+    spans = (N.DocSpan * len(docs))()
+    for i in range(len(docs)):
+        spans[i].offset = int(offs[i])
+        spans[i].size = int(sizes[i])
+    return buf, spans
+
+
 class Scanner:
     """gpu::scanner<S> (pipeline.cuh:1164-1258) for a runtime plan."""
 
@@ -210,6 +254,15 @@ class Scanner:
         _check(self._lib.pars_hip_pair_chunks(self._h, C.byref(last), C.c_void_p(stream or 0),
                                               int(want_pairs), C.byref(r)))
         return DeviceScan(r, 0)
+
+    def scan_batch(self, d_data: int, n: int, spans, stream: int | None = None, want_pairs: bool = False,
+                   run_pairs: bool = True, stage_times: N.StageTimes | None = None) -> BatchScan:
+        """Many documents in one launch (pars_hip_scan_batch); spans from pack_batch."""
+        r = N.BatchResult()
+        _check(self._lib.pars_hip_scan_batch(
+            self._h, C.c_void_p(d_data), n, spans, len(spans), C.c_void_p(stream or 0), int(want_pairs),
+            int(run_pairs), C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
+        return BatchScan(r)
 
     def struct_ptr(self) -> int:
         return self._lib.pars_hip_struct_ptr(self._h) or 0

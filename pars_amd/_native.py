@@ -25,6 +25,7 @@ PARS_OPT_WANT_CHARS = 1
 PARS_OPT_WANT_BITMAP = 2
 PARS_OPT_PATH = 3
 PARS_OPT_ROWS_PER_TILE = 4
+PARS_OPT_GRAPH = 5
 
 
 class StageTimes(C.Structure):
@@ -73,6 +74,28 @@ class ChunkCarry(C.Structure):
     ]
 
 
+class DocSpan(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("size", C.c_uint64)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("n_docs", C.c_uint64),
+        ("n_struct", C.c_uint64),
+        ("n_brackets", C.c_uint64),
+        ("structural", C.c_void_p),
+        ("struct_start", C.c_void_p),
+        ("bracket_pos", C.c_void_p),
+        ("bracket_char", C.c_void_p),
+        ("bracket_start", C.c_void_p),
+        ("bracket_pair", C.c_void_p),
+        ("pair_pos", C.c_void_p),
+        ("doc_flags", C.c_void_p),
+        ("max_depth", C.c_void_p),
+        ("struct_bitmap", C.c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/pars_hip.h declares
 PROTOTYPES = {
     "pars_hip_scanner_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -94,6 +117,11 @@ PROTOTYPES = {
     ),
     "pars_hip_pair_chunks": (
         C.c_int, [C.c_void_p, C.POINTER(ChunkCarry), C.c_void_p, C.c_int, C.POINTER(DeviceResult)]
+    ),
+    "pars_hip_scan_batch": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(DocSpan), C.c_size_t, C.c_void_p, C.c_int, C.c_int,
+         C.POINTER(StageTimes), C.POINTER(BatchResult)],
     ),
     "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
     "pars_hip_last_error": (C.c_char_p, []),

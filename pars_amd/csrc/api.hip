@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 #include <new>
 #include <string>
 
@@ -91,7 +92,6 @@ struct pars_scanner {
 	DevBuf	  positions, chars, bitmap, pair_pos;
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
-	uint32_t  epoch		 = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
 	DevSummary* h_sum	 = nullptr;	 // pinned
 	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
@@ -103,11 +103,28 @@ struct pars_scanner {
 	size_t	  h_struct_cap = 0;
 	int32_t*  h_pairs	  = nullptr;
 	size_t	  h_pairs_cap = 0;
+	// batches: the document table (pinned staging + device), per-document results
+	DevBuf	  doc_off, doc_len, tile_doc, head_bits, st_start, br_start, doc_final, doc_min, doc_max, doc_last,
+		bdoc, doc_bad, doc_flags;
+	uint64_t* h_doc_off	 = nullptr;
+	uint32_t* h_doc_len	 = nullptr;
+	size_t	  h_off_cap = 0, h_len_cap = 0;
+	int		  use_graph	 = 0;
+	hipGraphExec_t			gexec = nullptr;
+	std::vector<uintptr_t>	gkey;  // shape and buffers the captured graph was made for
 
 	~pars_scanner() {
+		if (gexec)
+			(void)hipGraphExecDestroy(gexec);
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
-						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in})
+						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
+						  &doc_off, &doc_len, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min,
+						  &doc_max, &doc_last, &bdoc, &doc_bad, &doc_flags})
 			b->release();
+		if (h_doc_off)
+			(void)hipHostFree(h_doc_off);
+		if (h_doc_len)
+			(void)hipHostFree(h_doc_len);
 		if (h_sum)
 			(void)hipHostFree(h_sum);
 		if (h_in)
@@ -119,13 +136,6 @@ struct pars_scanner {
 		for (hipEvent_t& e : ev)
 			if (e)
 				(void)hipEventDestroy(e);
-	}
-
-	uint32_t next_epoch() {
-		epoch = (epoch + 1) & 0x3FFFFFFFu;
-		if (epoch == 0)
-			epoch = 1;
-		return epoch;
 	}
 };
 
@@ -180,7 +190,6 @@ ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool 
 	a.data		 = d_in;
 	a.n			 = n;
 	a.n_tiles	 = (uint32_t)((n + kTile - 1) / kTile);
-	a.epoch		 = sc->next_epoch();
 	a.plan		 = sc->dplan.as<DevPlan>();
 	a.recs		 = sc->recs.as<uint64_t>();
 	a.ticket	 = sc->ticket.as<uint32_t>();
@@ -287,6 +296,153 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 	return PARS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batches.
+// ---------------------------------------------------------------------------
+BatchArgs make_batch_args(pars_scanner* sc, uint64_t n, uint64_t n_docs) {
+	BatchArgs b {};
+	b.doc_off	   = sc->doc_off.as<uint64_t>();
+	b.doc_len	   = sc->doc_len.as<uint32_t>();
+	b.tile_doc	   = sc->tile_doc.as<int32_t>();
+	b.head_bits	   = sc->head_bits.as<uint64_t>();
+	b.n_head_words = (((n + 63) / 64) + 63) / 64;
+	b.n_docs	   = (uint32_t)n_docs;
+	b.st_start	   = sc->st_start.as<uint32_t>();
+	b.br_start	   = sc->br_start.as<uint32_t>();
+	b.doc_final	   = sc->doc_final.as<uint32_t>();
+	b.doc_min	   = sc->doc_min.as<int32_t>();
+	b.doc_max	   = sc->doc_max.as<int32_t>();
+	b.doc_last	   = sc->doc_last.as<int32_t>();
+	b.bdoc		   = sc->bdoc.as<uint32_t>();
+	return b;
+}
+
+int ensure_batch(pars_scanner* sc, uint64_t n, uint64_t n_docs, uint64_t cap_br, bool want_pairs) {
+	int			   r;
+	const uint64_t n_tiles = (n + kTile - 1) / kTile;
+	if ((r = ensure_outputs(sc, n_tiles, std::max<uint64_t>(n, sc->cap_struct), cap_br, want_pairs)))
+		return r;
+	if (sc->want_bitmap && (r = sc->bitmap.ensure(((n + 63) / 64) * 8)))
+		return r;
+	const uint64_t words = (((n + 63) / 64) + 63) / 64;
+	if ((r = sc->doc_off.ensure(n_docs * 8)) || (r = sc->doc_len.ensure(n_docs * 4))
+		|| (r = sc->tile_doc.ensure(n_tiles * 4)) || (r = sc->head_bits.ensure(std::max<uint64_t>(words, 1) * 8))
+		|| (r = sc->st_start.ensure((n_docs + 1) * 4)) || (r = sc->br_start.ensure((n_docs + 1) * 4))
+		|| (r = sc->doc_final.ensure(n_docs * 4)) || (r = sc->doc_min.ensure(n_docs * 4))
+		|| (r = sc->doc_max.ensure(n_docs * 4)) || (r = sc->doc_last.ensure(n_docs * 4))
+		|| (r = sc->doc_bad.ensure(n_docs * 4)) || (r = sc->doc_flags.ensure(n_docs))
+		|| (r = sc->bdoc.ensure(sc->cap_br * 4)))
+		return r;
+	if ((r = ensure_pinned((void**)&sc->h_doc_off, sc->h_off_cap, n_docs * 8))
+		|| (r = ensure_pinned((void**)&sc->h_doc_len, sc->h_len_cap, n_docs * 4)))
+		return r;
+	return PARS_OK;
+}
+
+hipError_t launch_batch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
+	if (sc->path == 0 && sc->hplan.fast_kind == kFastEscape)
+		return launch_batch_escape(a, sc->rows, s);
+	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
+		return launch_batch_parity(a, sc->rows, s);
+	return launch_batch_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
+}
+
+// The whole batch: no host sync inside, so it can be captured into a graph.
+int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_docs, hipStream_t s,
+				  bool want_pairs, bool run_pairs, bool timed) {
+	hipError_t e;
+	if ((e = launch_summary_reset(sc->summary.as<DevSummary>(), SummaryInit {}, s)) != hipSuccess)
+		return hip_fail(e, "summary reset");
+	if ((e = hipMemcpyAsync(sc->doc_off.p, sc->h_doc_off, n_docs * 8, hipMemcpyHostToDevice, s)) != hipSuccess
+		|| (e = hipMemcpyAsync(sc->doc_len.p, sc->h_doc_len, n_docs * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+		return hip_fail(e, "document table H2D");
+	const BatchArgs b = make_batch_args(sc, n, n_docs);
+	if ((e = launch_batch_heads(b, s)) != hipSuccess)
+		return hip_fail(e, "batch heads");
+	if (run_pairs && (e = launch_fill_i32(sc->doc_bad.as<int32_t>(), n_docs, 0, s)) != hipSuccess)
+		return hip_fail(e, "batch reset");
+	if (timed && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	ScanArgs a = make_scan_args(sc, d_in, n, want_pairs);
+	a.b		   = b;
+	if ((e = launch_batch_scan(sc, a, s)) != hipSuccess)
+		return hip_fail(e, "batch scan launch");
+	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if (run_pairs) {
+		PairArgs p = make_pair_args(sc, want_pairs);
+		p.bdoc	   = b.bdoc;
+		p.br_start = b.br_start;
+		p.st_start = b.st_start;
+		p.n_docs   = b.n_docs;
+		p.doc_bad  = sc->doc_bad.as<uint32_t>();
+		if ((e = launch_pairing(p, s)) != hipSuccess)
+			return hip_fail(e, "pairing launch");
+	}
+	const uint32_t nd = sc->hplan.n_descs;
+	const uint32_t m  = nd ? ((1u << (3 * nd)) - 1u) : 0u;
+	if ((e = launch_batch_finish(b, sc->doc_bad.as<uint32_t>(), m, sc->hplan.start_packed, run_pairs ? 1 : 0,
+								 sc->doc_flags.as<uint8_t>(), s))
+		!= hipSuccess)
+		return hip_fail(e, "batch finish");
+	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	return PARS_OK;
+}
+
+std::vector<uintptr_t> batch_graph_key(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_docs,
+									   hipStream_t s, bool want_pairs, bool run_pairs) {
+	std::vector<uintptr_t> k = {(uintptr_t)d_in, (uintptr_t)n, (uintptr_t)n_docs, (uintptr_t)s,
+								(uintptr_t)want_pairs, (uintptr_t)run_pairs, (uintptr_t)sc->want_chars,
+								(uintptr_t)sc->want_bitmap, (uintptr_t)sc->path, (uintptr_t)sc->rows,
+								(uintptr_t)sc->cap_struct, (uintptr_t)sc->cap_br, (uintptr_t)sc->h_doc_off,
+								(uintptr_t)sc->h_doc_len};
+	for (DevBuf* b : {&sc->recs, &sc->ticket, &sc->summary, &sc->positions, &sc->chars, &sc->bitmap, &sc->pair_pos,
+					  &sc->bpos, &sc->bsidx, &sc->bkey, &sc->bchar, &sc->ukeys0, &sc->ukeys1, &sc->order0,
+					  &sc->order1, &sc->rank, &sc->pair, &sc->hist, &sc->hist_part, &sc->doc_off, &sc->doc_len,
+					  &sc->tile_doc, &sc->head_bits, &sc->st_start, &sc->br_start, &sc->doc_final, &sc->doc_min,
+					  &sc->doc_max, &sc->doc_last, &sc->bdoc, &sc->doc_bad, &sc->doc_flags})
+		k.push_back((uintptr_t)b->p);
+	return k;
+}
+
+// Replay the captured chain for this shape.  A new shape runs eagerly once
+// (which also settles every lazily initialised launch parameter), is captured
+// on its second call and replayed from then on.
+int run_batch_graph(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_docs, hipStream_t s,
+					bool want_pairs, bool run_pairs) {
+	const std::vector<uintptr_t> key = batch_graph_key(sc, d_in, n, n_docs, s, want_pairs, run_pairs);
+	hipError_t					 e;
+	if (sc->gkey != key) {
+		if (sc->gexec) {
+			(void)hipGraphExecDestroy(sc->gexec);
+			sc->gexec = nullptr;
+		}
+		sc->gkey = key;
+		return enqueue_batch(sc, d_in, n, n_docs, s, want_pairs, run_pairs, false);
+	}
+	if (!sc->gexec) {
+		if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+			return hip_fail(e, "hipStreamBeginCapture");
+		const int	r = enqueue_batch(sc, d_in, n, n_docs, s, want_pairs, run_pairs, false);
+		hipGraph_t	g = nullptr;
+		e			  = hipStreamEndCapture(s, &g);
+		if (r)
+			return r;
+		if (e != hipSuccess)
+			return hip_fail(e, "hipStreamEndCapture");
+		e = hipGraphInstantiate(&sc->gexec, g, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(g);
+		if (e != hipSuccess) {
+			sc->gexec = nullptr;
+			return hip_fail(e, "hipGraphInstantiate");
+		}
+	}
+	if ((e = hipGraphLaunch(sc->gexec, s)) != hipSuccess)
+		return hip_fail(e, "hipGraphLaunch");
+	return PARS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -351,6 +507,7 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 			return fail(PARS_ERR_ARGUMENT, "rows per tile must be 0 (default), 1, 2 or 4");
 		sc->rows = (int)value;
 		return PARS_OK;
+	case PARS_OPT_GRAPH: sc->use_graph = value != 0; return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }
@@ -452,6 +609,88 @@ int pars_hip_reserve_struct(pars_scanner* sc, size_t cap) {
 		return r;
 	if (sc->want_chars && (r = sc->chars.ensure(sc->cap_struct)))
 		return r;
+	return PARS_OK;
+}
+
+int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const pars_doc_span* docs, size_t n_docs,
+						pars_stream stream, int want_pairs, int run_pairs, pars_stage_times* st,
+						pars_batch_result* out) {
+	if (!sc || !out || !docs || !n_docs || !d_in)
+		return fail(PARS_ERR_ARGUMENT, "null argument or empty batch");
+	if ((uint64_t)n > 0xFFFFFFFFull)
+		return fail(PARS_ERR_ARGUMENT, "a batch is limited to 2^32-1 bytes (uint32 output indices)");
+	uint64_t prev_end = 0;
+	for (size_t d = 0; d < n_docs; ++d) {
+		const pars_doc_span& x = docs[d];
+		if (x.offset % 64 || x.size == 0 || x.offset < prev_end || x.offset + x.size > n)
+			return fail(PARS_ERR_ARGUMENT,
+						"document %zu: spans must start on 64-byte boundaries, be non-empty, ascending, "
+						"non-overlapping and inside the buffer",
+						d);
+		prev_end = x.offset + x.size;
+	}
+	hipStream_t s = (hipStream_t)stream;
+	if (st)
+		*st = pars_stage_times {0, 0, 0, 0};
+	if (!sc->br_hint)
+		sc->br_hint = std::max<uint64_t>(n / 8, 1u << 16);
+	uint64_t cap_br = std::min<uint64_t>(std::max<uint64_t>(sc->br_hint, sc->cap_br), std::max<uint64_t>(n, 1));
+	int		 r;
+	if ((r = ensure_batch(sc, n, n_docs, cap_br, want_pairs != 0)))
+		return r;
+	for (size_t d = 0; d < n_docs; ++d) {
+		sc->h_doc_off[d] = docs[d].offset;
+		sc->h_doc_len[d] = (uint32_t)docs[d].size;
+	}
+	const bool graph = sc->use_graph && s != nullptr;
+	for (int attempt = 0; attempt < 2; ++attempt) {
+		if (graph) {
+			hipError_t e;
+			if (st && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+				return hip_fail(e, "hipEventRecord");
+			if ((r = run_batch_graph(sc, d_in, n, n_docs, s, want_pairs != 0, run_pairs != 0)))
+				return r;
+			if (st && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+				return hip_fail(e, "hipEventRecord");
+		} else if ((r = enqueue_batch(sc, d_in, n, n_docs, s, want_pairs != 0, run_pairs != 0, st != nullptr))) {
+			return r;
+		}
+		if ((r = read_summary(sc, s)))
+			return r;
+		if (sc->h_sum->n_brackets <= sc->cap_br)
+			break;
+		sc->br_hint = sc->h_sum->n_brackets;  // grow exactly and rerun
+		if ((r = ensure_batch(sc, n, n_docs, sc->h_sum->n_brackets, want_pairs != 0)))
+			return r;
+	}
+	if (sc->h_sum->overflow & 4u)
+		return fail(PARS_ERR_ARGUMENT, "batch pairing key (document, depth) exceeds 32 bits; split the batch");
+	if (st) {
+		float ms = 0;
+		if (graph) {
+			if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[2]) == hipSuccess)
+				st->p1 = ms;
+		} else {
+			if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
+				st->p1 = ms;
+			if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
+				st->p4 = ms;
+		}
+	}
+	std::memset(out, 0, sizeof *out);
+	out->n_docs		   = n_docs;
+	out->n_struct	   = sc->h_sum->n_struct;
+	out->n_brackets	   = sc->h_sum->n_brackets;
+	out->structural	   = sc->positions.as<uint32_t>();
+	out->struct_start  = sc->st_start.as<uint32_t>();
+	out->bracket_pos   = sc->bpos.as<uint32_t>();
+	out->bracket_char  = sc->bchar.as<uint8_t>();
+	out->bracket_start = sc->br_start.as<uint32_t>();
+	out->bracket_pair  = run_pairs ? sc->pair.as<int32_t>() : nullptr;
+	out->pair_pos	   = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	out->doc_flags	   = sc->doc_flags.as<uint8_t>();
+	out->max_depth	   = sc->doc_max.as<int32_t>();
+	out->struct_bitmap = sc->want_bitmap ? sc->bitmap.as<uint64_t>() : nullptr;
 	return PARS_OK;
 }
 

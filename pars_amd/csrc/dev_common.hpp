@@ -82,6 +82,9 @@ struct MapT<false> {
 
 	__device__ static Map4 readlane(Map4 m, int lane) { return Map4 {pars_hip::readlane(m.lo, lane)}; }
 
+	// every state -> v
+	__device__ static Map4 constant(uint32_t v) { return Map4 {v * 0x01010101u}; }
+
 	// 3-bit packed form used in look-back words (24 bits for 8 states)
 	__device__ static uint32_t pack(Map4 m) {
 		uint32_t r = 0;
@@ -130,6 +133,8 @@ struct MapT<true> {
 	__device__ static Map8 readlane(Map8 m, int lane) {
 		return Map8 {pars_hip::readlane(m.lo, lane), pars_hip::readlane(m.hi, lane)};
 	}
+
+	__device__ static Map8 constant(uint32_t v) { return Map8 {v * 0x01010101u, v * 0x01010101u}; }
 
 	__device__ static uint32_t pack(Map8 m) {
 		uint32_t r = 0;

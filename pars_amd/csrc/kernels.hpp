@@ -39,14 +39,41 @@ struct DevSummary {
 	uint32_t n_chunks;	   // pairing sort chunks
 	uint32_t overflow;	   // an output exceeded its capacity
 	uint32_t digit_bits;   // radix digit width of the pairing sort
-	uint32_t pad[4];
+	uint32_t key_span;	   // batch pairing: keys per document (kmax - kmin + 1)
+	// Look-back epoch of the current launch: advanced by every summary reset
+	// (never reset itself), so records of earlier launches - also of earlier
+	// replays of one captured graph - read as unpublished.
+	uint32_t epoch;
+	uint32_t pad[2];
+};
+
+// Batch mode: the input holds n_docs documents in ascending order, each
+// starting on a 64-byte boundary (a lane's first byte); bytes between
+// documents belong to none.  A document head resets the state carry to the
+// start states and the depth to 0; structural and bracket outputs stay
+// compacted across the whole batch (documents in order), with each document's
+// first index recorded.  All null / 0 outside batch mode.
+struct BatchArgs {
+	const uint64_t* doc_off;	  // [n_docs] first byte of each document
+	const uint32_t* doc_len;	  // [n_docs] bytes (>= 1)
+	int32_t*		tile_doc;	  // [n_tiles] last document starting at or before the tile (-1: none)
+	const uint64_t* head_bits;	  // bit g set: a document starts at byte 64 g
+	uint64_t		n_head_words;
+	uint32_t		n_docs;
+	uint32_t*		st_start;	  // [n_docs + 1] index of each document's first structural
+	uint32_t*		br_start;	  // [n_docs + 1] index of each document's first bracket
+	uint32_t*		doc_final;	  // [n_docs] packed states after the document's last byte
+	int32_t*		doc_min;	  // [n_docs] min depth after a bracket (starts at 0)
+	int32_t*		doc_max;	  // [n_docs] max depth after a bracket (starts at 0)
+	int32_t*		doc_last;	  // [n_docs] depth after the document
+	uint32_t*		bdoc;		  // [per bracket] document index
 };
 
 struct ScanArgs {
 	const uint8_t*	data;
 	uint64_t		n;
 	uint32_t		n_tiles;  // set by the launcher from the evaluator's rows per tile
-	uint32_t		epoch;
+	uint32_t		epoch;	  // filled in by the kernel from sum->epoch
 	const DevPlan*	plan;
 	uint64_t*		recs;	  // one record per tile; sized for one-row tiles
 	uint32_t*		ticket;	  // [0] next tile, [1] blocks finished; both 0 between launches
@@ -67,6 +94,7 @@ struct ScanArgs {
 	uint32_t		br_base;	 // bracket index of the first bracket (chunked mode)
 	int32_t			depth_base;	 // depth before the first byte (chunked mode)
 	uint32_t		in_states;	 // packed per-descriptor states before the first byte
+	BatchArgs		b;			 // batch mode only
 };
 
 // R1-R5: one pass over the input, general descriptor path.  `rows` = 16 KiB
@@ -76,6 +104,19 @@ hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, int ro
 hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s);
 // R1-R5 for plans whose one descriptor is quote parity (CSV doubled quotes).
 hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
+// The same three over a batch (a.b filled in); the document heads must be set.
+hipError_t launch_batch_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
+hipError_t launch_batch_escape(const ScanArgs& a, int rows, hipStream_t s);
+hipError_t launch_batch_parity(const ScanArgs& a, int rows, hipStream_t s);
+// Batch preparation: clear and set the head bitmap, and reset the per-document
+// depth statistics.
+hipError_t launch_batch_heads(const BatchArgs& b, hipStream_t s);
+// tile_doc for tiles of `tile_bytes` (called by the batch launchers).
+hipError_t launch_tile_doc(const BatchArgs& b, uint32_t n_tiles, uint64_t tile_bytes, hipStream_t s);
+// Per-document flags once R6 has run: bit0 balanced, bit1 well_formed, bit2
+// unclosed_mode; start_packed = the plan's start states.
+hipError_t launch_batch_finish(const BatchArgs& b, const uint32_t* doc_bad, uint32_t start_mask,
+							   uint32_t start_packed, int run_pairs, uint8_t* doc_flags, hipStream_t s);
 // Initial summary values: a fresh document starts all at 0; a chunk of a
 // longer document starts from the running values its carry holds.
 struct SummaryInit {
@@ -106,6 +147,13 @@ struct PairArgs {
 	int32_t*	   pair;	  // CPU convention, per bracket index (optional)
 	int32_t*	   pair_pos;  // device convention, per structural index (optional)
 	uint32_t	   out_base;  // structural index base of pair_pos entries
+	// batch mode: sort by (document, key); pair within a document; indices in
+	// the outputs are document-local
+	const uint32_t* bdoc;
+	const uint32_t* br_start;
+	const uint32_t* st_start;
+	uint32_t		n_docs;
+	uint32_t*		doc_bad;  // [n_docs] a sorted pair of the document does not match
 };
 
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s);

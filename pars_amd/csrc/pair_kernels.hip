@@ -39,14 +39,21 @@ __device__ __forceinline__ uint32_t n_brackets(const PairArgs& a) {
 __global__ void k_sort_prep(PairArgs a) {
 	if (threadIdx.x != 0 || blockIdx.x != 0)
 		return;
-	DevSummary*	   s	 = a.sum;
-	const uint32_t nb	 = n_brackets(a);
-	const int32_t  kmin	 = s->min_bal - 1;	// open keys are depth-1, depths >= min_bal
-	const int32_t  kmax	 = s->max_bal;
-	const uint32_t range = (uint32_t)(kmax - kmin);
-	const uint32_t bits	 = range ? 32u - (uint32_t)__clz(range) : 1u;
-	const uint32_t p	 = (bits + 7u) / 8u;
-	s->kmin				 = kmin;
+	DevSummary*	   s	= a.sum;
+	const uint32_t nb	= n_brackets(a);
+	const int32_t  kmin = s->min_bal - 1;  // open keys are depth-1, depths >= min_bal
+	const int32_t  kmax = s->max_bal;
+	// batch: the sort key is (document, key) as document * span + (key - kmin)
+	const uint64_t span	 = (uint64_t)(kmax - kmin) + 1u;
+	const uint64_t range = a.bdoc ? (uint64_t)a.n_docs * span - 1u : (uint64_t)(kmax - kmin);
+	uint32_t	   bits	 = range ? 64u - (uint32_t)__clzll((long long)range) : 1u;
+	if (bits > 32u) {  // more than one 32-bit sort key can hold
+		s->overflow |= 4u;
+		bits = 32u;
+	}
+	const uint32_t p = (bits + 7u) / 8u;
+	s->key_span		 = (uint32_t)span;
+	s->kmin			 = kmin;
 	s->sort_passes		 = p;
 	s->digit_bits		 = (bits + p - 1) / p;
 	s->n_chunks			 = (nb + kChunk - 1) / kChunk;
@@ -55,7 +62,10 @@ __global__ void k_sort_prep(PairArgs a) {
 }
 
 __device__ __forceinline__ uint32_t key_at(const PairArgs& a, int pass, uint32_t i, int32_t kmin) {
-	return pass == 0 ? (uint32_t)(a.bkey[i] - kmin) : a.ukeys[(pass - 1) & 1][i];
+	if (pass != 0)
+		return a.ukeys[(pass - 1) & 1][i];
+	const uint32_t k = (uint32_t)(a.bkey[i] - kmin);
+	return a.bdoc ? a.bdoc[i] * a.sum->key_span + k : k;
 }
 
 template<int PASS>
@@ -231,22 +241,33 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 	const DevPlan*	P	  = a.plan;
 	bool			bad	  = false;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += gridDim.x * blockDim.x) {
-		const uint32_t p = a.rank[i], q = p ^ 1u;
+		// sorted positions count from the document's first bracket (batch) or 0
+		uint32_t d = 0, b0 = 0, n = nb, s0 = 0;
+		if (a.bdoc) {
+			d  = a.bdoc[i];
+			b0 = a.br_start[d];
+			n  = a.br_start[d + 1] - b0;
+			s0 = a.st_start[d];
+		}
+		const uint32_t p = a.rank[i] - b0, q = p ^ 1u;
 		int32_t		   partner = -1;
-		if (q < nb) {
-			const uint32_t j  = order[q];
+		if (q < n) {
+			const uint32_t j  = order[b0 + q];
 			const uint32_t ia = (p & 1u) ? j : i, ib = (p & 1u) ? i : j;
 			const uint8_t  ca = a.bchar[ia], cb = a.bchar[ib];
 			const bool ok = (P->br_flags[ca] & 1u) && (P->br_flags[cb] & 2u) && P->br_close_of[ca] == cb;
 			if (ok)
 				partner = (int32_t)j;
-			else if (!(p & 1u))
+			else if (!(p & 1u)) {
 				bad = true;	 // one report per sorted pair
+				if (a.doc_bad)
+					a.doc_bad[d] = 1u;
+			}
 		}
 		if (a.pair)
-			a.pair[i] = partner;
+			a.pair[i] = partner >= 0 ? partner - (int32_t)b0 : -1;
 		if (a.pair_pos)
-			a.pair_pos[a.bsidx[i]] = partner >= 0 ? (int32_t)a.bsidx[partner] : -1;
+			a.pair_pos[a.bsidx[i]] = partner >= 0 ? (int32_t)(a.bsidx[partner] - s0) : -1;
 	}
 	if (__ballot(bad) && lane_id() == 0)
 		atomicOr(&a.sum->mismatched, 1u);

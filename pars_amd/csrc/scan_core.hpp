@@ -31,6 +31,7 @@
 #include "kernels.hpp"
 
 #include <limits.h>
+#include <type_traits>
 
 namespace pars_hip {
 
@@ -43,6 +44,9 @@ constexpr uint32_t kErrLookback = 0x80000000u;
 // Predecessor records read per look-back round trip (the newest lanes of the
 // wave); the nearest inclusive prefix is typically 10-20 tiles back.
 constexpr int kLbWindow = 32;
+// Bit 31 of an aggregate's map word (one chain) or structural-count word (two
+// chains): the tile holds a document head (batch mode).
+constexpr uint32_t kHeadFlag = 0x80000000u;
 
 // 4x4 byte transpose: out[k] byte g = in[g] byte k.
 __device__ __forceinline__ void transpose4(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t& t0,
@@ -175,6 +179,9 @@ __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_
 }
 
 // Two-chain mode, chain 2: (structural, bracket, depth) offsets entering tile > 0.
+// A count aggregate whose structural word carries kHeadFlag holds a document
+// head: its depth word is the depth after its last head, and older depths do
+// not reach past it (structural and bracket offsets always do).
 __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint32_t epoch, const uint32_t base[3],
 									   uint32_t out[3], DevSummary* sum) {
 	const int  lane	  = lane_id();
@@ -182,13 +189,14 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 	uint32_t   acc[3] = {0u, 0u, 0u};
 	int64_t	   jb	  = (int64_t)tile - 1;
 	uint32_t   spins  = 0;
-	bool	   load = true, incl = false, none = false;
+	bool	   load = true, incl = false, none = false, head = false, closed = false;
 	uint32_t   v[3] = {0u, 0u, 0u};
 	while (true) {
 		const int64_t j = jb - 63 + lane;
 		if (!active || j < 0) {
 			incl = active;
 			none = false;
+			head = false;
 #pragma unroll
 			for (int f = 0; f < 3; ++f) v[f] = active ? base[f] : 0u;
 		} else if (load) {
@@ -207,8 +215,11 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 			}
 			incl = inc;
 			none = !inc && !agg;
+			head = !inc && (wa[0] & kHeadFlag);
 #pragma unroll
 			for (int f = 0; f < 3; ++f) v[f] = (uint32_t)(inc ? wi[f] : wa[f]);
+			if (!inc)
+				v[0] &= ~kHeadFlag;
 		}
 		const Window w = window_extent(incl, none);
 		if (!window_ready(w)) {
@@ -224,8 +235,16 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 			load = none;
 			continue;
 		}
-#pragma unroll
-		for (int f = 0; f < 3; ++f) acc[f] += wave_sum(lane >= w.kp ? v[f] : 0u);
+		// depth restarts at the newest document head (H) newer than the inclusive
+		const uint64_t hb = __ballot(head && lane > w.kp);
+		const int	   H  = hb ? 63 - (int)__clzll((long long)hb) : -1;
+		acc[0] += wave_sum(lane >= w.kp ? v[0] : 0u);
+		acc[1] += wave_sum(lane >= w.kp ? v[1] : 0u);
+		const bool own_d = H >= 0 ? lane >= H : lane >= w.kp;
+		const uint32_t wd = wave_sum(own_d ? v[2] : 0u);
+		if (!closed)
+			acc[2] += wd;
+		closed = closed || hb;
 		if (w.kp >= 0) {
 			out[0] = acc[0];
 			out[1] = acc[1];
@@ -238,7 +257,8 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 }
 
 // One-chain mode: the state entering tile > 0 (`st`) and the counts before it
-// (`cnt`).  A tile's aggregate is its map m plus v[f][e], its counts when
+// (`cnt`); a map word with kHeadFlag marks a tile holding a document head,
+// whose depth counts are the depth after its last head.  A tile's aggregate is its map m plus v[f][e], its counts when
 // entered in state e.  Folding the aggregates newer than an inclusive (S, C):
 // lane L enters in s_L = (m_{L-1} . ... . m_{kp+1})(S) and adds v_L[f][s_L].
 // A window without an inclusive is folded into `acc` as a function of its
@@ -258,7 +278,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 	}
 	int64_t	 jb	   = (int64_t)tile - 1;
 	uint32_t spins = 0;
-	bool	 load = true, incl = false, none = false;
+	bool	 load = true, incl = false, none = false, head = false, closed = false;
 	uint32_t iS = 0, iC[3] = {0u, 0u, 0u};
 	Map		 m = M::id();
 	uint32_t v[3][NS];
@@ -267,6 +287,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 		if (!active || j < 0) {	 // outside the window: neutral; before the input: the carry
 			incl = active;
 			none = false;
+			head = false;
 			iS	 = in_states;
 			m	 = M::id();
 #pragma unroll
@@ -291,6 +312,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 			for (int q = 0; q < 3 * NS; ++q) agg = agg && word_is(wv[q], epoch, kAgg);
 			incl  = inc;
 			none  = !inc && !agg;
+			head  = !inc && ((uint32_t)wm & kHeadFlag);
 			iS	  = (uint32_t)wi[0];
 			iC[0] = (uint32_t)wi[1];
 			iC[1] = (uint32_t)wi[2];
@@ -316,10 +338,14 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 			load = none;  // only records still missing are read again
 			continue;
 		}
-		const bool agg_lane = lane > w.kp;
-		const Map  inc		= wave_scan_maps<WIDE>(agg_lane ? m : M::id());
-		const Map  exc		= wave_shift_maps<WIDE>(inc);
-		const Map  wmap		= M::readlane(inc, 63);
+		const bool	   agg_lane = lane > w.kp;
+		const Map	   inc		= wave_scan_maps<WIDE>(agg_lane ? m : M::id());
+		const Map	   exc		= wave_shift_maps<WIDE>(inc);
+		const Map	   wmap		= M::readlane(inc, 63);
+		// depth restarts at the newest document head (H) newer than the inclusive
+		const uint64_t hb	 = __ballot(head && agg_lane);
+		const int	   H	 = hb ? 63 - (int)__clzll((long long)hb) : -1;
+		const bool	   own_d = H >= 0 ? lane >= H : agg_lane;
 		if (w.kp >= 0) {
 			const uint32_t S   = readlane(iS, w.kp) & 7u;
 			const uint32_t s_l = M::apply(exc, S);
@@ -327,11 +353,16 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 			st				   = pick(accS, so);
 #pragma unroll
 			for (int f = 0; f < 3; ++f) {
-				const uint32_t own = agg_lane ? pick(v[f], s_l) : (lane == w.kp ? iC[f] : 0u);
-				uint32_t	   a[NS];
+				uint32_t own;
+				if (f < 2)
+					own = agg_lane ? pick(v[f], s_l) : (lane == w.kp ? iC[f] : 0u);
+				else
+					own = own_d ? pick(v[f], s_l) : (lane == w.kp && H < 0 ? iC[f] : 0u);
+				uint32_t a[NS];
 #pragma unroll
 				for (int e = 0; e < NS; ++e) a[e] = accC[e][f];
-				cnt[f] = wave_sum(own) + pick(a, so);
+				const uint32_t ws = wave_sum(own);
+				cnt[f]			  = (f == 2 && closed) ? pick(a, so) : ws + pick(a, so);
 			}
 			return;
 		}
@@ -347,7 +378,9 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 				uint32_t a[NS];
 #pragma unroll
 				for (int q = 0; q < NS; ++q) a[q] = accC[q][f];
-				nC[e][f] = wave_sum(pick(v[f], s_l)) + pick(a, we);
+				const bool	   take = f < 2 ? agg_lane : own_d;
+				const uint32_t ws	= wave_sum(take ? pick(v[f], s_l) : 0u);
+				nC[e][f]			= (f == 2 && closed) ? pick(a, we) : ws + pick(a, we);
 			}
 		}
 #pragma unroll
@@ -356,6 +389,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 #pragma unroll
 			for (int f = 0; f < 3; ++f) accC[e][f] = nC[e][f];
 		}
+		closed = closed || hb;
 		jb -= kLbWindow;
 		load = true;
 	}
@@ -369,6 +403,13 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 // outputs are one contiguous run: they are staged in LDS (row-local 16-bit
 // offsets) and written back with coalesced stores (scattered 4-byte stores
 // amplify HBM writes several-fold).
+//
+// BATCH: the input is a batch of documents (ScanArgs::b).  A lane belongs to
+// one document (documents start on lane boundaries); at a head lane the state
+// maps become constant maps to the document's start-state image, so ordinary
+// map scans and look-backs restart there, and depth is scanned per document.
+// Positions and bracket keys come out document-relative; output arrays stay
+// compacted across the batch.
 //
 // E (the lane evaluator) provides:
 //   kND, kWide, Map        descriptors and their map type
@@ -388,15 +429,18 @@ constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
 template<class E>
 constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : 1;
 
-template<class E, int W>
+template<class E, int W, bool BATCH>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E>))) void k_scan(ScanArgs a) {
 	constexpr int  ND	 = E::kND;
 	constexpr int  NDA	 = ND > 0 ? ND : 1;
 	constexpr bool FUSED = ND == 1;
 	constexpr int  NS	 = E::kWide ? 8 : 4;  // entry states a one-descriptor tile is summarised for
 	constexpr int  NPK	 = NS / 4;			  // count dwords per field, four states each
+	constexpr int  NL	 = W * kBlock;		  // lanes per tile
+	constexpr int  NHW	 = NL / 64;			  // head-bitmap words per tile
 	using M				 = MapT<E::kWide>;
 	using Map			 = typename M::type;
+	using KeyT			 = typename std::conditional<BATCH, int32_t, int16_t>::type;
 	constexpr uint64_t kTileBytes = (uint64_t)W * kRowBytes;
 
 	__shared__ typename E::Lds s_lds;
@@ -404,16 +448,28 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 	__shared__ uint32_t		   s_misc[8];
 	__shared__ uint32_t		   s_wcnt[W][4][3];
 	__shared__ uint32_t		   s_vsum[4][3][2 * NPK];  // one-chain: per-wave 16-bit-pair count sums
+	__shared__ int32_t		   s_vpost[4];			   // one-chain batch: per-wave depth after the last head
 	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
 	__shared__ int32_t		   s_mm[4][2];
 	__shared__ uint16_t		   s_pos[kCapP];	// row-local byte offset
 	__shared__ uint16_t		   s_bpos[kCapB];	// row-local byte offset
 	__shared__ uint16_t		   s_bsidx[kCapB];	// row-local structural index
-	__shared__ int16_t		   s_bkey[kCapB];	// pairing key relative to the row's entry depth
+	__shared__ KeyT			   s_bkey[kCapB];	// key: relative to the row's entry depth / document-relative
+	__shared__ uint64_t		   s_hw[BATCH ? NHW : 1];		// the tile's head bits
+	__shared__ uint32_t		   s_hpre[BATCH ? NHW : 1];		// heads in the words before
+	__shared__ int32_t		   s_dhead[BATCH ? NL : 1];		// tile-relative depth at each head, by slot
+	__shared__ uint32_t		   s_lpos0[BATCH ? NL : 1];		// document-local offset of each lane's first byte
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	a.epoch		  = a.sum->epoch;  // set by the summary reset that precedes every launch
 	E::load(s_lds, a.plan);
 	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
+
+	// batch: lane L of the tile (row-major) -> its document slot (document d0 + slot)
+	auto slot_of = [&](int L) -> uint32_t {
+		const uint64_t hw = s_hw[L >> 6];
+		return s_hpre[L >> 6] + popc64(hw & (~0ull >> (63 - (L & 63)))) - (uint32_t)(s_hw[0] & 1ull);
+	};
 
 	// A block takes its next tile only when it is ready to start it: a ticket
 	// drawn early leaves a hole that every later tile's look-back waits on.
@@ -427,6 +483,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 			break;
 		const uint64_t tbase = (uint64_t)tile * kTileBytes;
 		uint64_t*	   rec	 = a.recs + (uint64_t)tile * kRecWords;
+
+		// ---- 0. batch: documents of the tile's lanes ----
+		int32_t	 d0 = 0;
+		int32_t	 ldoc[W];		  // document of the lane (-1: none)
+		bool	 lhead[W];		  // the lane starts its document
+		bool	 llast[W];		  // the lane holds its document's last byte
+		uint32_t vdoc[W];		  // the lane's bytes inside its document
+		int		 last_head = -1;  // last head lane of the tile
+		if constexpr (BATCH) {
+			if (tid < NHW) {
+				const uint64_t wi = (uint64_t)tile * NHW + tid;
+				s_hw[tid]		  = wi < a.b.n_head_words ? a.b.head_bits[wi] : 0ull;
+			}
+			if (tid == 0)
+				s_misc[6] = (uint32_t)a.b.tile_doc[tile];
+			__syncthreads();
+			if (wid == 0) {	 // exclusive prefix of the per-word head counts
+				const uint32_t c   = lane < NHW ? popc64(s_hw[lane]) : 0u;
+				const uint32_t inc = wave_incl_sum(c);
+				if (lane < NHW)
+					s_hpre[lane] = inc - c;
+			}
+			__syncthreads();
+			d0 = (int32_t)s_misc[6];
+#pragma unroll
+			for (int q = NHW - 1; q >= 0; --q)
+				if (last_head < 0 && s_hw[q])
+					last_head = q * 64 + 63 - (int)__clzll((long long)s_hw[q]);
+#pragma unroll
+			for (int r = 0; r < W; ++r) {
+				const int	   L	  = r * kBlock + tid;
+				const uint64_t lstart = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
+				lhead[r]			  = (s_hw[L >> 6] >> (L & 63)) & 1ull;
+				ldoc[r]				  = d0 + (int32_t)slot_of(L);
+				vdoc[r]				  = 0;
+				llast[r]			  = false;
+				if (ldoc[r] >= 0 && (uint32_t)ldoc[r] < a.b.n_docs) {
+					const uint64_t off = a.b.doc_off[ldoc[r]], end = off + a.b.doc_len[ldoc[r]];
+					if (lstart < end) {
+						vdoc[r]	 = (uint32_t)min<uint64_t>((uint64_t)kBPL, end - lstart);
+						llast[r] = lstart + kBPL >= end;
+					}
+					s_lpos0[L] = (uint32_t)(lstart - off);
+				} else {
+					ldoc[r] = -1;
+				}
+			}
+		}
 
 		// ---- 1. one walk per row ----
 		// Rows are loaded one ahead of the walk (two row buffers in flight, not W).
@@ -457,6 +561,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					x[k] = w;
 				}
 			}
+			if constexpr (BATCH)
+				valid[r] = min(valid[r], vdoc[r]);
 		};
 		load_row(0, xb[0]);
 #pragma unroll
@@ -466,6 +572,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				__asm__ volatile("" ::: "memory");	// keep later rows' loads behind this walk
 			}
 			ev[r].walk(s_lds, a.plan, xb[r & 1], valid[r]);
+			if constexpr (BATCH && ND > 0)
+				if (lhead[r])  // restart: whatever came before, the lane enters in the start states
+#pragma unroll
+					for (int d = 0; d < ND; ++d)
+						ev[r].T[d] = M::constant(M::apply(ev[r].T[d], (a.in_states >> (3 * d)) & 7u));
 		}
 
 		// ---- 2. entry states (and, one-chain, output offsets) ----
@@ -474,6 +585,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		uint32_t	   cex[3]	= {cbase[0], cbase[1], cbase[2]};
 		Map			   lpre[W][NDA];
 		uint32_t	   row_in[W];
+		uint32_t	   post = 0;  // batch: depth after the tile's last head
 		if constexpr (ND > 0) {
 			Map exc[W][ND];
 #pragma unroll
@@ -503,6 +615,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					tagg[d]	   = M::compose(wp, tagg[d]);
 				}
 			}
+			if constexpr (BATCH)
+#pragma unroll
+				for (int r = 0; r < W; ++r)
+					if (lhead[r])
+#pragma unroll
+						for (int d = 0; d < ND; ++d) lpre[r][d] = M::constant((a.in_states >> (3 * d)) & 7u);
 
 			if constexpr (FUSED) {
 				// The lane's counts for every lane-entry state, one byte per state
@@ -510,6 +628,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				// the lane's entry map, then summed as 16-bit pairs over rows and
 				// lanes (at most 4 x 128 x 64 = 32768 per field per wave).
 				uint32_t acc16[3][2 * NPK];
+				int32_t	 dpost = 0;	 // batch: depth deltas from the tile's last head on
 #pragma unroll
 				for (int f = 0; f < 3; ++f)
 #pragma unroll
@@ -549,6 +668,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 							acc16[f][2 * h] += g[h] & 0x00FF00FFu;
 							acc16[f][2 * h + 1] += (g[h] >> 8) & 0x00FF00FFu;
 						}
+						if constexpr (BATCH)
+							if (f == 2 && r * kBlock + tid >= last_head && last_head >= 0)
+								dpost += (int32_t)(g[0] & 0xFFu) - 64;	// entry-independent after a head
 					}
 					rowpre = M::compose(ragg[r][0], rowpre);
 				}
@@ -560,7 +682,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 						if (lane == 0)
 							s_vsum[wid][f][q] = t;
 					}
+				if constexpr (BATCH) {
+					const uint32_t t = wave_sum((uint32_t)dpost);
+					if (lane == 0)
+						s_vpost[wid] = (int32_t)t;
+				}
 				__syncthreads();
+				if constexpr (BATCH)
+					post = (uint32_t)(s_vpost[0] + s_vpost[1] + s_vpost[2] + s_vpost[3]);
 				if (tid < 3 * NS) {	 // thread (f, e): field e of the four wave sums
 					const int f = tid / NS, e = tid % NS;
 					const int q = 2 * (e >> 2) + (e & 1), sh = (e & 2) ? 16 : 0;
@@ -569,13 +698,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					for (int w2 = 0; w2 < 4; ++w2) t += (s_vsum[w2][f][q] >> sh) & 0xFFFFu;
 					if (f == 2)
 						t -= 64u * (uint32_t)(kBlock * W);
+					if (BATCH && f == 2 && last_head >= 0)
+						t = post;
 					s_v[f][e] = t;
 					if (tile != 0)
 						publish(rec + kRecFCnt + 3 * e + f, make_word(a.epoch, kAgg, t));
 				}
 				if (tile != 0) {
 					if (tid == 0)
-						publish(rec + kRecFMap, make_word(a.epoch, kAgg, M::pack(tagg[0])));
+						publish(rec + kRecFMap, make_word(a.epoch, kAgg,
+														  M::pack(tagg[0]) | (last_head >= 0 ? kHeadFlag : 0u)));
 					if (wid == 0) {
 						uint32_t st, o[3];
 						lookback_fused<E::kWide, NS>(a.recs, tile, a.epoch, a.in_states, cbase, st, o, a.sum);
@@ -621,8 +753,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					const uint32_t e = tile_in & 7u;
 					publish(rec + kRecFIncl, make_word(a.epoch, kIncl, cur));
 #pragma unroll
-					for (int f = 0; f < 3; ++f)
-						publish(rec + kRecFIncl + 1 + f, make_word(a.epoch, kIncl, cex[f] + s_v[f][e]));
+					for (int f = 0; f < 3; ++f) {
+						const uint32_t v = (BATCH && f == 2 && last_head >= 0) ? post : cex[f] + s_v[f][e];
+						publish(rec + kRecFIncl + 1 + f, make_word(a.epoch, kIncl, v));
+					}
 				} else {
 					publish(rec + kRecSIncl, make_word(a.epoch, kIncl, cur));
 				}
@@ -639,18 +773,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		// ---- 3. masks and in-tile offsets ----
 		uint64_t sm[W], om[W], opm[W];
 		uint32_t cnt[W][3];
+		uint32_t lexit[W];	// batch: packed states after the lane
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
 			uint32_t s0[NDA];
 #pragma unroll
 			for (int d = 0; d < ND; ++d) s0[d] = M::apply(lpre[r][d], (row_in[r] >> (3 * d)) & 7u);
+			if constexpr (BATCH) {
+				lexit[r] = 0;
+#pragma unroll
+				for (int d = 0; d < ND; ++d) lexit[r] |= M::apply(ev[r].T[d], s0[d]) << (3 * d);
+			}
 			const uint64_t masked = ND > 0 ? ev[r].masked(s0) : 0ull;
 			uint64_t	   S, OC, OP;
 			ev[r].classes(S, OC, OP);
 			sm[r]  = S & ~masked;
 			om[r]  = OC & ~masked;
 			opm[r] = OP & ~masked;
-			if (a.bitmap && valid[r])
+			if (a.bitmap && (tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL) < a.n)
 				a.bitmap[(tbase + (uint64_t)r * kRowBytes) / 64 + tid] = sm[r];
 			cnt[r][0] = popc64(sm[r]);
 			cnt[r][1] = popc64(om[r]);
@@ -685,13 +825,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					ctot[f] += rtot[r][f];
 				}
 		}
+		// batch: tile-relative depth at each head; the depth after the last head
+		if constexpr (BATCH) {
+			int32_t rd = 0;
+#pragma unroll
+			for (int r = 0; r < W; ++r) {
+				if (lhead[r])
+					s_dhead[ldoc[r] - d0] = rd + (int32_t)cpre[r][2];
+				rd += (int32_t)rtot[r][2];
+			}
+			__syncthreads();
+			if constexpr (!FUSED)
+				if (last_head >= 0)
+					post = (uint32_t)((int32_t)ctot[2] - s_dhead[slot_of(last_head)]);
+		}
 
 		// two-chain mode, chain 2: tile output offsets
 		if constexpr (!FUSED) {
+			const uint32_t dtot = (BATCH && last_head >= 0) ? post : ctot[2];
 			if (tile != 0) {
-				if (tid == 0)
-#pragma unroll
-					for (int f = 0; f < 3; ++f) publish(rec + kRecCAgg + f, make_word(a.epoch, kAgg, ctot[f]));
+				if (tid == 0) {
+					publish(rec + kRecCAgg + 0,
+							make_word(a.epoch, kAgg, ctot[0] | (BATCH && last_head >= 0 ? kHeadFlag : 0u)));
+					publish(rec + kRecCAgg + 1, make_word(a.epoch, kAgg, ctot[1]));
+					publish(rec + kRecCAgg + 2, make_word(a.epoch, kAgg, dtot));
+				}
 				if (wid == 0) {
 					uint32_t o[3];
 					lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
@@ -706,15 +864,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				cex[1] = s_misc[3];
 				cex[2] = s_misc[4];
 			}
-			if (tid == 0)
-#pragma unroll
-				for (int f = 0; f < 3; ++f)
-					publish(rec + kRecCIncl + f, make_word(a.epoch, kIncl, cex[f] + ctot[f]));
+			if (tid == 0) {
+				publish(rec + kRecCIncl + 0, make_word(a.epoch, kIncl, cex[0] + ctot[0]));
+				publish(rec + kRecCIncl + 1, make_word(a.epoch, kIncl, cex[1] + ctot[1]));
+				publish(rec + kRecCIncl + 2,
+						make_word(a.epoch, kIncl, (BATCH && last_head >= 0) ? post : cex[2] + ctot[2]));
+			}
 		}
 		if (tid == 0 && tile == a.n_tiles - 1) {
 			a.sum->n_struct	  = cex[0] + ctot[0];
 			a.sum->n_brackets = cex[1] + ctot[1];
 			a.sum->last_bal	  = (int32_t)(cex[2] + ctot[2]);
+			if constexpr (BATCH) {
+				a.b.st_start[a.b.n_docs] = cex[0] + ctot[0];
+				a.b.br_start[a.b.n_docs] = cex[1] + ctot[1];
+			}
 		}
 		if (a.pair_pos)	 // -1 for every structural index of the tile; R6 overwrites brackets
 			for (uint32_t i = tid; i < ctot[0]; i += kBlock)
@@ -722,16 +886,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					a.pair_pos[cex[0] + i] = -1;
 
 		// ---- 4. scatter, row by row ----
-		// Depths below are relative to the tile's entry depth until the end.
+		// Depths are relative to the tile's entry depth until the end; in batch
+		// mode each lane adds its document's offset (doc-relative depths).
 		uint32_t rb[3] = {cex[0], cex[1], cex[2]};
 		int32_t	 rowd = 0;	// the row's entry depth
 		int32_t	 lmin = INT_MAX, lmax = INT_MIN;
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
-			const uint64_t rowb	  = tbase + (uint64_t)r * kRowBytes;  // data index of the row
-			const uint32_t loff	  = (uint32_t)tid * kBPL;			  // lane's offset in the row
-			const uint64_t pbase  = a.byte_base + rowb;
-			const int32_t  dl	  = rowd + (int32_t)cpre[r][2];  // the lane's entry depth
+			const uint64_t rowb	 = tbase + (uint64_t)r * kRowBytes;	 // data index of the row
+			const uint32_t loff	 = (uint32_t)tid * kBPL;			 // lane's offset in the row
+			const uint64_t pbase = a.byte_base + rowb;
+			int32_t		   dl	 = rowd + (int32_t)cpre[r][2];	// the lane's entry depth
+			if constexpr (BATCH) {
+				if (ldoc[r] >= 0) {
+					const uint32_t sl	  = (uint32_t)(ldoc[r] - d0);
+					const bool	   inside = sl > 0 || (s_hw[0] & 1ull);	// the document's head is in the tile
+					dl += inside ? -s_dhead[sl] : (int32_t)cex[2];
+					if (lhead[r]) {
+						a.b.st_start[ldoc[r]] = rb[0] + cpre[r][0];
+						a.b.br_start[ldoc[r]] = rb[1] + cpre[r][1];
+					}
+					if (llast[r]) {
+						a.b.doc_final[ldoc[r]] = lexit[r];
+						a.b.doc_last[ldoc[r]]  = dl + (int32_t)cnt[r][2];
+					}
+				}
+			}
+			int32_t		   rmin = INT_MAX, rmax = INT_MIN;	// this lane, this row
 			const bool	   staged = rtot[r][0] <= (uint32_t)kCapP && rtot[r][1] <= (uint32_t)kCapB;
 			if (staged) {  // block-uniform
 				{  // both 32-bit halves of the mask at once: half the iterations
@@ -756,11 +937,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					const bool	  op   = (opm[r] >> p) & 1ull;
 					const int32_t bal0 = bal;
 					bal += op ? 1 : -1;
-					lmin		= min(lmin, bal);
-					lmax		= max(lmax, bal);
+					rmin		= min(rmin, bal);
+					rmax		= max(rmax, bal);
 					s_bpos[lb]	= (uint16_t)(loff + (uint32_t)p);
 					s_bsidx[lb] = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
-					s_bkey[lb]	= (int16_t)((op ? bal0 : bal) - rowd);
+					s_bkey[lb]	= (KeyT)((op ? bal0 : bal) - (BATCH ? 0 : rowd));
 					++lb;
 					bm &= bm - 1;
 				}
@@ -772,6 +953,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				// input bytes for chars are gathered four per thread at once, so
 				// their latencies overlap
 				constexpr int U = 4;
+				auto		  out_pos = [&](uint32_t off) -> uint32_t {
+					 if constexpr (BATCH)
+						 return s_lpos0[r * kBlock + (off >> 6)] + (off & 63u);
+					 else
+						 return (uint32_t)(pbase + off);
+				};
 				for (uint32_t i0 = 0; i0 < mp; i0 += U * kBlock) {
 					uint8_t ch[U];
 					if (a.chars)
@@ -784,13 +971,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					for (int u = 0; u < U; ++u) {
 						const uint32_t i = i0 + u * kBlock + tid;
 						if (i < mp) {
-							a.positions[rb[0] + i] = (uint32_t)(pbase + s_pos[i]);
+							a.positions[rb[0] + i] = out_pos(s_pos[i]);
 							if (a.chars)
 								a.chars[rb[0] + i] = ch[u];
 						}
 					}
 				}
-				const int32_t kb = (int32_t)rb[2];
+				const int32_t kb = BATCH ? 0 : (int32_t)rb[2];
 				for (uint32_t i0 = 0; i0 < mb; i0 += U * kBlock) {
 					uint8_t ch[U];
 #pragma unroll
@@ -802,10 +989,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					for (int u = 0; u < U; ++u) {
 						const uint32_t i = i0 + u * kBlock + tid;
 						if (i < mb) {
-							a.bpos[rb[1] + i]  = (uint32_t)(pbase + s_bpos[i]);
+							const uint32_t off = s_bpos[i];
+							a.bpos[rb[1] + i]  = out_pos(off);
 							a.bsidx[rb[1] + i] = rb[0] + s_bsidx[i];
 							a.bkey[rb[1] + i]  = kb + (int32_t)s_bkey[i];
 							a.bchar[rb[1] + i] = ch[u];
+							if constexpr (BATCH)
+								a.b.bdoc[rb[1] + i] = (uint32_t)d0 + slot_of(r * kBlock + (int)(off >> 6));
 						}
 					}
 				}
@@ -816,8 +1006,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				int32_t	 bal = dl;
 				while (m) {
 					const int	   p   = __builtin_ctzll(m);
-					const uint32_t pos = (uint32_t)(pbase + loff + (uint64_t)p);
-					const uint8_t  ch  = a.data[rowb + loff + p];
+					uint32_t	   pos = (uint32_t)(pbase + loff + (uint64_t)p);
+					if constexpr (BATCH)
+						pos = s_lpos0[r * kBlock + tid] + (uint32_t)p;
+					const uint8_t ch = a.data[rowb + loff + p];
 					if ((uint64_t)rb[0] + li < a.cap_struct) {
 						a.positions[rb[0] + li] = pos;
 						if (a.chars)
@@ -826,14 +1018,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					if ((om[r] >> p) & 1ull) {
 						const bool op = (opm[r] >> p) & 1ull;
 						bal += op ? 1 : -1;
-						lmin = min(lmin, bal);
-						lmax = max(lmax, bal);
+						rmin = min(rmin, bal);
+						rmax = max(rmax, bal);
 						if ((uint64_t)rb[1] + lb < a.cap_br) {
 							const uint32_t g = rb[1] + lb;
 							a.bpos[g]		 = pos;
 							a.bsidx[g]		 = rb[0] + li;
-							a.bkey[g]		 = (int32_t)cex[2] + (op ? bal - 1 : bal);
+							a.bkey[g]		 = (BATCH ? 0 : (int32_t)cex[2]) + (op ? bal - 1 : bal);
 							a.bchar[g]		 = ch;
+							if constexpr (BATCH)
+								a.b.bdoc[g] = (uint32_t)ldoc[r];
 						}
 						++lb;
 					}
@@ -841,6 +1035,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					m &= m - 1;
 				}
 			}
+			if constexpr (BATCH)
+				if (rmin != INT_MAX) {	// document-relative already
+					atomicMin(&a.b.doc_min[ldoc[r]], rmin);
+					atomicMax(&a.b.doc_max[ldoc[r]], rmax);
+				}
+			lmin = min(lmin, rmin);
+			lmax = max(lmax, rmax);
 			rowd += (int32_t)rtot[r][2];
 #pragma unroll
 			for (int f = 0; f < 3; ++f) rb[f] += rtot[r][f];
@@ -859,10 +1060,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				mn = min(mn, s_mm[w2][0]);
 				mx = max(mx, s_mm[w2][1]);
 			}
+			const int32_t off = BATCH ? 0 : (int32_t)cex[2];
 			if (mn != INT_MAX)
-				atomicMin(&a.sum->min_bal, (int32_t)cex[2] + mn);
+				atomicMin(&a.sum->min_bal, off + mn);
 			if (mx != INT_MIN)
-				atomicMax(&a.sum->max_bal, (int32_t)cex[2] + mx);
+				atomicMax(&a.sum->max_bal, off + mx);
 		}
 	}
 	// every block has drawn a ticket past the end: the last one out resets

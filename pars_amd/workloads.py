@@ -76,3 +76,49 @@ def sizes(seed: int, count: int, median: float = 4096.0, sigma: float = 1.85, lo
 
 def doc_seed(seed: int, index: int) -> int:
     return N.gen().pars_gen_doc_seed(seed, index)
+
+
+def doc_kinds(seed: int, count: int, mix: dict[str, int]) -> list[str]:
+    """Kind of each document, drawn per document from splitmix(seed ^ index) by the
+    integer weights in `mix` (e.g. {"json": 70, "csv": 30})."""
+    names = list(mix)
+    cuts = np.cumsum([mix[k] for k in names])
+    total = int(cuts[-1])
+    return [names[int(np.searchsorted(cuts, doc_seed(seed, i) % total, side="right"))] for i in range(count)]
+
+
+def make_doc(kind: str, seed: int, n: int) -> np.ndarray:
+    if kind == "json":
+        return json(seed, n, 8)
+    if kind == "csv":
+        return csv(seed, n)
+    if kind == "clike":
+        return clike(seed, n, 24)
+    raise ValueError(kind)
+
+
+def batch(seed: int, count: int, mix: dict[str, int] | None = None, total_bytes: int | None = None,
+          **size_kw) -> dict:
+    """The batched workload: `count` documents with log-normal sizes, kinds by
+    `mix`; grouped by kind (one batch per plan), each group packed for
+    pars_hip_scan_batch.  Returns {kind: (buffer, spans, doc indices)} plus
+    "sizes" and "kinds".  total_bytes, when given, rescales the median so the
+    sizes sum close to it."""
+    from . import pack_batch
+
+    mix = mix or {"json": 70, "csv": 30}
+    sz = sizes(seed, count, **size_kw)
+    if total_bytes is not None:
+        med = size_kw.get("median", 4096.0)
+        for _ in range(3):
+            scale = total_bytes / max(float(sz.sum()), 1.0)
+            med *= scale
+            sz = sizes(seed, count, **{**size_kw, "median": med})
+    kinds = doc_kinds(seed, count, mix)
+    out: dict = {"sizes": sz, "kinds": kinds}
+    for k in mix:
+        idx = [i for i in range(count) if kinds[i] == k]
+        docs = [make_doc(k, doc_seed(seed, i), int(sz[i])) for i in idx]
+        buf, spans = pack_batch(docs)
+        out[k] = (buf, spans, idx)
+    return out

@@ -6,6 +6,7 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
 import pytest
 
 from pars_amd import Plan, _native as N
@@ -79,3 +80,30 @@ def test_cpp_facade_builds_and_links():
 
     out = subprocess.run(["ldd", exe], stdout=subprocess.PIPE, text=True).stdout
     assert "libpars_hip.so" in out and "not found" not in out
+
+
+def test_pack_batch_layout():
+    """Documents start on 64-byte boundaries, in order, with spaces between."""
+    from pars_amd import pack_batch
+
+    docs = [b"x" * n for n in (1, 63, 64, 65, 200)]
+    buf, spans = pack_batch(docs)
+    assert buf.size % 64 == 0
+    prev = 0
+    for d, s in zip(docs, spans):
+        assert s.offset % 64 == 0 and s.offset >= prev and s.size == len(d)
+        assert bytes(buf[s.offset:s.offset + s.size]) == d
+        prev = s.offset + s.size
+    assert set(bytes(buf[1:64])) == {0x20}
+
+
+def test_batch_workload_is_deterministic():
+    from pars_amd import workloads as W
+
+    a = W.batch(4, 50)
+    b = W.batch(4, 50)
+    assert a["kinds"] == b["kinds"]
+    assert np.array_equal(a["sizes"], b["sizes"])
+    for k in ("json", "csv"):
+        assert np.array_equal(a[k][0], b[k][0])
+        assert len(a[k][1]) == len(a[k][2])

@@ -95,6 +95,81 @@ def cpu_baseline(doc, sample_bytes: int) -> dict:
     }
 
 
+def bench_batch(torch, n_docs: int, iters: int, plain_sample: int) -> dict:
+    """The 100k-document batch: 70% JSON / 30% CSV, log-normal sizes (median
+    4 KiB, sigma 1.85, clipped to [200 B, 4 MiB]), one batched launch chain per
+    plan (graph-replayed), against one plain launch chain per document."""
+    import numpy as np
+
+    from pars_amd import Scanner
+    from pars_amd import _native as N
+    from pars_amd import workloads as W
+
+    t0 = time.perf_counter()
+    b = W.batch(4, n_docs, {"json": 70, "csv": 30}, median=4096.0, sigma=1.85)
+    total = int(b["sizes"].sum())
+    log(f"batch: {n_docs} documents, {total} bytes, generated in {time.perf_counter() - t0:.1f} s")
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    groups = []
+    for kind in ("json", "csv"):
+        buf, spans, idx = b[kind]
+        d = torch.from_numpy(buf).to("cuda")
+        sc = Scanner(kind)
+        sc.set_option(N.PARS_OPT_GRAPH, 1)
+        groups.append((kind, d, spans, sc, len(idx)))
+    torch.cuda.synchronize()
+
+    def one_pass():
+        out = []
+        for kind, d, spans, sc, nd in groups:
+            r = sc.scan_batch(d.data_ptr(), d.numel(), spans, sp, want_pairs=False, run_pairs=True)
+            out.append((r.n_struct, r.n_brackets))
+        return out
+
+    first = one_pass()  # eager
+    for _ in range(2):  # capture, then replay
+        assert one_pass() == first
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        one_pass()
+    t_batch = (time.perf_counter() - t0) / iters
+
+    # the same documents through one plain launch chain each (bounded sample, in document order)
+    plain_t, plain_docs, plain_bytes = 0.0, 0, 0
+    for kind, d, spans, sc, nd in groups:
+        k = max(1, plain_sample * nd // n_docs)
+        sc1 = Scanner(kind)
+        base = d.data_ptr()
+        sc1.scan_device(base + spans[0].offset, spans[0].size, sp)  # warm
+        t0 = time.perf_counter()
+        for i in range(min(k, nd)):
+            sc1.scan_device(base + spans[i].offset, spans[i].size, sp, want_pairs=False, run_pairs=True)
+        plain_t += time.perf_counter() - t0
+        plain_docs += min(k, nd)
+        plain_bytes += int(sum(spans[i].size for i in range(min(k, nd))))
+        sc1.close()
+    for g in groups:
+        g[3].close()
+    docs_s = n_docs / t_batch
+    plain_docs_s = plain_docs / plain_t
+    return {
+        "workload": "100k uneven documents, 70% JSON / 30% CSV, log-normal median 4 KiB (seed 4)",
+        "docs": n_docs,
+        "bytes": total,
+        "ms_per_batch": round(t_batch * 1e3, 3),
+        "docs_per_s": round(docs_s, 1),
+        "gb_per_s": round(total / t_batch / 1e9, 3),
+        "plain_docs_per_s": round(plain_docs_s, 1),
+        "plain_gb_per_s": round(plain_bytes / plain_t / 1e9, 4),
+        "plain_sample_docs": plain_docs,
+        "speedup_docs_per_s": round(docs_s / plain_docs_s, 1),
+        "n_struct": int(sum(x[0] for x in first)),
+        "n_brackets": int(sum(x[1] for x in first)),
+        "graph": True,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +180,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of the document (counter calibration under rocprofv3)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-documents measurement")
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched-documents measurement")
+    ap.add_argument("--batch-docs", type=int, default=100_000)
+    ap.add_argument("--batch-iters", type=int, default=5)
+    ap.add_argument("--plain-sample", type=int, default=3000, help="documents timed through plain launches")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -115,6 +195,14 @@ def main() -> int:
     from pars_amd import Plan, Scanner
     from pars_amd import _native as N
     from pars_amd import workloads as W
+
+    if args.only_batch:
+        import torch
+
+        torch.cuda.init()
+        r = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample)
+        print(json.dumps({"batched": r}), flush=True)
+        return 0
 
     n = args.size_mib << 20
     log(f"generating {n} bytes of JSON (seed 2, depth 32, 30% strings)")
@@ -211,6 +299,11 @@ def main() -> int:
     if cpu is not None:
         line["cpu_baseline"] = cpu
     sc.close()
+    del d_doc
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample)
+        log(f"batched: {line['batched']}")
     print(json.dumps(line), flush=True)
     return 0
 

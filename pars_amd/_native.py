@@ -139,6 +139,7 @@ GEN_PROTOTYPES = {
         C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_uint64, C.c_void_p]
     ),
     "pars_gen_doc_seed": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "pars_gen_docs": (C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 

@@ -531,6 +531,10 @@ struct CGen {
 // ---------------------------------------------------------------------------
 template<class F>
 void run_regions(uint64_t nseg, F&& fill_region) {
+	if (nseg == 1) {  // small documents: no threads
+		fill_region(0);
+		return;
+	}
 	unsigned nt = std::thread::hardware_concurrency();
 	nt			= nt ? (nt > 32 ? 32 : nt) : 1;
 	if (nt > nseg)
@@ -683,6 +687,38 @@ int pars_gen_sizes(uint64_t seed, uint64_t count, double median, double sigma, u
 			v = (double)hi;
 		out[i] = (uint64_t)v;
 	}
+	return 0;
+}
+
+// Many documents into one buffer: document i (kind 0 JSON with depth <= 8,
+// 1 CSV, 2 C-like with depth <= 24) of sizes[i] bytes from seeds[i] at
+// out + offsets[i]; documents are generated in parallel, each exactly as the
+// single-document generator makes it.
+int pars_gen_docs(uint64_t count, const uint8_t* kinds, const uint64_t* seeds, const uint64_t* sizes,
+				  const uint64_t* offsets, uint8_t* out) {
+	unsigned nt = std::thread::hardware_concurrency();
+	nt			= nt ? (nt > 32 ? 32 : nt) : 1;
+	std::vector<std::thread> th;
+	std::vector<int>		 rc(nt, 0);
+	for (unsigned t = 0; t < nt; ++t)
+		th.emplace_back([&, t] {
+			for (uint64_t i = t; i < count; i += nt) {
+				uint8_t* o = out + offsets[i];
+				int		 r = 0;
+				switch (kinds[i]) {
+				case 0: r = pars_gen_json(seeds[i], sizes[i], 8, -1, 1, o, nullptr); break;
+				case 1: r = pars_gen_csv(seeds[i], sizes[i], 1, o); break;
+				case 2: r = pars_gen_clike(seeds[i], sizes[i], 24, 1, o); break;
+				default: r = -1;
+				}
+				if (r)
+					rc[t] = r;
+			}
+		});
+	for (auto& x : th) x.join();
+	for (int r : rc)
+		if (r)
+			return r;
 	return 0;
 }
 

@@ -97,28 +97,45 @@ def make_doc(kind: str, seed: int, n: int) -> np.ndarray:
     raise ValueError(kind)
 
 
-def batch(seed: int, count: int, mix: dict[str, int] | None = None, total_bytes: int | None = None,
-          **size_kw) -> dict:
-    """The batched workload: `count` documents with log-normal sizes, kinds by
-    `mix`; grouped by kind (one batch per plan), each group packed for
-    pars_hip_scan_batch.  Returns {kind: (buffer, spans, doc indices)} plus
-    "sizes" and "kinds".  total_bytes, when given, rescales the median so the
-    sizes sum close to it."""
-    from . import pack_batch
+KIND_CODES = {"json": 0, "csv": 1, "clike": 2}
 
+
+def pack_generated(kinds: list[str], seeds: np.ndarray, sz: np.ndarray, align: int = 64):
+    """Generate documents straight into one batch buffer (each on a 64-byte
+    boundary, spaces between), in parallel; returns (buffer, DocSpan array)."""
+    count = len(kinds)
+    sz = np.asarray(sz, dtype=np.uint64)
+    offs = np.zeros(count, dtype=np.uint64)
+    if count > 1:
+        offs[1:] = np.cumsum((sz + (align - 1)) // align * align)[:-1]
+    total = int(offs[-1] + sz[-1]) if count else 0
+    buf = np.full(max((total + align - 1) // align * align, align), 0x20, dtype=np.uint8)
+    codes = np.array([KIND_CODES[k] for k in kinds], dtype=np.uint8)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    rc = N.gen().pars_gen_docs(count, codes.ctypes.data, seeds.ctypes.data, sz.ctypes.data, offs.ctypes.data,
+                               buf.ctypes.data)
+    if rc:
+        raise RuntimeError(f"pars_gen_docs failed ({rc})")
+    spans = (N.DocSpan * count)()
+    for i in range(count):
+        spans[i].offset = int(offs[i])
+        spans[i].size = int(sz[i])
+    return buf, spans
+
+
+def batch(seed: int, count: int, mix: dict[str, int] | None = None, **size_kw) -> dict:
+    """The batched workload: `count` documents with log-normal sizes, kinds by
+    `mix`; grouped by kind (one batch per plan), each group generated into its
+    packed buffer.  Returns {kind: (buffer, spans, doc indices)} plus "sizes"
+    and "kinds"."""
     mix = mix or {"json": 70, "csv": 30}
     sz = sizes(seed, count, **size_kw)
-    if total_bytes is not None:
-        med = size_kw.get("median", 4096.0)
-        for _ in range(3):
-            scale = total_bytes / max(float(sz.sum()), 1.0)
-            med *= scale
-            sz = sizes(seed, count, **{**size_kw, "median": med})
     kinds = doc_kinds(seed, count, mix)
+    seeds = np.array([doc_seed(seed, i) for i in range(count)], dtype=np.uint64)
     out: dict = {"sizes": sz, "kinds": kinds}
+    karr = np.array(kinds)
     for k in mix:
-        idx = [i for i in range(count) if kinds[i] == k]
-        docs = [make_doc(k, doc_seed(seed, i), int(sz[i])) for i in idx]
-        buf, spans = pack_batch(docs)
-        out[k] = (buf, spans, idx)
+        idx = np.nonzero(karr == k)[0]
+        buf, spans = pack_generated([k] * idx.size, seeds[idx], sz[idx])
+        out[k] = (buf, spans, idx.tolist())
     return out

@@ -44,34 +44,28 @@ __device__ __forceinline__ void gather_classes(const uint32_t (&c)[8], uint64_t&
 
 // Class lookups of a lane's 64 bytes through the position-shifted two-dword
 // table: group g ORs the entries of bytes 8g..8g+7 (a balanced OR tree, so the
-// compiler forms three-input ORs).
-__device__ __forceinline__ void lookup_groups(const uint2* cls, const uint32_t (&x)[16], uint32_t valid,
-											  uint32_t (&clo)[8], uint32_t (&chi)[8]) {
-	if (valid == kBPL) {
+// compiler forms three-input ORs).  Every lane looks up all 64 bytes (no
+// divergent tail path); callers clear the bits at and past `valid` in the
+// gathered masks.
+__device__ __forceinline__ void lookup_groups(const uint2* cls, const uint32_t (&x)[16], uint32_t (&clo)[8],
+											  uint32_t (&chi)[8]) {
 #pragma unroll
-		for (int g = 0; g < 8; ++g) {
-			uint2 e[8];
+	for (int g = 0; g < 8; ++g) {
+		uint2 e[8];
 #pragma unroll
-			for (int b = 0; b < 8; ++b) {
-				const int	   j = 8 * g + b;
-				const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-				e[b]			 = cls[b * 256 + c];
-			}
-			clo[g] = ((e[0].x | e[1].x) | (e[2].x | e[3].x)) | ((e[4].x | e[5].x) | (e[6].x | e[7].x));
-			chi[g] = ((e[0].y | e[1].y) | (e[2].y | e[3].y)) | ((e[4].y | e[5].y) | (e[6].y | e[7].y));
+		for (int b = 0; b < 8; ++b) {
+			const int	   j = 8 * g + b;
+			const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
+			e[b]			 = cls[b * 256 + c];
 		}
-	} else {
-#pragma unroll
-		for (int g = 0; g < 8; ++g) clo[g] = chi[g] = 0;
-#pragma unroll
-		for (int j = 0; j < kBPL; ++j)
-			if ((uint32_t)j < valid) {
-				const uint32_t c = (x[j >> 2] >> ((j & 3) * 8)) & 0xFFu;
-				const uint2	   e = cls[(j & 7) * 256 + c];
-				clo[j >> 3] |= e.x;
-				chi[j >> 3] |= e.y;
-			}
+		clo[g] = ((e[0].x | e[1].x) | (e[2].x | e[3].x)) | ((e[4].x | e[5].x) | (e[6].x | e[7].x));
+		chi[g] = ((e[0].y | e[1].y) | (e[2].y | e[3].y)) | ((e[4].y | e[5].y) | (e[6].y | e[7].y));
 	}
+}
+
+// Bits below `valid` (valid <= 64).
+__device__ __forceinline__ uint64_t valid_mask(uint32_t valid) {
+	return valid >= 64 ? ~0ull : ((1ull << valid) - 1ull);
 }
 
 // Byte 0 of eight accumulated groups -> a 64-bit mask (byte g = group g).
@@ -279,10 +273,16 @@ struct EscapeEval {
 
 	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
 		uint32_t clo[8], chi[8];
-		lookup_groups(L.cls, x, valid, clo, chi);
+		lookup_groups(L.cls, x, clo, chi);
 		uint64_t Q;
 		gather_classes(clo, S, OC, OP, Q);
-		const uint64_t BS = gather_byte0(chi);
+		uint64_t	   BS = gather_byte0(chi);
+		const uint64_t vm = valid_mask(valid);	// bytes past the lane's end have no class
+		S &= vm;
+		OC &= vm;
+		OP &= vm;
+		Q &= vm;
+		BS &= vm;
 
 		const uint32_t f0 = plan->fk_state[0], f1 = plan->fk_state[1], f2 = plan->fk_state[2],
 					   f3 = plan->fk_state[3];
@@ -358,8 +358,13 @@ struct ParityEval {
 
 	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
 		uint32_t clo[8], chi[8];
-		lookup_groups(L.cls, x, valid, clo, chi);
+		lookup_groups(L.cls, x, clo, chi);
 		gather_classes(clo, S, OC, OP, Q);
+		const uint64_t vm = valid_mask(valid);	// bytes past the lane's end have no class
+		S &= vm;
+		OC &= vm;
+		OP &= vm;
+		Q &= vm;
 		const uint32_t f_out = plan->fk_state[0];
 		f_in				 = plan->fk_state[1];
 		f_cl				 = plan->fk_state[2];

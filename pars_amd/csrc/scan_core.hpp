@@ -1035,11 +1035,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					m &= m - 1;
 				}
 			}
-			if constexpr (BATCH)
-				if (rmin != INT_MAX) {	// document-relative already
-					atomicMin(&a.b.doc_min[ldoc[r]], rmin);
-					atomicMax(&a.b.doc_max[ldoc[r]], rmax);
+			if constexpr (BATCH) {
+				// per-document extremes: a segmented wave scan over the lanes of
+				// one document (a contiguous run), then one atomic per run
+				int32_t		   dmin = rmin, dmax = rmax;
+				const uint32_t doc	= (uint32_t)ldoc[r];
+				auto		   seg	= [&](uint32_t od, int32_t omin, int32_t omax) {
+					   if (od == doc) {
+						   dmin = min(dmin, omin);
+						   dmax = max(dmax, omax);
+					   }
+				};
+#define PARS_SEG_STEP(CTRL, RM)                                                                          \
+	seg(dpp<CTRL, RM>(0xFFFFFFFFu, doc), (int32_t)dpp<CTRL, RM>((uint32_t)INT_MAX, (uint32_t)dmin),      \
+		(int32_t)dpp<CTRL, RM>((uint32_t)INT_MIN, (uint32_t)dmax))
+				PARS_SEG_STEP(kDppRowShr1, 0xF);
+				PARS_SEG_STEP(kDppRowShr2, 0xF);
+				PARS_SEG_STEP(kDppRowShr4, 0xF);
+				PARS_SEG_STEP(kDppRowShr8, 0xF);
+				PARS_SEG_STEP(kDppBcast15, 0xA);
+				PARS_SEG_STEP(kDppBcast31, 0xC);
+#undef PARS_SEG_STEP
+				const uint64_t hw	= s_hw[r * 4 + wid];  // heads of this wave's lanes
+				const bool	   tail = lane == 63 || ((hw >> (lane + 1)) & 1ull);
+				if (tail && ldoc[r] >= 0 && dmin != INT_MAX) {	// document-relative already
+					atomicMin(&a.b.doc_min[ldoc[r]], dmin);
+					atomicMax(&a.b.doc_max[ldoc[r]], dmax);
 				}
+			}
 			lmin = min(lmin, rmin);
 			lmax = max(lmax, rmax);
 			rowd += (int32_t)rtot[r][2];

@@ -189,6 +189,80 @@ int pars_hip_scan_batch(
 	int want_pairs, int run_pairs, pars_stage_times* st, pars_batch_result* out
 );
 
+/* ---- serving (no reference counterpart) ------------------------------------
+ * A server accepts parse requests as they arrive, stages their bytes in a
+ * pinned host ring, copies them asynchronously into a device ring buffer, and
+ * runs them through the batched scan with continuous batching: whatever is
+ * pending when a scanner slot frees up forms the next launch, so requests that
+ * arrive while others are on the GPU join the next batch while the previous
+ * one still runs.  Admission is gated by an HBM budget (inputs in the ring
+ * plus worst-case outputs of every request not yet released).  Results stay on
+ * the device until released.  Each result equals scanning the request alone. */
+typedef struct pars_server pars_server;
+
+typedef struct {
+	const pars_plan* plans;			 /* n_plans plans; requests name one by index */
+	uint32_t		 n_plans;
+	uint32_t		 slots_per_plan;  /* scanners per plan (batches in flight), default 2 */
+	uint64_t		 ring_bytes;	  /* device input ring, all plans (default 4 GiB) */
+	uint64_t		 staging_bytes;	  /* pinned host staging, all plans (default 1 GiB) */
+	uint64_t		 hbm_budget;	  /* bytes the server may hold on the device (default 64 GiB) */
+	uint64_t		 max_batch_bytes; /* per launch (default 64 MiB; also the largest request) */
+	uint32_t		 max_batch_docs;  /* per launch (default 16384) */
+	uint32_t		 run_pairs;		  /* run R6 (default 1) */
+} pars_server_config;
+
+typedef struct {
+	uint64_t		id;
+	int32_t			status;	   /* PARS_OK, or the error that failed the request's batch */
+	uint32_t		plan;
+	uint64_t		n_bytes;
+	uint64_t		n_struct;
+	uint64_t		n_brackets;
+	const uint32_t* structural;	   /* device; document-local byte offsets */
+	const uint32_t* bracket_pos;   /* device */
+	const uint8_t*	bracket_char;  /* device */
+	const int32_t*	bracket_pair;  /* device, document-local indices (NULL without R6) */
+	uint32_t		flags;		   /* bit0 balanced, bit1 well_formed, bit2 unclosed_mode */
+	int32_t			max_depth;
+	uint64_t		t_submit_ns;   /* host steady clock */
+	uint64_t		t_launch_ns;   /* its batch was enqueued */
+	uint64_t		t_done_ns;	   /* its batch's outputs were complete on the device */
+	uint32_t		batch_docs;	   /* documents in its batch */
+	uint32_t		reserved;
+} pars_request_result;
+
+typedef struct {
+	uint64_t submitted, completed, released;
+	uint64_t batches;
+	uint64_t bytes_completed;
+	uint64_t busy_rejects;		 /* submits refused for lack of staging space */
+	uint64_t budget_waits;		 /* batch formations cut short by the HBM budget */
+	uint64_t device_bytes;		 /* held on the device (rings + scanner arenas) */
+	uint64_t inflight_reserved;	 /* budget held by admitted, unreleased requests */
+} pars_server_stats;
+
+int pars_hip_server_create(const pars_server_config* cfg, pars_server** out);
+int pars_hip_server_destroy(pars_server* sv);
+/* Copies n bytes into the pinned staging ring; PARS_ERR_BUSY when it is full
+ * (retry after polling and releasing). */
+int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, size_t n, uint64_t* id);
+/* Up to cap completed requests not returned before; waits up to wait_us for
+ * the first one. */
+int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, size_t* n_out, int64_t wait_us);
+/* The request's outputs are no longer needed (frees its budget; a scanner
+ * slot is reused once all of its batch's requests are released). */
+int pars_hip_server_release(pars_server* sv, uint64_t id);
+int pars_hip_server_stats(pars_server* sv, pars_server_stats* out);
+/* Load driver: submits n requests (host bytes data[i], sizes[i], plan[i]) at
+ * arrival_ns[i] after the start (all 0: as fast as admission allows), polls
+ * and releases as they complete; latency_ns[i] = completion - scheduled
+ * arrival.  *elapsed_ns = first arrival to last completion. */
+int pars_hip_server_replay(
+	pars_server* sv, size_t n, const uint32_t* plan, const uint8_t* const* data, const uint64_t* sizes,
+	const uint64_t* arrival_ns, uint64_t* latency_ns, uint64_t* elapsed_ns
+);
+
 /* pipeline.cuh:1225 scanner::struct_ptr. */
 const uint32_t* pars_hip_struct_ptr(const pars_scanner* sc);
 

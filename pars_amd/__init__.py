@@ -268,6 +268,80 @@ class Scanner:
         return self._lib.pars_hip_struct_ptr(self._h) or 0
 
 
+class Server:
+    """The serving runtime (pars_hip_server_*): submit requests as they arrive,
+    poll completed ones, release them when their outputs are no longer needed."""
+
+    def __init__(self, plans, slots_per_plan: int = 2, ring_bytes: int = 4 << 30, staging_bytes: int = 1 << 30,
+                 hbm_budget: int = 64 << 30, max_batch_bytes: int = 64 << 20, max_batch_docs: int = 16384,
+                 run_pairs: bool = True):
+        import numpy as np
+
+        self.plans = [Plan.load(p) if isinstance(p, str) else p for p in plans]
+        blob = b"".join(p.blob for p in self.plans)
+        self._plan_buf = np.frombuffer(blob, dtype=np.uint8).copy()
+        cfg = N.ServerConfig(self._plan_buf.ctypes.data, len(self.plans), slots_per_plan, ring_bytes,
+                             staging_bytes, hbm_budget, max_batch_bytes, max_batch_docs, int(run_pairs))
+        self._lib = N.lib()
+        h = C.c_void_p()
+        _check(self._lib.pars_hip_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pars_hip_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, plan: int, data) -> int | None:
+        """Request id, or None when the staging ring is full (poll, release, retry)."""
+        import numpy as np
+
+        b = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8) if not isinstance(data, np.ndarray)
+                                 else data)
+        rid = C.c_uint64()
+        rc = self._lib.pars_hip_server_submit(self._h, plan, b.ctypes.data, b.size, C.byref(rid))
+        if rc == 6:  # PARS_ERR_BUSY
+            return None
+        _check(rc)
+        return rid.value
+
+    def poll(self, cap: int = 256, wait_us: int = 0) -> list[N.RequestResult]:
+        out = (N.RequestResult * cap)()
+        k = C.c_size_t()
+        _check(self._lib.pars_hip_server_poll(self._h, out, cap, C.byref(k), wait_us))
+        return [out[i] for i in range(k.value)]
+
+    def release(self, rid: int) -> None:
+        _check(self._lib.pars_hip_server_release(self._h, rid))
+
+    def stats(self) -> N.ServerStats:
+        s = N.ServerStats()
+        _check(self._lib.pars_hip_server_stats(self._h, C.byref(s)))
+        return s
+
+    def replay(self, plan_idx, docs, arrival_ns):
+        """Submit docs[i] (numpy uint8) at arrival_ns[i] after the start; returns
+        (latency_ns per request, elapsed_ns)."""
+        import numpy as np
+
+        n = len(docs)
+        plans = np.ascontiguousarray(plan_idx, dtype=np.uint32)
+        ptrs = np.array([d.ctypes.data for d in docs], dtype=np.uint64)
+        sizes = np.array([d.size for d in docs], dtype=np.uint64)
+        arr = np.ascontiguousarray(arrival_ns, dtype=np.uint64)
+        lat = np.zeros(n, dtype=np.uint64)
+        el = C.c_uint64()
+        _check(self._lib.pars_hip_server_replay(self._h, n, plans.ctypes.data, ptrs.ctypes.data, sizes.ctypes.data,
+                                                arr.ctypes.data, lat.ctypes.data, C.byref(el)))
+        return lat, el.value
+
+
 _tls = threading.local()
 
 

@@ -96,6 +96,56 @@ class BatchResult(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("plans", C.c_void_p),
+        ("n_plans", C.c_uint32),
+        ("slots_per_plan", C.c_uint32),
+        ("ring_bytes", C.c_uint64),
+        ("staging_bytes", C.c_uint64),
+        ("hbm_budget", C.c_uint64),
+        ("max_batch_bytes", C.c_uint64),
+        ("max_batch_docs", C.c_uint32),
+        ("run_pairs", C.c_uint32),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64),
+        ("status", C.c_int32),
+        ("plan", C.c_uint32),
+        ("n_bytes", C.c_uint64),
+        ("n_struct", C.c_uint64),
+        ("n_brackets", C.c_uint64),
+        ("structural", C.c_void_p),
+        ("bracket_pos", C.c_void_p),
+        ("bracket_char", C.c_void_p),
+        ("bracket_pair", C.c_void_p),
+        ("flags", C.c_uint32),
+        ("max_depth", C.c_int32),
+        ("t_submit_ns", C.c_uint64),
+        ("t_launch_ns", C.c_uint64),
+        ("t_done_ns", C.c_uint64),
+        ("batch_docs", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("released", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("bytes_completed", C.c_uint64),
+        ("busy_rejects", C.c_uint64),
+        ("budget_waits", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+        ("inflight_reserved", C.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/pars_hip.h declares
 PROTOTYPES = {
     "pars_hip_scanner_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -122,6 +172,19 @@ PROTOTYPES = {
         C.c_int,
         [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(DocSpan), C.c_size_t, C.c_void_p, C.c_int, C.c_int,
          C.POINTER(StageTimes), C.POINTER(BatchResult)],
+    ),
+    "pars_hip_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "pars_hip_server_destroy": (C.c_int, [C.c_void_p]),
+    "pars_hip_server_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint64)]),
+    "pars_hip_server_poll": (
+        C.c_int, [C.c_void_p, C.POINTER(RequestResult), C.c_size_t, C.POINTER(C.c_size_t), C.c_int64]
+    ),
+    "pars_hip_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "pars_hip_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "pars_hip_server_replay": (
+        C.c_int,
+        [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+         C.POINTER(C.c_uint64)],
     ),
     "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
     "pars_hip_last_error": (C.c_char_p, []),

@@ -9,6 +9,7 @@
 // bracket capacity; if the device reports more brackets than fit, the scan is
 // rerun once with the exact capacity.
 #include "../../include/pars_hip.h"
+#include "internal.hpp"
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
@@ -444,6 +445,46 @@ int run_batch_graph(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t 
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Internal entry points for the serving runtime (internal.hpp).
+// ---------------------------------------------------------------------------
+namespace pars_hip {
+
+int batch_reserve(pars_scanner* sc, uint64_t max_bytes, uint64_t max_docs) {
+	const uint64_t n = std::max<uint64_t>(max_bytes, 64);
+	sc->br_hint		 = n;
+	return ensure_batch(sc, n, std::max<uint64_t>(max_docs, 1), n, false);
+}
+
+int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint64_t* offsets,
+				  const uint32_t* lengths, uint64_t n_docs, hipStream_t s, bool run_pairs, BatchPtrs* out) {
+	if (!n_docs || n > sc->cap_struct || n > sc->cap_br || n_docs * 8 > sc->h_off_cap)
+		return fail(PARS_ERR_ARGUMENT, "batch exceeds the reserved capacity");
+	std::memcpy(sc->h_doc_off, offsets, n_docs * 8);
+	std::memcpy(sc->h_doc_len, lengths, n_docs * 4);
+	int r;
+	if ((r = enqueue_batch(sc, d_in, n, n_docs, s, false, run_pairs, false)))
+		return r;
+	out->structural	   = sc->positions.as<uint32_t>();
+	out->st_start	   = sc->st_start.as<uint32_t>();
+	out->bracket_pos   = sc->bpos.as<uint32_t>();
+	out->bracket_char  = sc->bchar.as<uint8_t>();
+	out->br_start	   = sc->br_start.as<uint32_t>();
+	out->bracket_pair  = run_pairs ? sc->pair.as<int32_t>() : nullptr;
+	out->doc_flags	   = sc->doc_flags.as<uint8_t>();
+	out->max_depth	   = sc->doc_max.as<int32_t>();
+	out->summary	   = sc->summary.p;
+	out->h_summary	   = sc->h_sum;
+	out->summary_bytes = sizeof(DevSummary);
+	return PARS_OK;
+}
+
+bool batch_failed(const void* h_summary) {
+	return (static_cast<const DevSummary*>(h_summary)->overflow & 0x80000005u) != 0;
+}
+
+}  // namespace pars_hip
 
 extern "C" {
 

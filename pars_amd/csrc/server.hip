@@ -1,0 +1,606 @@
+// server.hip — the serving runtime: parse requests arrive one by one, are
+// staged in pinned host memory, copied asynchronously into a device ring, and
+// scanned by the batched kernel with continuous batching.
+//
+// Per plan: a pinned staging ring, a region of the device ring, a copy stream,
+// a compute stream and `slots` scanners.  A worker thread loops:
+//   - retires batches whose completion event fired (their per-document counts
+//     and flags were copied to pinned memory at the end of the batch), marks
+//     their requests done and frees their ring and staging space;
+//   - frees a slot once every request of its batch is released;
+//   - for every plan with pending requests and a free slot, takes the longest
+//     run of pending requests that is contiguous in staging and fits the batch
+//     limits, the device ring and the HBM budget, and enqueues H2D copy ->
+//     batched scan -> D2H of the per-document results -> completion event.
+// Batches of one plan run in order on its compute stream (so ring space frees
+// in allocation order); plans run on their own streams, and a batch's copy
+// overlaps the previous batch's kernels.  No counterpart in the reference.
+#include "../../include/pars_hip.h"
+#include "internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace pars_hip;
+
+namespace {
+
+uint64_t now_ns() {
+	return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+			   std::chrono::steady_clock::now().time_since_epoch())
+		.count();
+}
+
+constexpr uint64_t kAlign = 64;
+
+uint64_t align_up(uint64_t x) {
+	return (x + kAlign - 1) / kAlign * kAlign;
+}
+
+// A byte ring with FIFO release: allocations are contiguous and released in
+// the order they were made; live data spans [tail, head), possibly wrapped.
+struct Ring {
+	uint64_t			 cap = 0, head = 0, tail = 0;
+	std::deque<uint64_t> live;	// start offsets in allocation order
+
+	bool alloc(uint64_t bytes, uint64_t& off) {
+		if (bytes == 0 || bytes > cap)
+			return false;
+		if (live.empty())
+			head = tail = 0;
+		uint64_t at;
+		if (live.empty() || head > tail) {	// free: [head, cap) and [0, tail)
+			if (head + bytes <= cap)
+				at = head;
+			else if (bytes <= tail)
+				at = 0;	 // wrap; [head, cap) idles until the tail passes it
+			else
+				return false;
+		} else if (head < tail && head + bytes <= tail) {  // free: [head, tail)
+			at = head;
+		} else {
+			return false;
+		}
+		off	 = at;
+		head = at + bytes;
+		live.push_back(at);
+		return true;
+	}
+
+	void release_oldest() {
+		if (live.empty())
+			return;
+		live.pop_front();
+		if (live.empty())
+			head = tail = 0;
+		else
+			tail = live.front();
+	}
+};
+
+enum ReqState : int { kPending = 0, kInBatch = 1, kDone = 2 };
+
+struct Request {
+	uint64_t id;
+	uint32_t plan;
+	uint64_t n, stage_off, reserve;
+	uint64_t t_submit, t_launch = 0, t_done = 0;
+	int		 state	= kPending;
+	int		 slot	= -1;
+	uint32_t doc	= 0;
+	bool	 polled = false;
+	pars_request_result res {};
+};
+
+enum SlotState : int { kFree = 0, kRunning = 1, kHeld = 2 };
+
+struct Slot {
+	pars_scanner*		  sc = nullptr;
+	hipEvent_t			  h2d = nullptr, done = nullptr;
+	int					  state = kFree;
+	std::vector<uint64_t> reqs;
+	uint32_t			  outstanding = 0;
+	uint64_t			  launch_ns	  = 0;
+	BatchPtrs			  ptrs {};
+	// pinned per-document results of the batch
+	uint32_t* h_st	  = nullptr;
+	uint32_t* h_br	  = nullptr;
+	uint8_t*  h_flags = nullptr;
+	int32_t*  h_maxd  = nullptr;
+	std::vector<uint64_t> offs;
+	std::vector<uint32_t> lens;
+};
+
+struct PlanState {
+	uint8_t*		  stage = nullptr;	// pinned
+	Ring			  stage_ring;
+	uint8_t*		  ring = nullptr;  // device
+	Ring			  dev_ring;
+	hipStream_t		  copy = nullptr, compute = nullptr;
+	std::deque<uint64_t> pending;  // staged request ids in staging order
+	std::vector<Slot>	 slots;
+	std::deque<int>		 running;  // slots in launch order
+};
+
+}  // namespace
+
+struct pars_server {
+	pars_server_config			  cfg {};
+	std::vector<PlanState>		  plans;
+	std::mutex					  mu;
+	std::condition_variable		  cv_done;
+	std::unordered_map<uint64_t, Request> reqs;
+	std::deque<uint64_t>		  completed;
+	uint64_t					  next_id  = 1;
+	uint64_t					  reserved = 0;	 // budget held by admitted, unreleased requests
+	uint64_t					  fixed	   = 0;	 // device bytes held for rings and arenas
+	pars_server_stats			  st {};
+	std::thread					  worker;
+	std::atomic<bool>			  stop {false};
+	std::string					  error;
+	int							  dev = 0;
+
+	~pars_server();
+	void run();
+	bool retire();
+	bool launch(uint32_t p);
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int sfail(int code, const char* msg) {
+	g_err = msg;
+	return code;
+}
+
+int shipfail(hipError_t e, const char* what) {
+	g_err = std::string(what) + ": " + hipGetErrorString(e);
+	return PARS_ERR_DEVICE;
+}
+
+}  // namespace
+
+pars_server::~pars_server() {
+	stop = true;
+	if (worker.joinable())
+		worker.join();
+	for (PlanState& p : plans) {
+		if (p.compute)
+			(void)hipStreamSynchronize(p.compute);
+		for (Slot& s : p.slots) {
+			if (s.sc)
+				pars_hip_scanner_destroy(s.sc);
+			if (s.h2d)
+				(void)hipEventDestroy(s.h2d);
+			if (s.done)
+				(void)hipEventDestroy(s.done);
+			for (void* h : {(void*)s.h_st, (void*)s.h_br, (void*)s.h_flags, (void*)s.h_maxd})
+				if (h)
+					(void)hipHostFree(h);
+		}
+		if (p.stage)
+			(void)hipHostFree(p.stage);
+		if (p.ring)
+			(void)hipFree(p.ring);
+		if (p.copy)
+			(void)hipStreamDestroy(p.copy);
+		if (p.compute)
+			(void)hipStreamDestroy(p.compute);
+	}
+}
+
+// Batches whose completion event fired: fill their requests' results.
+bool pars_server::retire() {
+	bool any = false;
+	for (uint32_t p = 0; p < plans.size(); ++p) {
+		PlanState& ps = plans[p];
+		while (!ps.running.empty()) {
+			Slot&	   s = ps.slots[ps.running.front()];
+			hipError_t e = hipEventQuery(s.done);
+			if (e == hipErrorNotReady)
+				break;
+			const uint64_t t	  = now_ns();
+			const bool	   failed = e != hipSuccess || batch_failed(s.ptrs.h_summary);
+			std::lock_guard<std::mutex> lk(mu);
+			for (size_t d = 0; d < s.reqs.size(); ++d) {
+				Request&			 q = reqs[s.reqs[d]];
+				pars_request_result& r = q.res;
+				r.id				   = q.id;
+				r.status			   = failed ? PARS_ERR_DEVICE : PARS_OK;
+				r.plan				   = p;
+				r.n_bytes			   = q.n;
+				r.n_struct			   = s.h_st[d + 1] - s.h_st[d];
+				r.n_brackets		   = s.h_br[d + 1] - s.h_br[d];
+				r.structural		   = s.ptrs.structural + s.h_st[d];
+				r.bracket_pos		   = s.ptrs.bracket_pos + s.h_br[d];
+				r.bracket_char		   = s.ptrs.bracket_char + s.h_br[d];
+				r.bracket_pair		   = s.ptrs.bracket_pair ? s.ptrs.bracket_pair + s.h_br[d] : nullptr;
+				r.flags				   = s.h_flags[d];
+				r.max_depth			   = s.h_maxd[d];
+				r.t_submit_ns		   = q.t_submit;
+				r.t_launch_ns		   = q.t_launch;
+				r.t_done_ns			   = t;
+				r.batch_docs		   = (uint32_t)s.reqs.size();
+				q.t_done			   = t;
+				q.state				   = kDone;
+				completed.push_back(q.id);
+				st.completed++;
+				st.bytes_completed += q.n;
+			}
+			ps.dev_ring.release_oldest();
+			for (size_t d = 0; d < s.reqs.size(); ++d) ps.stage_ring.release_oldest();
+			s.state = kHeld;
+			ps.running.pop_front();
+			any = true;
+		}
+	}
+	if (any)
+		cv_done.notify_all();
+	return any;
+}
+
+// Form and enqueue one batch for plan p if it has pending work and a free slot.
+bool pars_server::launch(uint32_t p) {
+	PlanState& ps	= plans[p];
+	int		   slot = -1;
+	std::vector<uint64_t> take;
+	uint64_t			  bytes = 0, first_off = 0;
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		for (size_t i = 0; i < ps.slots.size(); ++i)
+			if (ps.slots[i].state == kFree || (ps.slots[i].state == kHeld && ps.slots[i].outstanding == 0)) {
+				slot = (int)i;
+				break;
+			}
+		if (slot < 0 || ps.pending.empty())
+			return false;
+		const uint64_t avail = cfg.hbm_budget > fixed ? cfg.hbm_budget - fixed : 0;
+		uint64_t	   next	 = 0;
+		for (uint64_t id : ps.pending) {
+			const Request& q = reqs[id];
+			if (take.empty())
+				first_off = next = q.stage_off;
+			else if (q.stage_off != next)  // staging wrapped: the batch ends here
+				break;
+			const uint64_t span = align_up(q.n);
+			if (take.size() >= cfg.max_batch_docs || bytes + span > cfg.max_batch_bytes)
+				break;
+			if (reserved + q.reserve > avail) {
+				st.budget_waits++;
+				break;
+			}
+			reserved += q.reserve;
+			take.push_back(id);
+			bytes += span;
+			next = q.stage_off + span;
+		}
+		if (take.empty())
+			return false;
+	}
+	uint64_t ring_off = 0;
+	if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: give the budget back and retry later
+		std::lock_guard<std::mutex> lk(mu);
+		for (uint64_t id : take) reserved -= reqs[id].reserve;
+		return false;
+	}
+	Slot& s = ps.slots[slot];
+	s.reqs	= take;
+	s.offs.resize(take.size());
+	s.lens.resize(take.size());
+	const uint64_t t = now_ns();
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		for (size_t d = 0; d < take.size(); ++d) {
+			Request& q = reqs[take[d]];
+			s.offs[d]  = q.stage_off - first_off;
+			s.lens[d]  = (uint32_t)q.n;
+			q.state	   = kInBatch;
+			q.slot	   = slot;
+			q.doc	   = (uint32_t)d;
+			q.t_launch = t;
+			ps.pending.pop_front();
+		}
+		s.outstanding = (uint32_t)take.size();
+		st.batches++;
+	}
+	uint8_t*   d_in = ps.ring + ring_off;
+	hipError_t e	= hipMemcpyAsync(d_in, ps.stage + first_off, bytes, hipMemcpyHostToDevice, ps.copy);
+	if (e == hipSuccess)
+		e = hipEventRecord(s.h2d, ps.copy);
+	if (e == hipSuccess)
+		e = hipStreamWaitEvent(ps.compute, s.h2d, 0);
+	int r = e == hipSuccess ? PARS_OK : PARS_ERR_DEVICE;
+	if (r == PARS_OK)
+		r = batch_enqueue(s.sc, d_in, bytes, s.offs.data(), s.lens.data(), take.size(), ps.compute,
+						  cfg.run_pairs != 0, &s.ptrs);
+	const size_t nd = take.size();
+	if (r == PARS_OK) {
+		e = hipMemcpyAsync(s.h_st, s.ptrs.st_start, (nd + 1) * 4, hipMemcpyDeviceToHost, ps.compute);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(s.h_br, s.ptrs.br_start, (nd + 1) * 4, hipMemcpyDeviceToHost, ps.compute);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(s.h_flags, s.ptrs.doc_flags, nd, hipMemcpyDeviceToHost, ps.compute);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(s.h_maxd, s.ptrs.max_depth, nd * 4, hipMemcpyDeviceToHost, ps.compute);
+		if (e == hipSuccess)
+			e = hipMemcpyAsync(s.ptrs.h_summary, s.ptrs.summary, s.ptrs.summary_bytes, hipMemcpyDeviceToHost,
+							   ps.compute);
+		if (e != hipSuccess)
+			r = PARS_ERR_DEVICE;
+	}
+	if (r == PARS_OK && (e = hipEventRecord(s.done, ps.compute)) != hipSuccess)
+		r = PARS_ERR_DEVICE;
+	if (r != PARS_OK) {	 // the device is in trouble: fail the batch's requests now
+		std::lock_guard<std::mutex> lk(mu);
+		error = std::string("batch launch failed: ") + g_err;
+		for (size_t d = 0; d < nd; ++d) {
+			Request& q	  = reqs[take[d]];
+			q.res.id	  = q.id;
+			q.res.status  = r;
+			q.res.plan	  = p;
+			q.state		  = kDone;
+			q.t_done	  = now_ns();
+			completed.push_back(q.id);
+		}
+		ps.dev_ring.release_oldest();
+		for (size_t d = 0; d < nd; ++d) ps.stage_ring.release_oldest();
+		s.state = kHeld;
+		cv_done.notify_all();
+		return false;
+	}
+	s.state		= kRunning;
+	s.launch_ns = t;
+	ps.running.push_back(slot);
+	return true;
+}
+
+void pars_server::run() {
+	(void)hipSetDevice(dev);
+	while (!stop) {
+		bool busy = retire();
+		for (uint32_t p = 0; p < plans.size(); ++p)
+			while (launch(p)) busy = true;
+		bool running = false;
+		for (const PlanState& ps : plans) running = running || !ps.running.empty();
+		if (!busy)
+			std::this_thread::sleep_for(std::chrono::microseconds(running ? 5 : 50));
+	}
+}
+
+extern "C" {
+
+int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) {
+	if (!cfg_in || !out || !cfg_in->plans || !cfg_in->n_plans)
+		return sfail(PARS_ERR_ARGUMENT, "server needs at least one plan");
+	*out			   = nullptr;
+	pars_server_config cfg = *cfg_in;
+	if (!cfg.slots_per_plan)
+		cfg.slots_per_plan = 2;
+	if (!cfg.ring_bytes)
+		cfg.ring_bytes = 4ull << 30;
+	if (!cfg.staging_bytes)
+		cfg.staging_bytes = 1ull << 30;
+	if (!cfg.hbm_budget)
+		cfg.hbm_budget = 64ull << 30;
+	if (!cfg.max_batch_bytes)
+		cfg.max_batch_bytes = 64ull << 20;
+	if (!cfg.max_batch_docs)
+		cfg.max_batch_docs = 16384;
+	cfg.max_batch_bytes = align_up(cfg.max_batch_bytes);
+	const uint64_t ring_each  = cfg.ring_bytes / cfg.n_plans / kAlign * kAlign;
+	const uint64_t stage_each = cfg.staging_bytes / cfg.n_plans / kAlign * kAlign;
+	if (ring_each < cfg.max_batch_bytes || stage_each < cfg.max_batch_bytes)
+		return sfail(PARS_ERR_ARGUMENT, "ring and staging must hold one largest batch per plan");
+	pars_server* sv = new (std::nothrow) pars_server();
+	if (!sv)
+		return sfail(PARS_ERR_MEMORY, "host allocation failed");
+	sv->cfg	 = cfg;
+	(void)hipGetDevice(&sv->dev);
+	sv->plans.resize(cfg.n_plans);
+	int r = PARS_OK;
+	for (uint32_t p = 0; p < cfg.n_plans && r == PARS_OK; ++p) {
+		PlanState& ps = sv->plans[p];
+		hipError_t e  = hipHostMalloc((void**)&ps.stage, stage_each, hipHostMallocDefault);
+		if (e == hipSuccess)
+			e = hipMalloc((void**)&ps.ring, ring_each);
+		if (e == hipSuccess)
+			e = hipStreamCreateWithFlags(&ps.copy, hipStreamNonBlocking);
+		if (e == hipSuccess)
+			e = hipStreamCreateWithFlags(&ps.compute, hipStreamNonBlocking);
+		if (e != hipSuccess) {
+			r = shipfail(e, "server rings");
+			break;
+		}
+		ps.stage_ring.cap = stage_each;
+		ps.dev_ring.cap	  = ring_each;
+		sv->fixed += ring_each;
+		ps.slots.resize(cfg.slots_per_plan);
+		for (Slot& s : ps.slots) {
+			if ((r = pars_hip_scanner_create(&cfg.plans[p], &s.sc)))
+				break;
+			if ((r = pars_hip_set_option(s.sc, PARS_OPT_WANT_BITMAP, 0)))
+				break;
+			if ((r = batch_reserve(s.sc, cfg.max_batch_bytes, cfg.max_batch_docs)))
+				break;
+			const uint64_t nd = cfg.max_batch_docs;
+			e				  = hipEventCreateWithFlags(&s.h2d, hipEventDisableTiming);
+			if (e == hipSuccess)
+				e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+			if (e == hipSuccess)
+				e = hipHostMalloc((void**)&s.h_st, (nd + 1) * 4, hipHostMallocDefault);
+			if (e == hipSuccess)
+				e = hipHostMalloc((void**)&s.h_br, (nd + 1) * 4, hipHostMallocDefault);
+			if (e == hipSuccess)
+				e = hipHostMalloc((void**)&s.h_flags, nd, hipHostMallocDefault);
+			if (e == hipSuccess)
+				e = hipHostMalloc((void**)&s.h_maxd, nd * 4, hipHostMallocDefault);
+			if (e != hipSuccess) {
+				r = shipfail(e, "server slot");
+				break;
+			}
+			// worst-case outputs of a full batch: positions 4 B, bracket records
+			// (position, index, key, byte, document, sort keys and ranks, pair) 45 B
+			sv->fixed += cfg.max_batch_bytes * 49;
+		}
+	}
+	if (r == PARS_OK && sv->fixed > cfg.hbm_budget)
+		r = sfail(PARS_ERR_BUDGET, "rings and scanner arenas alone exceed the HBM budget");
+	if (r != PARS_OK) {
+		delete sv;
+		return r;
+	}
+	sv->st.device_bytes = sv->fixed;
+	sv->worker			= std::thread([sv] { sv->run(); });
+	*out				= sv;
+	return PARS_OK;
+}
+
+int pars_hip_server_destroy(pars_server* sv) {
+	delete sv;
+	return PARS_OK;
+}
+
+int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, size_t n, uint64_t* id) {
+	if (!sv || !id || (n && !data) || plan >= sv->plans.size())
+		return sfail(PARS_ERR_ARGUMENT, "bad submit arguments");
+	if (n == 0 || n > sv->cfg.max_batch_bytes || n > 0xFFFFFFFFull)
+		return sfail(PARS_ERR_ARGUMENT, "request size must be 1 .. max_batch_bytes");
+	std::lock_guard<std::mutex> lk(sv->mu);
+	PlanState&					ps = sv->plans[plan];
+	uint64_t					off;
+	if (!ps.stage_ring.alloc(align_up(n), off)) {
+		sv->st.busy_rejects++;
+		return sfail(PARS_ERR_BUSY, "staging ring full");
+	}
+	std::memcpy(ps.stage + off, data, n);
+	if (align_up(n) > n)
+		std::memset(ps.stage + off + n, ' ', align_up(n) - n);
+	Request q;
+	q.id		= sv->next_id++;
+	q.plan		= plan;
+	q.n			= n;
+	q.stage_off = off;
+	q.reserve	= n * 18;  // the request's input plus its worst-case outputs
+	q.t_submit	= now_ns();
+	sv->reqs.emplace(q.id, q);
+	ps.pending.push_back(q.id);
+	sv->st.submitted++;
+	*id = q.id;
+	return PARS_OK;
+}
+
+int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, size_t* n_out, int64_t wait_us) {
+	if (!sv || !n_out || (cap && !out))
+		return sfail(PARS_ERR_ARGUMENT, "bad poll arguments");
+	std::unique_lock<std::mutex> lk(sv->mu);
+	if (sv->completed.empty() && wait_us > 0)
+		sv->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [sv] { return !sv->completed.empty(); });
+	size_t k = 0;
+	while (k < cap && !sv->completed.empty()) {
+		Request& q = sv->reqs[sv->completed.front()];
+		sv->completed.pop_front();
+		q.polled = true;
+		out[k++] = q.res;
+	}
+	*n_out = k;
+	return PARS_OK;
+}
+
+int pars_hip_server_release(pars_server* sv, uint64_t id) {
+	if (!sv)
+		return sfail(PARS_ERR_ARGUMENT, "null server");
+	std::lock_guard<std::mutex> lk(sv->mu);
+	auto						it = sv->reqs.find(id);
+	if (it == sv->reqs.end() || it->second.state != kDone)
+		return sfail(PARS_ERR_ARGUMENT, "unknown or unfinished request");
+	Request& q = it->second;
+	if (q.slot >= 0) {
+		Slot& s = sv->plans[q.plan].slots[q.slot];
+		if (s.outstanding)
+			s.outstanding--;
+		if (s.outstanding == 0 && s.state == kHeld)
+			s.state = kFree;
+	}
+	sv->reserved -= std::min(sv->reserved, q.reserve);
+	sv->st.released++;
+	sv->reqs.erase(it);
+	return PARS_OK;
+}
+
+int pars_hip_server_stats(pars_server* sv, pars_server_stats* out) {
+	if (!sv || !out)
+		return sfail(PARS_ERR_ARGUMENT, "null argument");
+	std::lock_guard<std::mutex> lk(sv->mu);
+	*out				   = sv->st;
+	out->inflight_reserved = sv->reserved;
+	return PARS_OK;
+}
+
+int pars_hip_server_replay(pars_server* sv, size_t n, const uint32_t* plan, const uint8_t* const* data,
+						   const uint64_t* sizes, const uint64_t* arrival_ns, uint64_t* latency_ns,
+						   uint64_t* elapsed_ns) {
+	if (!sv || !plan || !data || !sizes || !arrival_ns || !latency_ns)
+		return sfail(PARS_ERR_ARGUMENT, "null argument");
+	std::unordered_map<uint64_t, size_t> index;	 // request id -> i
+	std::vector<pars_request_result>	 res(256);
+	size_t								 done = 0;
+	uint64_t							 last = 0;
+	const uint64_t						 t0	  = now_ns();
+	uint64_t							 progress = t0;	 // last completion (stall guard)
+	auto								 drain	  = [&](int64_t wait_us) -> int {
+		size_t k = 0;
+		int	   r = pars_hip_server_poll(sv, res.data(), res.size(), &k, wait_us);
+		if (r)
+			return r;
+		if (k)
+			progress = now_ns();
+		else if (now_ns() - progress > 30ull * 1000000000ull && done < index.size())
+			return sfail(PARS_ERR_DEVICE, "server made no progress for 30 s");
+		for (size_t j = 0; j < k; ++j) {
+			const size_t i = index[res[j].id];
+			if (res[j].status != PARS_OK)
+				return sfail(res[j].status, "a request failed");
+			latency_ns[i] = res[j].t_done_ns - (t0 + arrival_ns[i]);
+			last		  = std::max(last, res[j].t_done_ns);
+			if ((r = pars_hip_server_release(sv, res[j].id)))
+				return r;
+			++done;
+		}
+		return PARS_OK;
+	};
+	int r;
+	for (size_t i = 0; i < n; ++i) {
+		while (now_ns() < t0 + arrival_ns[i])
+			if ((r = drain(0)))
+				return r;
+		uint64_t id;
+		while ((r = pars_hip_server_submit(sv, plan[i], data[i], sizes[i], &id)) == PARS_ERR_BUSY)
+			if ((r = drain(20)))
+				return r;
+		if (r)
+			return r;
+		index[id] = i;
+	}
+	while (done < n)
+		if ((r = drain(1000)))
+			return r;
+	if (elapsed_ns)
+		*elapsed_ns = last - (t0 + arrival_ns[0]);
+	return PARS_OK;
+}
+
+}  // extern "C"

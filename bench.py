@@ -170,6 +170,66 @@ def bench_batch(torch, n_docs: int, iters: int, plain_sample: int) -> dict:
     }
 
 
+def bench_serve(n_req: int, seed: int = 5) -> dict:
+    """Serving: Poisson arrivals of JSON / CSV / C-like requests (60/25/15,
+    log-normal sizes, median 4 KiB, sigma 1.85, clipped to [200 B, 4 MiB])
+    through a 4 GiB device ring under a 64 GiB HBM budget.  Saturation is
+    measured first (closed loop: every request offered at once); then offered
+    loads of 20%..100% of it in 10% steps.  Latency = scheduled arrival to
+    outputs complete on the device."""
+    import numpy as np
+
+    from pars_amd import Server
+    from pars_amd import workloads as W
+
+    plans = ["json", "csv", "clike"]
+    t0 = time.perf_counter()
+    kinds = W.doc_kinds(seed, n_req, {"json": 60, "csv": 25, "clike": 15})
+    sz = W.sizes(seed, n_req, median=4096.0, sigma=1.85)
+    docs = [W.make_doc(k, W.doc_seed(seed, i), int(sz[i])) for i, k in enumerate(kinds)]
+    pidx = [plans.index(k) for k in kinds]
+    total = int(sz.sum())
+    log(f"serve: {n_req} requests, {total} bytes, generated in {time.perf_counter() - t0:.1f} s")
+    sv = Server(plans)  # 4 GiB ring, 1 GiB staging, 64 GiB budget, 64 MiB batches
+    sv.replay(pidx[:200], docs[:200], np.zeros(200, np.uint64))  # warm every plan's kernels
+    lat, el = sv.replay(pidx, docs, np.zeros(n_req, np.uint64))
+    sat = n_req / (el / 1e9)
+    log(f"serve: saturation {sat:.0f} req/s, {total / el:.1f} GB/s")
+    rng = np.random.default_rng(seed)
+    points = []
+    for pct in range(20, 101, 10):
+        rate = sat * pct / 100.0
+        gaps = rng.exponential(1e9 / rate, n_req)
+        arrival = np.cumsum(gaps)
+        arrival = (arrival - arrival[0]).astype(np.uint64)
+        lat, el = sv.replay(pidx, docs, arrival)
+        pt = {
+            "load_pct": pct,
+            "offered_req_s": round(rate, 1),
+            "p50_us": round(float(np.percentile(lat, 50)) / 1e3, 1),
+            "p99_us": round(float(np.percentile(lat, 99)) / 1e3, 1),
+            "gb_per_s": round(total / el, 3),
+        }
+        points.append(pt)
+        log(f"serve: {pt}")
+    st = sv.stats()
+    sv.close()
+    at80 = next(p for p in points if p["load_pct"] == 80)
+    return {
+        "workload": "Poisson requests, 60% JSON / 25% CSV / 15% C-like, log-normal median 4 KiB (seed 5)",
+        "requests_per_point": n_req,
+        "bytes_per_point": total,
+        "saturation_req_s": round(sat, 1),
+        "saturation_gb_s": round(total / (n_req / sat) / 1e9, 3),
+        "p50_us_at_80": at80["p50_us"],
+        "p99_us_at_80": at80["p99_us"],
+        "gb_per_s_at_80": at80["gb_per_s"],
+        "points": points,
+        "batches": int(st.batches),
+        "device_bytes": int(st.device_bytes),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +245,9 @@ def main() -> int:
     ap.add_argument("--batch-docs", type=int, default=100_000)
     ap.add_argument("--batch-iters", type=int, default=5)
     ap.add_argument("--plain-sample", type=int, default=3000, help="documents timed through plain launches")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--only-serve", action="store_true", help="run only the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -196,6 +259,12 @@ def main() -> int:
     from pars_amd import _native as N
     from pars_amd import workloads as W
 
+    if args.only_serve:
+        import torch
+
+        torch.cuda.init()
+        print(json.dumps({"served": bench_serve(args.serve_requests)}), flush=True)
+        return 0
     if args.only_batch:
         import torch
 
@@ -304,6 +373,9 @@ def main() -> int:
     if not args.no_batch:
         line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample)
         log(f"batched: {line['batched']}")
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        line["served"] = bench_serve(args.serve_requests)
     print(json.dumps(line), flush=True)
     return 0
 

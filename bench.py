@@ -95,7 +95,7 @@ def cpu_baseline(doc, sample_bytes: int) -> dict:
     }
 
 
-def bench_batch(torch, n_docs: int, iters: int, plain_sample: int) -> dict:
+def bench_batch(torch, n_docs: int, iters: int, plain_sample: int, tune: bool = True) -> dict:
     """The 100k-document batch: 70% JSON / 30% CSV, log-normal sizes (median
     4 KiB, sigma 1.85, clipped to [200 B, 4 MiB]), one batched launch chain per
     plan (graph-replayed), against one plain launch chain per document."""
@@ -119,6 +119,12 @@ def bench_batch(torch, n_docs: int, iters: int, plain_sample: int) -> dict:
         sc.set_option(N.PARS_OPT_GRAPH, 1)
         groups.append((kind, d, spans, sc, len(idx)))
     torch.cuda.synchronize()
+    tuned = {}
+    if tune:
+        for kind, d, spans, sc, nd in groups:
+            t0 = time.perf_counter()
+            tuned[kind] = sc.autotune(d.data_ptr(), d.numel(), spans, stream=sp, reps=2)
+            log(f"batch autotune {kind} ({time.perf_counter() - t0:.1f} s): {tuned[kind]}")
 
     def one_pass():
         out = []
@@ -167,6 +173,7 @@ def bench_batch(torch, n_docs: int, iters: int, plain_sample: int) -> dict:
         "n_struct": int(sum(x[0] for x in first)),
         "n_brackets": int(sum(x[1] for x in first)),
         "graph": True,
+        "tuned": tuned or None,
     }
 
 
@@ -248,6 +255,10 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--only-serve", action="store_true", help="run only the serving measurement")
     ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs (no autotuner)")
+    ap.add_argument("--knobs", default="",
+                    help="fixed launch knobs for the document scan instead of tuning, e.g. lb_window=16,rows=4 "
+                         "(rows, lb_window, blocks_per_cu, radix_bits); profiles use this")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -269,7 +280,7 @@ def main() -> int:
         import torch
 
         torch.cuda.init()
-        r = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample)
+        r = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, tune=not args.no_tune)
         print(json.dumps({"batched": r}), flush=True)
         return 0
 
@@ -304,6 +315,20 @@ def main() -> int:
     sp = stream.cuda_stream
     st = N.StageTimes()
     ptr = d_doc.data_ptr()
+    tuned = None
+    if args.knobs:
+        opts = {"rows": N.PARS_OPT_ROWS_PER_TILE, "lb_window": N.PARS_OPT_LOOKBACK_WINDOW,
+                "blocks_per_cu": N.PARS_OPT_BLOCKS_PER_CU, "radix_bits": N.PARS_OPT_RADIX_BITS}
+        tuned = {}
+        for kv in args.knobs.split(","):
+            k, v = kv.split("=")
+            sc.set_option(opts[k.strip()], int(v))
+            tuned[k.strip()] = int(v)
+        log(f"fixed knobs: {tuned}")
+    elif not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = sc.autotune(ptr, n, stream=sp, reps=3)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}")
 
     for i in range(args.warmup):
         r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
@@ -350,6 +375,7 @@ def main() -> int:
             "n_struct": int(k),
             "n_brackets": int(kb),
             "parallelism": "single GPU",
+            "tuned": tuned,
         },
         "roofline": {
             "bound": "hbm",
@@ -371,7 +397,8 @@ def main() -> int:
     del d_doc
     torch.cuda.empty_cache()
     if not args.no_batch:
-        line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample)
+        line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample,
+                                      tune=not args.no_tune)
         log(f"batched: {line['batched']}")
     if not args.no_serve:
         torch.cuda.empty_cache()

@@ -103,8 +103,11 @@ typedef enum {
 	PARS_OPT_WANT_CHARS	 = 1, /* also emit structural bytes (default 0) */
 	PARS_OPT_WANT_BITMAP = 2, /* also emit the structural bitmap (default 1) */
 	PARS_OPT_PATH		 = 3, /* 0 = auto, 1 = force the general descriptor path */
-	PARS_OPT_ROWS_PER_TILE = 4, /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 */
-	PARS_OPT_GRAPH		   = 5  /* batches: capture the launch chain once per shape and replay it */
+	PARS_OPT_ROWS_PER_TILE = 4, /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 (bytes per lane: 64 x rows) */
+	PARS_OPT_GRAPH		   = 5, /* batches: capture the launch chain once per shape and replay it */
+	PARS_OPT_LOOKBACK_WINDOW = 6, /* predecessor records per look-back round trip: 0 = default (32), 8..64 */
+	PARS_OPT_BLOCKS_PER_CU	 = 7, /* resident scan blocks per CU (4 waves each): 0 = as many as fit */
+	PARS_OPT_RADIX_BITS		 = 8  /* pairing sort: digit width cap per pass, 0 = default (8), 1..8 */
 } pars_option;
 
 /* One document of a batch: its bytes are d_in[offset, offset + size). */
@@ -261,6 +264,30 @@ int pars_hip_server_stats(pars_server* sv, pars_server_stats* out);
 int pars_hip_server_replay(
 	pars_server* sv, size_t n, const uint32_t* plan, const uint8_t* const* data, const uint64_t* sizes,
 	const uint64_t* arrival_ns, uint64_t* latency_ns, uint64_t* elapsed_ns
+);
+
+/* Current value of an option. */
+int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value);
+
+/* ---- autotuner (no reference counterpart) ---------------------------------
+ * Times the scanner on the caller's input - one document, or a batch when docs
+ * is non-NULL - over its launch knobs and keeps the fastest setting on the
+ * scanner: rows per tile (bytes per lane and tile = 64 x rows), look-back
+ * window, resident blocks per CU (waves per SIMD), and the pairing sort's
+ * radix digit cap.  Coordinate descent from the current setting; a candidate
+ * must beat the best so far by 2 %; each is the median of `reps` runs (1..15)
+ * after a warm-up.  Overwrites the scanner's outputs; blocks. */
+typedef struct {
+	int32_t	 rows, lb_window, blocks_per_cu, radix_bits;  /* chosen option values (0 = default) */
+	float	 scan_ms, pair_ms;							  /* with the chosen setting */
+	float	 base_scan_ms, base_pair_ms;				  /* with the setting before tuning */
+	uint32_t trials;									  /* runs timed */
+	uint32_t max_blocks_per_cu;							  /* the scan kernel's occupancy limit */
+} pars_tune_result;
+
+int pars_hip_autotune(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, const pars_doc_span* docs, size_t n_docs, pars_stream stream,
+	int reps, pars_tune_result* out
 );
 
 /* pipeline.cuh:1225 scanner::struct_ptr. */

@@ -264,6 +264,20 @@ class Scanner:
             int(run_pairs), C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
         return BatchScan(r)
 
+    def get_option(self, option: int) -> int:
+        v = C.c_int64()
+        _check(self._lib.pars_hip_get_option(self._h, option, C.byref(v)))
+        return v.value
+
+    def autotune(self, d_data: int, n: int, spans=None, stream: int | None = None, reps: int = 3) -> dict:
+        """Time the launch knobs on this input (a document, or a batch with
+        spans) and keep the fastest setting (pars_hip_autotune)."""
+        r = N.TuneResult()
+        _check(self._lib.pars_hip_autotune(
+            self._h, C.c_void_p(d_data), n, spans, len(spans) if spans is not None else 0,
+            C.c_void_p(stream or 0), reps, C.byref(r)))
+        return r.as_dict()
+
     def struct_ptr(self) -> int:
         return self._lib.pars_hip_struct_ptr(self._h) or 0
 

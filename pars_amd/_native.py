@@ -26,6 +26,9 @@ PARS_OPT_WANT_BITMAP = 2
 PARS_OPT_PATH = 3
 PARS_OPT_ROWS_PER_TILE = 4
 PARS_OPT_GRAPH = 5
+PARS_OPT_LOOKBACK_WINDOW = 6
+PARS_OPT_BLOCKS_PER_CU = 7
+PARS_OPT_RADIX_BITS = 8
 
 
 class StageTimes(C.Structure):
@@ -146,6 +149,24 @@ class ServerStats(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int32),
+        ("lb_window", C.c_int32),
+        ("blocks_per_cu", C.c_int32),
+        ("radix_bits", C.c_int32),
+        ("scan_ms", C.c_float),
+        ("pair_ms", C.c_float),
+        ("base_scan_ms", C.c_float),
+        ("base_pair_ms", C.c_float),
+        ("trials", C.c_uint32),
+        ("max_blocks_per_cu", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: (round(getattr(self, k), 4) if t is C.c_float else getattr(self, k)) for k, t in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol include/pars_hip.h declares
 PROTOTYPES = {
     "pars_hip_scanner_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
@@ -185,6 +206,12 @@ PROTOTYPES = {
         C.c_int,
         [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
          C.POINTER(C.c_uint64)],
+    ),
+    "pars_hip_get_option": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]),
+    "pars_hip_autotune": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(DocSpan), C.c_size_t, C.c_void_p, C.c_int,
+         C.POINTER(TuneResult)],
     ),
     "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
     "pars_hip_last_error": (C.c_char_p, []),

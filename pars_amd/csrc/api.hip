@@ -94,6 +94,7 @@ struct pars_scanner {
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
+	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0;  // tuning; 0 = default
 	DevSummary* h_sum	 = nullptr;	 // pinned
 	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
 	// scan_host staging
@@ -207,6 +208,8 @@ ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool 
 	a.cap_struct = sc->cap_struct;
 	a.cap_br	 = sc->cap_br;
 	a.in_states	 = sc->hplan.start_packed;
+	a.lb_window	 = (uint32_t)sc->lb_window;
+	a.blocks_per_cu = (uint32_t)sc->blocks_per_cu;
 	return a;
 }
 
@@ -228,6 +231,7 @@ PairArgs make_pair_args(pars_scanner* sc, bool want_pairs) {
 	p.max_chunks = pairing_max_chunks(sc->cap_br);
 	p.pair		 = sc->pair.as<int32_t>();
 	p.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	p.max_digit_bits = (uint32_t)sc->radix_bits;
 	return p;
 }
 
@@ -240,6 +244,8 @@ int read_summary(pars_scanner* sc, hipStream_t s) {
 		return hip_fail(e, "hipStreamSynchronize");
 	if (sc->h_sum->overflow & 0x80000000u)
 		return fail(PARS_ERR_DEVICE, "scan look-back exceeded its spin bound (internal error)");
+	if (sc->h_sum->overflow & 8u)
+		return fail(PARS_ERR_ARGUMENT, "the radix digit cap needs more than four pairing passes for this key range");
 	return PARS_OK;
 }
 
@@ -396,6 +402,7 @@ std::vector<uintptr_t> batch_graph_key(pars_scanner* sc, const uint8_t* d_in, ui
 	std::vector<uintptr_t> k = {(uintptr_t)d_in, (uintptr_t)n, (uintptr_t)n_docs, (uintptr_t)s,
 								(uintptr_t)want_pairs, (uintptr_t)run_pairs, (uintptr_t)sc->want_chars,
 								(uintptr_t)sc->want_bitmap, (uintptr_t)sc->path, (uintptr_t)sc->rows,
+								(uintptr_t)sc->lb_window, (uintptr_t)sc->blocks_per_cu, (uintptr_t)sc->radix_bits,
 								(uintptr_t)sc->cap_struct, (uintptr_t)sc->cap_br, (uintptr_t)sc->h_doc_off,
 								(uintptr_t)sc->h_doc_len};
 	for (DevBuf* b : {&sc->recs, &sc->ticket, &sc->summary, &sc->positions, &sc->chars, &sc->bitmap, &sc->pair_pos,
@@ -481,7 +488,7 @@ int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint6
 }
 
 bool batch_failed(const void* h_summary) {
-	return (static_cast<const DevSummary*>(h_summary)->overflow & 0x80000005u) != 0;
+	return (static_cast<const DevSummary*>(h_summary)->overflow & 0x8000000Du) != 0;
 }
 
 }  // namespace pars_hip
@@ -549,6 +556,21 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 		sc->rows = (int)value;
 		return PARS_OK;
 	case PARS_OPT_GRAPH: sc->use_graph = value != 0; return PARS_OK;
+	case PARS_OPT_LOOKBACK_WINDOW:
+		if (value != 0 && (value < 8 || value > 64))
+			return fail(PARS_ERR_ARGUMENT, "look-back window must be 0 (default) or 8..64");
+		sc->lb_window = (int)value;
+		return PARS_OK;
+	case PARS_OPT_BLOCKS_PER_CU:
+		if (value < 0 || value > 64)
+			return fail(PARS_ERR_ARGUMENT, "blocks per CU must be 0 (as many as fit) .. 64");
+		sc->blocks_per_cu = (int)value;
+		return PARS_OK;
+	case PARS_OPT_RADIX_BITS:
+		if (value < 0 || value > 8)
+			return fail(PARS_ERR_ARGUMENT, "radix digit bits must be 0 (default 8) .. 8");
+		sc->radix_bits = (int)value;
+		return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }
@@ -825,6 +847,119 @@ int pars_hip_pair_chunks(pars_scanner* sc, const pars_chunk_carry* last, pars_st
 					  != (sc->hplan.start_packed & ((1u << (3 * sc->hplan.n_descs)) - 1u));
 	out->struct_bitmap = nullptr;
 	return PARS_OK;
+}
+
+
+int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value) {
+	if (!sc || !value)
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	switch (option) {
+	case PARS_OPT_WANT_CHARS: *value = sc->want_chars; return PARS_OK;
+	case PARS_OPT_WANT_BITMAP: *value = sc->want_bitmap; return PARS_OK;
+	case PARS_OPT_PATH: *value = sc->path; return PARS_OK;
+	case PARS_OPT_ROWS_PER_TILE: *value = sc->rows; return PARS_OK;
+	case PARS_OPT_GRAPH: *value = sc->use_graph; return PARS_OK;
+	case PARS_OPT_LOOKBACK_WINDOW: *value = sc->lb_window; return PARS_OK;
+	case PARS_OPT_BLOCKS_PER_CU: *value = sc->blocks_per_cu; return PARS_OK;
+	case PARS_OPT_RADIX_BITS: *value = sc->radix_bits; return PARS_OK;
+	}
+	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
+}
+
+// Coordinate descent over the launch knobs, each candidate timed on the
+// caller's input with the scanner's own stage events (scan kernel for the
+// scan knobs, the pairing chain for the digit cap).
+int pars_hip_autotune(pars_scanner* sc, const uint8_t* d_in, size_t n, const pars_doc_span* docs, size_t n_docs,
+					  pars_stream stream, int reps, pars_tune_result* out) {
+	if (!sc || !out || !d_in || !n || (docs && !n_docs))
+		return fail(PARS_ERR_ARGUMENT, "autotune needs a scanner, an input and a result");
+	reps = std::min(std::max(reps, 1), 15);
+	std::memset(out, 0, sizeof *out);
+	const int saved_graph = sc->use_graph;
+	sc->use_graph		  = 0;	// stage times split scan and pairing only eagerly
+	uint64_t n_brackets	  = 0;
+	// median stage times of `reps` runs after one warm-up; failing settings
+	// (a digit cap too narrow for the key range) report a status
+	auto measure = [&](float& scan_ms, float& pair_ms) -> int {
+		std::vector<float> a, b;
+		for (int i = 0; i <= reps; ++i) {
+			pars_stage_times st {};
+			int				 r;
+			if (docs) {
+				pars_batch_result br;
+				r			= pars_hip_scan_batch(sc, d_in, n, docs, n_docs, stream, 0, 1, &st, &br);
+				n_brackets = br.n_brackets;
+			} else {
+				pars_device_result dr;
+				r			= pars_hip_scan_device(sc, d_in, n, stream, 0, 1, &st, &dr);
+				n_brackets = sc->h_sum->n_brackets;
+			}
+			if (r)
+				return r;
+			++out->trials;
+			if (i) {
+				a.push_back(st.p1);
+				b.push_back(st.p4);
+			}
+		}
+		std::sort(a.begin(), a.end());
+		std::sort(b.begin(), b.end());
+		scan_ms = a[a.size() / 2];
+		pair_ms = b[b.size() / 2];
+		return PARS_OK;
+	};
+	int	  r;
+	float best_scan = 0, best_pair = 0;
+	if ((r = measure(best_scan, best_pair))) {
+		sc->use_graph = saved_graph;
+		return r;
+	}
+	out->base_scan_ms = best_scan;
+	out->base_pair_ms = best_pair;
+	// one knob at a time; `field` keeps the best value seen
+	auto sweep = [&](int& field, const std::vector<int>& values, bool pairing) {
+		const int keep = field;
+		int		  best = keep;
+		for (int v : values) {
+			if (v == keep)
+				continue;
+			field = v;
+			float sm = 0, pm = 0;
+			if (measure(sm, pm))
+				continue;  // not usable for this input
+			if (pairing ? pm < best_pair * 0.98f : sm < best_scan * 0.98f) {
+				best	  = v;
+				best_scan = pairing ? best_scan : sm;
+				best_pair = pairing ? pm : best_pair;
+			}
+		}
+		field = best;
+	};
+	const bool one_desc = sc->hplan.n_descs == 1;  // several descriptors: one row per tile always
+	if (one_desc)
+		sweep(sc->rows, {1, 2, 4}, false);
+	sweep(sc->lb_window, {16, 32, 48, 64}, false);
+	// resident blocks per CU below the kernel's occupancy limit
+	float sm = 0, pm = 0;
+	if ((r = measure(sm, pm)) == PARS_OK) {
+		const int occ			= std::max(last_scan_occupancy(), 1);
+		out->max_blocks_per_cu = (uint32_t)occ;
+		std::vector<int> bpc;
+		for (int b = 1; b < occ; ++b) bpc.push_back(b);
+		sweep(sc->blocks_per_cu, bpc, false);
+	}
+	if (n_brackets)
+		sweep(sc->radix_bits, {8, 7, 6, 5, 4}, true);
+	if ((r = measure(sm, pm)) == PARS_OK) {
+		out->scan_ms = sm;
+		out->pair_ms = pm;
+	}
+	sc->use_graph		= saved_graph;
+	out->rows			= sc->rows;
+	out->lb_window		= sc->lb_window;
+	out->blocks_per_cu	= sc->blocks_per_cu;
+	out->radix_bits		= sc->radix_bits;
+	return r;
 }
 
 }  // extern "C"

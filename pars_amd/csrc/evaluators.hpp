@@ -14,6 +14,8 @@
 
 namespace pars_hip {
 
+void set_last_scan_occupancy(int per_cu);
+
 namespace {
 
 template<bool WIDE>
@@ -78,9 +80,10 @@ __device__ __forceinline__ uint64_t gather_byte0(const uint32_t (&c)[8]) {
 // ---------------------------------------------------------------------------
 template<int ND, bool WIDE>
 struct GeneralEval {
-	static constexpr int  kND	= ND;
-	static constexpr bool kWide = WIDE;
-	static constexpr int  NDA	= ND > 0 ? ND : 1;
+	static constexpr int  kND		   = ND;
+	static constexpr bool kWide		   = WIDE;
+	static constexpr bool kBitParallel = false;
+	static constexpr int  NDA		   = ND > 0 ? ND : 1;
 	using M						= MapT<WIDE>;
 	using Map					= typename M::type;
 	using DT					= typename DTEntry<WIDE>::type;
@@ -251,8 +254,9 @@ __device__ __forceinline__ uint32_t esc_code(const EscHyp& h, uint64_t BS, uint6
 }
 
 struct EscapeEval {
-	static constexpr int  kND	= 1;
-	static constexpr bool kWide = false;
+	static constexpr int  kND		   = 1;
+	static constexpr bool kWide		   = false;
+	static constexpr bool kBitParallel = true;
 	using Map					= Map4;
 
 	struct Lds {
@@ -345,8 +349,9 @@ struct EscapeEval {
 // i-1 is a delimiter.  One prefix-xor ladder serves every entry state.
 // ---------------------------------------------------------------------------
 struct ParityEval {
-	static constexpr int  kND	= 1;
-	static constexpr bool kWide = false;
+	static constexpr int  kND		   = 1;
+	static constexpr bool kWide		   = false;
+	static constexpr bool kBitParallel = true;
 	using Map					= Map4;
 	using Lds					= EscapeEval::Lds;
 
@@ -411,19 +416,22 @@ struct ParityEval {
 // than there are tiles); each draws tiles until none is left.
 template<class E, int W, bool BATCH>
 hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
-	static int resident = 0;  // blocks resident at once, per kernel instance
-	if (!resident) {
-		int dev = 0, cus = 0, per_cu = 0;
+	static int cus = 0, per_cu = 0;	 // per kernel instance
+	if (!cus) {
+		int		   dev = 0, c = 0, p = 0;
 		hipError_t e = hipGetDevice(&dev);
 		if (e == hipSuccess)
-			e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+			e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
 		if (e == hipSuccess)
-			e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_scan<E, W, BATCH>, kBlock, 0);
+			e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_scan<E, W, BATCH>, kBlock, 0);
 		if (e != hipSuccess)
 			return e;
-		resident = (cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+		per_cu = p > 0 ? p : 1;
+		cus	   = c > 0 ? c : 1;
 	}
-	ScanArgs a			  = in;
+	set_last_scan_occupancy(per_cu);
+	const int resident = cus * (in.blocks_per_cu > 0 && (int)in.blocks_per_cu < per_cu ? (int)in.blocks_per_cu : per_cu);
+	ScanArgs  a		   = in;
 	const uint64_t tbytes = (uint64_t)W * kTile;
 	a.n_tiles			  = (uint32_t)((a.n + tbytes - 1) / tbytes);
 	if (a.n_tiles == 0)

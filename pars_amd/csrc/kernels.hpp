@@ -94,8 +94,17 @@ struct ScanArgs {
 	uint32_t		br_base;	 // bracket index of the first bracket (chunked mode)
 	int32_t			depth_base;	 // depth before the first byte (chunked mode)
 	uint32_t		in_states;	 // packed per-descriptor states before the first byte
+	// Tuning (launch shape): predecessor records read per look-back round trip
+	// (8..64), and resident blocks per CU of the persistent grid (0 = as many
+	// as fit).
+	uint32_t		lb_window;
+	uint32_t		blocks_per_cu;
 	BatchArgs		b;			 // batch mode only
 };
+
+// Blocks per CU the most recent scan launch on this thread could keep
+// resident (its occupancy limit), so a tuner can sweep below it.
+int last_scan_occupancy();
 
 // R1-R5: one pass over the input, general descriptor path.  `rows` = 16 KiB
 // rows per tile (1, 2 or 4; 0 = the default); tuned per plan.
@@ -154,6 +163,7 @@ struct PairArgs {
 	const uint32_t* st_start;
 	uint32_t		n_docs;
 	uint32_t*		doc_bad;  // [n_docs] a sorted pair of the document does not match
+	uint32_t		max_digit_bits;	 // radix digit width cap per pass (1..8; 0 = 8)
 };
 
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s);

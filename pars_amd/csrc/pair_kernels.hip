@@ -11,7 +11,7 @@
 // Layout: the scan kernel leaves bkey/bchar/bsidx per bracket.  The radix sort
 // runs LSD passes over u = key - kmin with a digit width fitted to the key
 // range found on device (one 6-bit pass for every document up to depth 63,
-// at most 8 bits per pass, 1..4 passes): per-chunk digit histograms (LDS
+// at most 8 bits per pass - or a tuned narrower cap - and 1..4 passes): per-chunk digit histograms (LDS
 // atomics — counts are order-free), one device-wide exclusive scan of the
 // digit-major histogram, then a stable scatter that ranks equal digits in
 // input order with wave ballots.  The last pass also records each bracket's
@@ -51,7 +51,10 @@ __global__ void k_sort_prep(PairArgs a) {
 		s->overflow |= 4u;
 		bits = 32u;
 	}
-	const uint32_t p = (bits + 7u) / 8u;
+	const uint32_t cap = a.max_digit_bits >= 1u && a.max_digit_bits <= 8u ? a.max_digit_bits : 8u;
+	const uint32_t p   = (bits + cap - 1u) / cap;
+	if (p > 4u)	 // four passes are launched: wider digits
+		s->overflow |= 8u;
 	s->key_span		 = (uint32_t)span;
 	s->kmin			 = kmin;
 	s->sort_passes		 = p;

@@ -42,8 +42,13 @@ namespace pars_hip {
 constexpr uint32_t kSpinLimit	= 1u << 20;
 constexpr uint32_t kErrLookback = 0x80000000u;
 // Predecessor records read per look-back round trip (the newest lanes of the
-// wave); the nearest inclusive prefix is typically 10-20 tiles back.
+// wave) unless tuned; the nearest inclusive prefix is typically 10-20 tiles
+// back.
 constexpr int kLbWindow = 32;
+
+__device__ __forceinline__ int lb_window(uint32_t w) {
+	return w >= 8u && w <= 64u ? (int)w : kLbWindow;
+}
 // Bit 31 of an aggregate's map word (one chain) or structural-count word (two
 // chains): the tile holds a document head (batch mode).
 constexpr uint32_t kHeadFlag = 0x80000000u;
@@ -79,7 +84,7 @@ __device__ __forceinline__ uint32_t pick(const uint32_t (&v)[N], uint32_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Look-backs run in one full wave.  Lane L >= 64 - kLbWindow reads the record
+// Look-backs run in one full wave.  Lane L >= 64 - window reads the record
 // of tile jb - 63 + L, so lane 63 holds the nearest predecessor and folding a
 // window is a forward wave scan (lanes below the window are neutral); the
 // newest lane holding an inclusive prefix (kp) ends the walk.  While a
@@ -107,11 +112,11 @@ __device__ __forceinline__ bool window_ready(const Window& w) {
 // Two-chain mode, chain 1: packed per-descriptor states entering tile > 0.
 template<int ND, bool WIDE>
 __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_t epoch, uint32_t in_states,
-									DevSummary* sum) {
+									int win, DevSummary* sum) {
 	using M	  = MapT<WIDE>;
 	using Map = typename M::type;
 	const int  lane	  = lane_id();
-	const bool active = lane >= kWave - kLbWindow;
+	const bool active = lane >= kWave - win;
 	Map		   acc[ND];	 // windows folded so far: their entry -> the tile's entry
 #pragma unroll
 	for (int d = 0; d < ND; ++d) acc[d] = M::id();
@@ -156,24 +161,24 @@ __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_
 			load = none;  // only records still missing are read again
 			continue;
 		}
-		Map win[ND];  // m_63 . m_62 . ... . m_{kp+1}
+		Map wmap[ND];  // m_63 . m_62 . ... . m_{kp+1}
 #pragma unroll
 		for (int d = 0; d < ND; ++d)
-			win[d] = M::readlane(wave_scan_maps<WIDE>(lane > w.kp ? agg[d] : M::id()), 63);
+			wmap[d] = M::readlane(wave_scan_maps<WIDE>(lane > w.kp ? agg[d] : M::id()), 63);
 		if (w.kp >= 0) {
 			const uint32_t S   = readlane(ival, w.kp);
 			uint32_t	   out = 0;
 #pragma unroll
 			for (int d = 0; d < ND; ++d) {
 				uint32_t sd = (S >> (3 * d)) & 7u;
-				sd			= M::apply(acc[d], M::apply(win[d], sd));
+				sd			= M::apply(acc[d], M::apply(wmap[d], sd));
 				out |= sd << (3 * d);
 			}
 			return out;
 		}
 #pragma unroll
-		for (int d = 0; d < ND; ++d) acc[d] = M::compose(acc[d], win[d]);
-		jb -= kLbWindow;
+		for (int d = 0; d < ND; ++d) acc[d] = M::compose(acc[d], wmap[d]);
+		jb -= win;
 		load = true;
 	}
 }
@@ -183,9 +188,9 @@ __device__ uint32_t lookback_states(const uint64_t* recs, uint32_t tile, uint32_
 // head: its depth word is the depth after its last head, and older depths do
 // not reach past it (structural and bracket offsets always do).
 __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint32_t epoch, const uint32_t base[3],
-									   uint32_t out[3], DevSummary* sum) {
+									   int win, uint32_t out[3], DevSummary* sum) {
 	const int  lane	  = lane_id();
-	const bool active = lane >= kWave - kLbWindow;
+	const bool active = lane >= kWave - win;
 	uint32_t   acc[3] = {0u, 0u, 0u};
 	int64_t	   jb	  = (int64_t)tile - 1;
 	uint32_t   spins  = 0;
@@ -251,7 +256,7 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 			out[2] = acc[2];
 			return;
 		}
-		jb -= kLbWindow;
+		jb -= win;
 		load = true;
 	}
 }
@@ -265,11 +270,11 @@ __device__ inline void lookback_counts(const uint64_t* recs, uint32_t tile, uint
 // entry state (for every entry state) and the walk moves on.
 template<bool WIDE, int NS>
 __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epoch, uint32_t in_states,
-							   const uint32_t base[3], uint32_t& st, uint32_t cnt[3], DevSummary* sum) {
+							   const uint32_t base[3], int win, uint32_t& st, uint32_t cnt[3], DevSummary* sum) {
 	using M	  = MapT<WIDE>;
 	using Map = typename M::type;
 	const int  lane	  = lane_id();
-	const bool active = lane >= kWave - kLbWindow;
+	const bool active = lane >= kWave - win;
 	uint32_t   accS[NS], accC[NS][3];  // entry state e of the folded windows -> (tile entry state, counts)
 #pragma unroll
 	for (int e = 0; e < NS; ++e) {
@@ -390,7 +395,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 			for (int f = 0; f < 3; ++f) accC[e][f] = nC[e][f];
 		}
 		closed = closed || hb;
-		jb -= kLbWindow;
+		jb -= win;
 		load = true;
 	}
 }
@@ -413,6 +418,7 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 //
 // E (the lane evaluator) provides:
 //   kND, kWide, Map        descriptors and their map type
+//   kBitParallel           a bit-parallel lane evaluator (no per-byte state walk)
 //   Lds, load(Lds&, plan)  LDS tables (cooperative load, all threads)
 //   walk(Lds, plan, x, valid)   one pass over a lane's 64 bytes
 //   T[d]                    the lane's map after walk
@@ -424,13 +430,16 @@ constexpr uint32_t kRowBytes = kBlock * kBPL;  // 16 KiB
 constexpr int	   kCapP	 = 4096;		   // staged structural outputs per row
 constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
 
-// Many-descriptor plans with wide descriptors need most of the register file;
-// they are held to two waves per SIMD rather than left to fall to one.
-template<class E>
-constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : 1;
+// Waves per SIMD the register allocation must leave room for.  Many-descriptor
+// plans with wide descriptors need most of the register file; they are held to
+// two rather than left to fall to one.  The bit-parallel evaluators at four
+// rows per tile fit three (168 registers, a few spilled) and run 15 % faster
+// there than at two, where the look-back and scatter latencies stay exposed.
+template<class E, int W>
+constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : (E::kBitParallel && W == 4) ? 3 : 1;
 
 template<class E, int W, bool BATCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E>))) void k_scan(ScanArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E, W>))) void k_scan(ScanArgs a) {
 	constexpr int  ND	 = E::kND;
 	constexpr int  NDA	 = ND > 0 ? ND : 1;
 	constexpr bool FUSED = ND == 1;
@@ -533,10 +542,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		}
 
 		// ---- 1. one walk per row ----
-		// Rows are loaded one ahead of the walk (two row buffers in flight, not W).
+		// Every row's loads are issued before the first walk.
 		E		 ev[W];
 		uint32_t valid[W];
-		uint32_t xb[2][16];
+		uint32_t xb[W][16];
 		auto	 load_row = [&](int r, uint32_t (&x)[16]) {
 			const uint64_t base = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
 			valid[r]			= base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
@@ -564,14 +573,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 			if constexpr (BATCH)
 				valid[r] = min(valid[r], vdoc[r]);
 		};
-		load_row(0, xb[0]);
+#pragma unroll
+		for (int r = 0; r < W; ++r) load_row(r, xb[r]);
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
-			if (r + 1 < W) {
-				load_row(r + 1, xb[(r + 1) & 1]);
-				__asm__ volatile("" ::: "memory");	// keep later rows' loads behind this walk
-			}
-			ev[r].walk(s_lds, a.plan, xb[r & 1], valid[r]);
+			ev[r].walk(s_lds, a.plan, xb[r], valid[r]);
 			if constexpr (BATCH && ND > 0)
 				if (lhead[r])  // restart: whatever came before, the lane enters in the start states
 #pragma unroll
@@ -710,7 +716,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 														  M::pack(tagg[0]) | (last_head >= 0 ? kHeadFlag : 0u)));
 					if (wid == 0) {
 						uint32_t st, o[3];
-						lookback_fused<E::kWide, NS>(a.recs, tile, a.epoch, a.in_states, cbase, st, o, a.sum);
+						lookback_fused<E::kWide, NS>(a.recs, tile, a.epoch, a.in_states, cbase, lb_window(a.lb_window), st,
+													 o, a.sum);
 						if (lane == 0) {
 							s_misc[1] = st;
 							s_misc[2] = o[0];
@@ -732,7 +739,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					for (int d = 0; d < ND; ++d)
 						publish(rec + kRecSAgg + d, make_word(a.epoch, kAgg, M::pack(tagg[d])));
 				if (wid == 0) {
-					const uint32_t st = lookback_states<ND, E::kWide>(a.recs, tile, a.epoch, a.in_states, a.sum);
+					const uint32_t st =
+						lookback_states<ND, E::kWide>(a.recs, tile, a.epoch, a.in_states, lb_window(a.lb_window), a.sum);
 					if (lane == 0)
 						s_misc[1] = st;
 				}
@@ -852,7 +860,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				}
 				if (wid == 0) {
 					uint32_t o[3];
-					lookback_counts(a.recs, tile, a.epoch, cbase, o, a.sum);
+					lookback_counts(a.recs, tile, a.epoch, cbase, lb_window(a.lb_window), o, a.sum);
 					if (lane == 0) {
 						s_misc[2] = o[0];
 						s_misc[3] = o[1];

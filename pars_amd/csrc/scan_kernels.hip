@@ -26,6 +26,18 @@ __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 
 }  // namespace
 
+namespace {
+thread_local int t_last_occupancy = 0;
+}
+
+void set_last_scan_occupancy(int per_cu) {
+	t_last_occupancy = per_cu;
+}
+
+int last_scan_occupancy() {
+	return t_last_occupancy;
+}
+
 hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStream_t s) {
 	hipLaunchKernelGGL(k_summary_reset, dim3(1), dim3(64), 0, s, sum, in);
 	return hipGetLastError();

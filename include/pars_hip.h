@@ -103,7 +103,7 @@ typedef enum {
 	PARS_OPT_WANT_CHARS	 = 1, /* also emit structural bytes (default 0) */
 	PARS_OPT_WANT_BITMAP = 2, /* also emit the structural bitmap (default 1) */
 	PARS_OPT_PATH		 = 3, /* 0 = auto, 1 = force the general descriptor path */
-	PARS_OPT_ROWS_PER_TILE = 4, /* 16 KiB rows per scan tile: 0 = default, 1, 2 or 4 (bytes per lane: 64 x rows) */
+	PARS_OPT_ROWS_PER_TILE = 4, /* 16 KiB rows per scan tile: 0 = default, 1..4 (bytes per lane: 64 x rows) */
 	PARS_OPT_GRAPH		   = 5, /* batches: capture the launch chain once per shape and replay it */
 	PARS_OPT_LOOKBACK_WINDOW = 6, /* predecessor records per look-back round trip: 0 = default (32), 8..64 */
 	PARS_OPT_BLOCKS_PER_CU	 = 7, /* resident scan blocks per CU (4 waves each): 0 = as many as fit */

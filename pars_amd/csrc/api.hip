@@ -551,8 +551,8 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 		sc->path = (int)value;
 		return PARS_OK;
 	case PARS_OPT_ROWS_PER_TILE:
-		if (value != 0 && value != 1 && value != 2 && value != 4)
-			return fail(PARS_ERR_ARGUMENT, "rows per tile must be 0 (default), 1, 2 or 4");
+		if (value < 0 || value > 4)
+			return fail(PARS_ERR_ARGUMENT, "rows per tile must be 0 (default), 1, 2, 3 or 4");
 		sc->rows = (int)value;
 		return PARS_OK;
 	case PARS_OPT_GRAPH: sc->use_graph = value != 0; return PARS_OK;
@@ -937,7 +937,7 @@ int pars_hip_autotune(pars_scanner* sc, const uint8_t* d_in, size_t n, const par
 	};
 	const bool one_desc = sc->hplan.n_descs == 1;  // several descriptors: one row per tile always
 	if (one_desc)
-		sweep(sc->rows, {1, 2, 4}, false);
+		sweep(sc->rows, {1, 2, 3, 4}, false);
 	sweep(sc->lb_window, {16, 32, 48, 64}, false);
 	// resident blocks per CU below the kernel's occupancy limit
 	float sm = 0, pm = 0;

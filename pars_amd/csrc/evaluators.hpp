@@ -452,6 +452,7 @@ template<class E, bool BATCH = false>
 hipError_t launch_rows(const ScanArgs& a, int rows, int def, hipStream_t s) {
 	switch (rows ? rows : def) {
 	case 1: return launch_eval<E, 1, BATCH>(a, s);
+	case 3: return launch_eval<E, 3, BATCH>(a, s);
 	case 4: return launch_eval<E, 4, BATCH>(a, s);
 	default: return launch_eval<E, 2, BATCH>(a, s);
 	}

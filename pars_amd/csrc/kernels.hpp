@@ -107,7 +107,7 @@ struct ScanArgs {
 int last_scan_occupancy();
 
 // R1-R5: one pass over the input, general descriptor path.  `rows` = 16 KiB
-// rows per tile (1, 2 or 4; 0 = the default); tuned per plan.
+// rows per tile (1..4; 0 = the default); tuned per plan.
 hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 // R1-R5 for plans whose one descriptor is an affine escape automaton (JSON).
 hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s);

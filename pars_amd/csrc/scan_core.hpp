@@ -432,11 +432,12 @@ constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
 
 // Waves per SIMD the register allocation must leave room for.  Many-descriptor
 // plans with wide descriptors need most of the register file; they are held to
-// two rather than left to fall to one.  The bit-parallel evaluators at four
-// rows per tile fit three (168 registers, a few spilled) and run 15 % faster
-// there than at two, where the look-back and scatter latencies stay exposed.
+// two rather than left to fall to one.  The bit-parallel evaluators at three
+// and four rows per tile fit three (four rows: 168 registers, a few spilled)
+// and run 15 % faster there than at two, where the look-back and scatter
+// latencies stay exposed.
 template<class E, int W>
-constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : (E::kBitParallel && W == 4) ? 3 : 1;
+constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : (E::kBitParallel && W >= 3) ? 3 : 1;
 
 template<class E, int W, bool BATCH>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E, W>))) void k_scan(ScanArgs a) {

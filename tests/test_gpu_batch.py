@@ -107,7 +107,7 @@ def test_batch_mixed_sizes(torch_cuda, g):
     check_batch(g, docs, got, "default")
 
 
-@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("rows", [1, 2, 3, 4])
 @pytest.mark.parametrize("path", [0, 1])
 def test_batch_rows_and_paths(torch_cuda, rows, path):
     docs = mixed_docs("json", 23, 80, big=2)

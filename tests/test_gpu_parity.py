@@ -92,7 +92,7 @@ def test_generated_documents_multi_tile(torch_cuda, g, gen):
 
 
 @pytest.mark.parametrize("path", [0, 1])
-@pytest.mark.parametrize("rows", [1, 2, 4])
+@pytest.mark.parametrize("rows", [1, 2, 3, 4])
 @pytest.mark.parametrize("g", ["json", "csv", "clike6"])
 def test_rows_per_tile_and_paths(torch_cuda, g, rows, path):
     """Every tile shape and both evaluators (fast path, general walk) agree with the oracle."""

@@ -107,7 +107,8 @@ typedef enum {
 	PARS_OPT_GRAPH		   = 5, /* batches: capture the launch chain once per shape and replay it */
 	PARS_OPT_LOOKBACK_WINDOW = 6, /* predecessor records per look-back round trip: 0 = default (32), 8..64 */
 	PARS_OPT_BLOCKS_PER_CU	 = 7, /* resident scan blocks per CU (4 waves each): 0 = as many as fit */
-	PARS_OPT_RADIX_BITS		 = 8  /* pairing sort: digit width cap per pass, 0 = default (8), 1..8 */
+	PARS_OPT_RADIX_BITS		 = 8, /* pairing sort: digit width cap per pass, 0 = default (8), 1..8 */
+	PARS_OPT_CHASER			 = 9  /* one-descriptor plans: one block publishes inclusive prefixes ahead (default 0) */
 } pars_option;
 
 /* One document of a batch: its bytes are d_in[offset, offset + size). */

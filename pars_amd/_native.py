@@ -29,6 +29,7 @@ PARS_OPT_GRAPH = 5
 PARS_OPT_LOOKBACK_WINDOW = 6
 PARS_OPT_BLOCKS_PER_CU = 7
 PARS_OPT_RADIX_BITS = 8
+PARS_OPT_CHASER = 9
 
 
 class StageTimes(C.Structure):

@@ -95,6 +95,7 @@ struct pars_scanner {
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
 	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0;  // tuning; 0 = default
+	int		  chase		= 0;
 	DevSummary* h_sum	 = nullptr;	 // pinned
 	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
 	// scan_host staging
@@ -210,6 +211,7 @@ ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool 
 	a.in_states	 = sc->hplan.start_packed;
 	a.lb_window	 = (uint32_t)sc->lb_window;
 	a.blocks_per_cu = (uint32_t)sc->blocks_per_cu;
+	a.chase			= (uint32_t)sc->chase;
 	return a;
 }
 
@@ -403,6 +405,7 @@ std::vector<uintptr_t> batch_graph_key(pars_scanner* sc, const uint8_t* d_in, ui
 								(uintptr_t)want_pairs, (uintptr_t)run_pairs, (uintptr_t)sc->want_chars,
 								(uintptr_t)sc->want_bitmap, (uintptr_t)sc->path, (uintptr_t)sc->rows,
 								(uintptr_t)sc->lb_window, (uintptr_t)sc->blocks_per_cu, (uintptr_t)sc->radix_bits,
+								(uintptr_t)sc->chase,
 								(uintptr_t)sc->cap_struct, (uintptr_t)sc->cap_br, (uintptr_t)sc->h_doc_off,
 								(uintptr_t)sc->h_doc_len};
 	for (DevBuf* b : {&sc->recs, &sc->ticket, &sc->summary, &sc->positions, &sc->chars, &sc->bitmap, &sc->pair_pos,
@@ -571,6 +574,7 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 			return fail(PARS_ERR_ARGUMENT, "radix digit bits must be 0 (default 8) .. 8");
 		sc->radix_bits = (int)value;
 		return PARS_OK;
+	case PARS_OPT_CHASER: sc->chase = value != 0; return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }
@@ -862,6 +866,7 @@ int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value) {
 	case PARS_OPT_LOOKBACK_WINDOW: *value = sc->lb_window; return PARS_OK;
 	case PARS_OPT_BLOCKS_PER_CU: *value = sc->blocks_per_cu; return PARS_OK;
 	case PARS_OPT_RADIX_BITS: *value = sc->radix_bits; return PARS_OK;
+	case PARS_OPT_CHASER: *value = sc->chase; return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }

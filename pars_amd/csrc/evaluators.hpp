@@ -70,10 +70,16 @@ __device__ __forceinline__ uint64_t valid_mask(uint32_t valid) {
 	return valid >= 64 ? ~0ull : ((1ull << valid) - 1ull);
 }
 
-// Byte 0 of eight accumulated groups -> a 64-bit mask (byte g = group g).
+// Byte 0 (byte 1) of eight accumulated groups -> a 64-bit mask (byte g = group g).
 __device__ __forceinline__ uint64_t gather_byte0(const uint32_t (&c)[8]) {
 	const uint32_t lo = perm(c[1], c[0], 0x0C0C0400u) | perm(c[3], c[2], 0x04000C0Cu);
 	const uint32_t hi = perm(c[5], c[4], 0x0C0C0400u) | perm(c[7], c[6], 0x04000C0Cu);
+	return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t gather_byte1(const uint32_t (&c)[8]) {
+	const uint32_t lo = perm(c[1], c[0], 0x0C0C0501u) | perm(c[3], c[2], 0x05010C0Cu);
+	const uint32_t hi = perm(c[5], c[4], 0x0C0C0501u) | perm(c[7], c[6], 0x05010C0Cu);
 	return ((uint64_t)hi << 32) | lo;
 }
 
@@ -272,6 +278,7 @@ struct EscapeEval {
 
 	Map		 T[1];
 	uint64_t S, OC, OP;
+	uint64_t BO;  // bracket ordinal bits (plan br_chr); stored to LDS right after the walk
 	uint64_t mO, mI, mE;
 	uint32_t st_in, st_esc;
 
@@ -281,6 +288,7 @@ struct EscapeEval {
 		uint64_t Q;
 		gather_classes(clo, S, OC, OP, Q);
 		uint64_t	   BS = gather_byte0(chi);
+		BO				  = gather_byte1(chi);
 		const uint64_t vm = valid_mask(valid);	// bytes past the lane's end have no class
 		S &= vm;
 		OC &= vm;
@@ -359,12 +367,14 @@ struct ParityEval {
 
 	Map		 T[1];
 	uint64_t S, OC, OP, Q, Pe;
+	uint64_t BO;  // bracket ordinal bits (plan br_chr); stored to LDS right after the walk
 	uint32_t f_in, f_cl, doubled;
 
 	__device__ void walk(const Lds& L, const DevPlan* plan, const uint32_t (&x)[16], uint32_t valid) {
 		uint32_t clo[8], chi[8];
 		lookup_groups(L.cls, x, clo, chi);
 		gather_classes(clo, S, OC, OP, Q);
+		BO				  = gather_byte1(chi);
 		const uint64_t vm = valid_mask(valid);	// bytes past the lane's end have no class
 		S &= vm;
 		OC &= vm;
@@ -442,6 +452,8 @@ hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
 			return e;
 	}
 	const uint32_t grid = a.n_tiles < (uint32_t)resident ? a.n_tiles : (uint32_t)resident;
+	// a chaser pays off once tiles outnumber the resident blocks several times
+	a.chase = E::kND == 1 && !BATCH && in.chase != 0 && grid >= 2 && a.n_tiles >= 4 * grid;
 	hipLaunchKernelGGL((k_scan<E, W, BATCH>), dim3(grid), dim3(kBlock), 0, s, a);
 	return hipGetLastError();
 }

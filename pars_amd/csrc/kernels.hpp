@@ -99,6 +99,7 @@ struct ScanArgs {
 	// as fit).
 	uint32_t		lb_window;
 	uint32_t		blocks_per_cu;
+	uint32_t		chase;		 // set by the launcher: the last block chases aggregates (one chain)
 	BatchArgs		b;			 // batch mode only
 };
 

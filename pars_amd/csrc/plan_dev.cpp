@@ -171,12 +171,28 @@ int lower_plan(const pars_plan& S, DevPlan& P) {
 		P.fast_kind = kFastEscape;
 	else if (S.n_descs == 1 && is_parity_desc(S.descs[0], P.fk_delim, P.fk_state, P.fk_doubled))
 		P.fast_kind = kFastParity;
+	// one-bit ordinals of the bracket bytes: among the bytes that open, and
+	// among the close-only bytes
+	uint8_t ord[256] = {0};
+	int		n_open = 0, n_close = 0;
+	for (int c = 0; c < 256; ++c) {
+		if (op[c]) {
+			if (n_open < 2)
+				P.br_chr[1][n_open] = (uint8_t)c;
+			ord[c] = (uint8_t)(n_open++ & 1);
+		} else if (oc[c]) {
+			if (n_close < 2)
+				P.br_chr[0][n_close] = (uint8_t)c;
+			ord[c] = (uint8_t)(n_close++ & 1);
+		}
+	}
+	P.br_bit_ok = n_open <= 2 && n_close <= 2;
 	for (int j = 0; j < 8; ++j)
 		for (int c = 0; c < 256; ++c) {
 			const uint32_t q = P.fast_kind != kFastNone && c == P.fk_delim;
 			const uint32_t e = P.fast_kind == kFastEscape && c == P.fk_escape;
 			P.cls8[j][c][0]	 = P.cls[j][c] | (q << (24 + j));
-			P.cls8[j][c][1]	 = e << j;
+			P.cls8[j][c][1]	 = (e << j) | ((uint32_t)ord[c] << (8 + j));
 		}
 	return 0;
 }

@@ -42,6 +42,13 @@ struct DevPlan {
 	uint8_t fk_delim, fk_escape;
 	uint8_t fk_state[4];  // escape: out, in, closed, esc; parity: out, in, closed, closed
 	uint8_t fk_doubled;	  // parity: a delimiter right after a closing one is masked (CSV "")
+	// Bracket bytes without re-reading the input (bit-parallel evaluators):
+	// byte 1 of cls8[j][c][1] carries bit j = the ordinal of bracket byte c
+	// among the bytes that open (c opens) or among the close-only bytes, and
+	// br_chr[opens][ordinal] gives c back.  Only when each set has at most two
+	// bytes (br_bit_ok); otherwise bracket bytes are read from the input.
+	uint8_t br_bit_ok;
+	uint8_t br_chr[2][2];
 };
 
 enum FastKind : uint8_t {

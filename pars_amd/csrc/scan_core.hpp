@@ -400,6 +400,102 @@ __device__ void lookback_fused(const uint64_t* recs, uint32_t tile, uint32_t epo
 	}
 }
 
+// One-chain mode, single document: the chaser.  One wave of a block that draws
+// no tiles follows the tiles' aggregates in order, 64 tiles per round trip,
+// and publishes their inclusive words as soon as a prefix of them is ready -
+// ahead of the tiles' own look-backs, which then find an inclusive close by
+// instead of waiting for one to propagate tile by tile.  A tile publishes the
+// same inclusive itself (the values are a function of the input), and no tile
+// waits for the chaser, so it only ever shortens waits.  Lane L reads tile
+// f + L; the window's ready prefix (every tile with an aggregate or an
+// inclusive) is folded with a forward wave scan from its newest inclusive.
+template<bool WIDE, int NS>
+__device__ void chase_fused(uint64_t* recs, uint32_t n_tiles, uint32_t epoch, uint32_t in_states,
+							const uint32_t base[3], DevSummary* sum) {
+	using M	  = MapT<WIDE>;
+	using Map = typename M::type;
+	const int lane = lane_id();
+	uint32_t  S	   = in_states & 7u;  // state and counts before tile f
+	uint32_t  C[3] = {base[0], base[1], base[2]};
+	uint32_t  f = 0, spins = 0;
+	while (f < n_tiles) {
+		const uint32_t t	= f + (uint32_t)lane;
+		const bool	   live = t < n_tiles;
+		bool		   incl = false, agg = false;
+		uint32_t	   iS = 0, iC[3] = {0u, 0u, 0u};
+		Map			   m = M::id();
+		uint32_t	   v[3][NS];
+#pragma unroll
+		for (int e = 0; e < NS; ++e) v[0][e] = v[1][e] = v[2][e] = 0u;
+		if (live) {
+			const uint64_t* r = recs + (uint64_t)t * kRecWords;
+			uint64_t		wi[4], wv[3 * NS];
+#pragma unroll
+			for (int q = 0; q < 4; ++q) wi[q] = peek(r + kRecFIncl + q);
+			const uint64_t wm = peek(r + kRecFMap);
+#pragma unroll
+			for (int q = 0; q < 3 * NS; ++q) wv[q] = peek(r + kRecFCnt + q);
+			incl = true;
+#pragma unroll
+			for (int q = 0; q < 4; ++q) incl = incl && word_is(wi[q], epoch, kIncl);
+			agg = word_is(wm, epoch, kAgg);
+#pragma unroll
+			for (int q = 0; q < 3 * NS; ++q) agg = agg && word_is(wv[q], epoch, kAgg);
+			iS	  = (uint32_t)wi[0];
+			iC[0] = (uint32_t)wi[1];
+			iC[1] = (uint32_t)wi[2];
+			iC[2] = (uint32_t)wi[3];
+			m	  = M::unpack((uint32_t)wm);
+#pragma unroll
+			for (int e = 0; e < NS; ++e)
+#pragma unroll
+				for (int g = 0; g < 3; ++g) v[g][e] = (uint32_t)wv[3 * e + g];
+		}
+		const uint64_t ready = __ballot(live && (incl || agg));
+		const int	   P	 = ~ready ? (int)__builtin_ctzll(~ready) : 64;  // ready prefix
+		if (P == 0) {
+			if (++spins > kSpinLimit)
+				return;	 // an accelerator only: the tiles finish without it
+			__builtin_amdgcn_s_sleep(2);
+			continue;
+		}
+		spins			  = 0;
+		const bool	   in = lane < P;
+		const uint64_t bi = __ballot(in && incl);
+		const int	   kp = bi ? 63 - (int)__clzll((long long)bi) : -1;	 // newest inclusive
+		uint32_t	   S0 = S, B[3] = {C[0], C[1], C[2]};
+		if (kp >= 0) {
+			S0	 = readlane(iS, kp) & 7u;
+			B[0] = readlane(iC[0], kp);
+			B[1] = readlane(iC[1], kp);
+			B[2] = readlane(iC[2], kp);
+		}
+		const bool	   fold = in && lane > kp;	// tiles folded from their aggregates
+		const Map	   inc	= wave_scan_maps<WIDE>(fold ? m : M::id());
+		const Map	   exc	= wave_shift_maps<WIDE>(inc);
+		const uint32_t s_l	= M::apply(exc, S0);
+		uint32_t	   eS	= M::apply(inc, S0), eC[3];
+#pragma unroll
+		for (int g = 0; g < 3; ++g) eC[g] = B[g] + wave_incl_sum(fold ? pick(v[g], s_l) : 0u);
+		if (fold) {
+			uint64_t* r = recs + (uint64_t)t * kRecWords;
+			publish(r + kRecFIncl, make_word(epoch, kIncl, eS));
+#pragma unroll
+			for (int g = 0; g < 3; ++g) publish(r + kRecFIncl + 1 + g, make_word(epoch, kIncl, eC[g]));
+		}
+		if (lane == kp) {
+			eS = iS & 7u;
+#pragma unroll
+			for (int g = 0; g < 3; ++g) eC[g] = iC[g];
+		}
+		S = readlane(eS, P - 1) & 7u;
+#pragma unroll
+		for (int g = 0; g < 3; ++g) C[g] = readlane(eC[g], P - 1);
+		f += (uint32_t)P;
+	}
+	(void)sum;
+}
+
 // ---------------------------------------------------------------------------
 // The tile kernel.  Persistent: every block loads its LDS tables once, then
 // draws tiles from an atomic ticket in dispatch order until none is left (so a
@@ -465,6 +561,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 	__shared__ uint16_t		   s_bpos[kCapB];	// row-local byte offset
 	__shared__ uint16_t		   s_bsidx[kCapB];	// row-local structural index
 	__shared__ KeyT			   s_bkey[kCapB];	// key: relative to the row's entry depth / document-relative
+	// bit-parallel evaluators: each lane's bracket ordinal bits per row (from the
+	// walk) and the staged bracket bytes rebuilt from them
+	__shared__ uint64_t		   s_bo[E::kBitParallel ? W : 1][E::kBitParallel ? kBlock : 1];
+	__shared__ uint8_t		   s_bch[E::kBitParallel ? kCapB : 1];
 	__shared__ uint64_t		   s_hw[BATCH ? NHW : 1];		// the tile's head bits
 	__shared__ uint32_t		   s_hpre[BATCH ? NHW : 1];		// heads in the words before
 	__shared__ int32_t		   s_dhead[BATCH ? NL : 1];		// tile-relative depth at each head, by slot
@@ -474,6 +574,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 	a.epoch		  = a.sum->epoch;  // set by the summary reset that precedes every launch
 	E::load(s_lds, a.plan);
 	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
+	// bracket bytes from the walk's ordinal bits instead of the input
+	const bool	   bit_chr = E::kBitParallel && a.plan->br_bit_ok;
+	const uint32_t brc	   = (uint32_t)a.plan->br_chr[0][0] | ((uint32_t)a.plan->br_chr[0][1] << 8)
+						 | ((uint32_t)a.plan->br_chr[1][0] << 16) | ((uint32_t)a.plan->br_chr[1][1] << 24);
 
 	// batch: lane L of the tile (row-major) -> its document slot (document d0 + slot)
 	auto slot_of = [&](int L) -> uint32_t {
@@ -481,9 +585,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		return s_hpre[L >> 6] + popc64(hw & (~0ull >> (63 - (L & 63)))) - (uint32_t)(s_hw[0] & 1ull);
 	};
 
+	// One chain, one document: the grid's last block draws no tiles and runs
+	// the chaser instead when the launch asks for it.
+	bool chaser = false;
+	if constexpr (FUSED && !BATCH) {
+		chaser = a.chase && blockIdx.x == gridDim.x - 1;
+		if (chaser && wid == 0) {
+			const uint32_t base[3] = {a.out_base, a.br_base, (uint32_t)a.depth_base};
+			chase_fused<E::kWide, NS>(a.recs, a.n_tiles, a.epoch, a.in_states, base, a.sum);
+		}
+	}
+
 	// A block takes its next tile only when it is ready to start it: a ticket
 	// drawn early leaves a hole that every later tile's look-back waits on.
-	while (true) {
+	while (!chaser) {
 		__syncthreads();  // LDS of the previous tile is free
 		if (tid == 0)
 			s_misc[0] = atomicAdd(a.ticket, 1u);
@@ -579,6 +694,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
 			ev[r].walk(s_lds, a.plan, xb[r], valid[r]);
+			if constexpr (E::kBitParallel)
+				s_bo[r][tid] = ev[r].BO;  // read back by this thread only, at the scatter
 			if constexpr (BATCH && ND > 0)
 				if (lhead[r])  // restart: whatever came before, the lane enters in the start states
 #pragma unroll
@@ -941,6 +1058,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				uint64_t bm = om[r];
 				uint32_t lb = cpre[r][1];
 				int32_t	 bal = dl;	// brackets come in order: a running depth
+				uint64_t bo = 0;
+				if constexpr (E::kBitParallel)
+					bo = s_bo[r][tid];
 				while (bm) {
 					const int	  p	   = __builtin_ctzll(bm);
 					const bool	  op   = (opm[r] >> p) & 1ull;
@@ -951,6 +1071,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					s_bpos[lb]	= (uint16_t)(loff + (uint32_t)p);
 					s_bsidx[lb] = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
 					s_bkey[lb]	= (KeyT)((op ? bal0 : bal) - (BATCH ? 0 : rowd));
+					if constexpr (E::kBitParallel)
+						s_bch[lb] = (uint8_t)(brc >> (8 * ((op ? 2 : 0) + (int)((bo >> p) & 1ull))));
 					++lb;
 					bm &= bm - 1;
 				}
@@ -992,7 +1114,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 #pragma unroll
 					for (int u = 0; u < U; ++u) {
 						const uint32_t i = i0 + u * kBlock + tid;
-						ch[u]			 = i < mb ? a.data[rowb + s_bpos[i]] : 0;
+						if constexpr (E::kBitParallel)
+							ch[u] = i >= mb ? 0 : bit_chr ? s_bch[i] : a.data[rowb + s_bpos[i]];
+						else
+							ch[u] = i < mb ? a.data[rowb + s_bpos[i]] : 0;
 					}
 #pragma unroll
 					for (int u = 0; u < U; ++u) {

@@ -107,11 +107,11 @@ struct pars_scanner {
 	int32_t*  h_pairs	  = nullptr;
 	size_t	  h_pairs_cap = 0;
 	// batches: the document table (pinned staging + device), per-document results
-	DevBuf	  doc_off, doc_len, tile_doc, head_bits, st_start, br_start, doc_final, doc_min, doc_max, doc_last,
-		bdoc, doc_bad, doc_flags;
-	uint64_t* h_doc_off	 = nullptr;
-	uint32_t* h_doc_len	 = nullptr;
-	size_t	  h_off_cap = 0, h_len_cap = 0;
+	// the document table: n offsets (u64) then n lengths (u32), one copy up
+	DevBuf	  doc_tab, tile_doc, head_bits, st_start, br_start, doc_final, doc_min, doc_max, doc_last, bdoc, doc_bad,
+		doc_flags, doc_pack;
+	uint8_t*  h_doctab	 = nullptr;
+	size_t	  h_tab_cap	 = 0;
 	int		  use_graph	 = 0;
 	hipGraphExec_t			gexec = nullptr;
 	std::vector<uintptr_t>	gkey;  // shape and buffers the captured graph was made for
@@ -121,13 +121,11 @@ struct pars_scanner {
 			(void)hipGraphExecDestroy(gexec);
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
-						  &doc_off, &doc_len, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min,
-						  &doc_max, &doc_last, &bdoc, &doc_bad, &doc_flags})
+						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
+						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack})
 			b->release();
-		if (h_doc_off)
-			(void)hipHostFree(h_doc_off);
-		if (h_doc_len)
-			(void)hipHostFree(h_doc_len);
+		if (h_doctab)
+			(void)hipHostFree(h_doctab);
 		if (h_sum)
 			(void)hipHostFree(h_sum);
 		if (h_in)
@@ -297,6 +295,7 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 		return hip_fail(e, "hipEventRecord");
 	if (run_pairs) {
 		PairArgs p = make_pair_args(sc, want_pairs);
+		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, 1);  // a set digit cap asks for the radix path
 		if ((e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 	}
@@ -308,10 +307,19 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 // ---------------------------------------------------------------------------
 // Batches.
 // ---------------------------------------------------------------------------
+// The host document table: offsets at [0, 8n), lengths at [8n, 12n).
+uint64_t* tab_off(pars_scanner* sc) {
+	return reinterpret_cast<uint64_t*>(sc->h_doctab);
+}
+
+uint32_t* tab_len(pars_scanner* sc, uint64_t n_docs) {
+	return reinterpret_cast<uint32_t*>(sc->h_doctab + 8 * n_docs);
+}
+
 BatchArgs make_batch_args(pars_scanner* sc, uint64_t n, uint64_t n_docs) {
 	BatchArgs b {};
-	b.doc_off	   = sc->doc_off.as<uint64_t>();
-	b.doc_len	   = sc->doc_len.as<uint32_t>();
+	b.doc_off	   = sc->doc_tab.as<uint64_t>();
+	b.doc_len	   = reinterpret_cast<const uint32_t*>(sc->doc_tab.as<uint8_t>() + 8 * n_docs);
 	b.tile_doc	   = sc->tile_doc.as<int32_t>();
 	b.head_bits	   = sc->head_bits.as<uint64_t>();
 	b.n_head_words = (((n + 63) / 64) + 63) / 64;
@@ -334,7 +342,7 @@ int ensure_batch(pars_scanner* sc, uint64_t n, uint64_t n_docs, uint64_t cap_br,
 	if (sc->want_bitmap && (r = sc->bitmap.ensure(((n + 63) / 64) * 8)))
 		return r;
 	const uint64_t words = (((n + 63) / 64) + 63) / 64;
-	if ((r = sc->doc_off.ensure(n_docs * 8)) || (r = sc->doc_len.ensure(n_docs * 4))
+	if ((r = sc->doc_tab.ensure(n_docs * 12)) || (r = sc->doc_pack.ensure(batch_pack_words(n_docs) * 4))
 		|| (r = sc->tile_doc.ensure(n_tiles * 4)) || (r = sc->head_bits.ensure(std::max<uint64_t>(words, 1) * 8))
 		|| (r = sc->st_start.ensure((n_docs + 1) * 4)) || (r = sc->br_start.ensure((n_docs + 1) * 4))
 		|| (r = sc->doc_final.ensure(n_docs * 4)) || (r = sc->doc_min.ensure(n_docs * 4))
@@ -342,8 +350,7 @@ int ensure_batch(pars_scanner* sc, uint64_t n, uint64_t n_docs, uint64_t cap_br,
 		|| (r = sc->doc_bad.ensure(n_docs * 4)) || (r = sc->doc_flags.ensure(n_docs))
 		|| (r = sc->bdoc.ensure(sc->cap_br * 4)))
 		return r;
-	if ((r = ensure_pinned((void**)&sc->h_doc_off, sc->h_off_cap, n_docs * 8))
-		|| (r = ensure_pinned((void**)&sc->h_doc_len, sc->h_len_cap, n_docs * 4)))
+	if ((r = ensure_pinned((void**)&sc->h_doctab, sc->h_tab_cap, n_docs * 12)))
 		return r;
 	return PARS_OK;
 }
@@ -358,18 +365,17 @@ hipError_t launch_batch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s)
 
 // The whole batch: no host sync inside, so it can be captured into a graph.
 int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_docs, hipStream_t s,
-				  bool want_pairs, bool run_pairs, bool timed) {
+				  bool want_pairs, bool run_pairs, bool timed, bool packed = false) {
 	hipError_t e;
-	if ((e = launch_summary_reset(sc->summary.as<DevSummary>(), SummaryInit {}, s)) != hipSuccess)
-		return hip_fail(e, "summary reset");
-	if ((e = hipMemcpyAsync(sc->doc_off.p, sc->h_doc_off, n_docs * 8, hipMemcpyHostToDevice, s)) != hipSuccess
-		|| (e = hipMemcpyAsync(sc->doc_len.p, sc->h_doc_len, n_docs * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+	if ((e = hipMemcpyAsync(sc->doc_tab.p, sc->h_doctab, n_docs * 12, hipMemcpyHostToDevice, s)) != hipSuccess)
 		return hip_fail(e, "document table H2D");
-	const BatchArgs b = make_batch_args(sc, n, n_docs);
-	if ((e = launch_batch_heads(b, s)) != hipSuccess)
-		return hip_fail(e, "batch heads");
-	if (run_pairs && (e = launch_fill_i32(sc->doc_bad.as<int32_t>(), n_docs, 0, s)) != hipSuccess)
-		return hip_fail(e, "batch reset");
+	BatchArgs b = make_batch_args(sc, n, n_docs);
+	b.pack		= packed ? sc->doc_pack.as<uint32_t>() : nullptr;
+	b.tile_rows = batch_tile_rows(sc->hplan, sc->path, sc->rows);
+	if ((e = launch_batch_prep(b, n, sc->summary.as<DevSummary>(), run_pairs ? sc->doc_bad.as<uint32_t>() : nullptr,
+							   b.tile_rows, s))
+		!= hipSuccess)
+		return hip_fail(e, "batch prep");
 	if (timed && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
 	ScanArgs a = make_scan_args(sc, d_in, n, want_pairs);
@@ -385,13 +391,14 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 		p.st_start = b.st_start;
 		p.n_docs   = b.n_docs;
 		p.doc_bad  = sc->doc_bad.as<uint32_t>();
+		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, n_docs);
 		if ((e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 	}
 	const uint32_t nd = sc->hplan.n_descs;
 	const uint32_t m  = nd ? ((1u << (3 * nd)) - 1u) : 0u;
 	if ((e = launch_batch_finish(b, sc->doc_bad.as<uint32_t>(), m, sc->hplan.start_packed, run_pairs ? 1 : 0,
-								 sc->doc_flags.as<uint8_t>(), s))
+								 sc->doc_flags.as<uint8_t>(), sc->summary.as<DevSummary>(), s))
 		!= hipSuccess)
 		return hip_fail(e, "batch finish");
 	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
@@ -406,12 +413,10 @@ std::vector<uintptr_t> batch_graph_key(pars_scanner* sc, const uint8_t* d_in, ui
 								(uintptr_t)sc->want_bitmap, (uintptr_t)sc->path, (uintptr_t)sc->rows,
 								(uintptr_t)sc->lb_window, (uintptr_t)sc->blocks_per_cu, (uintptr_t)sc->radix_bits,
 								(uintptr_t)sc->chase,
-								(uintptr_t)sc->cap_struct, (uintptr_t)sc->cap_br, (uintptr_t)sc->h_doc_off,
-								(uintptr_t)sc->h_doc_len};
+								(uintptr_t)sc->cap_struct, (uintptr_t)sc->cap_br, (uintptr_t)sc->h_doctab};
 	for (DevBuf* b : {&sc->recs, &sc->ticket, &sc->summary, &sc->positions, &sc->chars, &sc->bitmap, &sc->pair_pos,
 					  &sc->bpos, &sc->bsidx, &sc->bkey, &sc->bchar, &sc->ukeys0, &sc->ukeys1, &sc->order0,
-					  &sc->order1, &sc->rank, &sc->pair, &sc->hist, &sc->hist_part, &sc->doc_off, &sc->doc_len,
-					  &sc->tile_doc, &sc->head_bits, &sc->st_start, &sc->br_start, &sc->doc_final, &sc->doc_min,
+					  &sc->order1, &sc->rank, &sc->pair, &sc->hist, &sc->hist_part, &sc->doc_tab, &sc->tile_doc, &sc->head_bits, &sc->st_start, &sc->br_start, &sc->doc_final, &sc->doc_min,
 					  &sc->doc_max, &sc->doc_last, &sc->bdoc, &sc->doc_bad, &sc->doc_flags})
 		k.push_back((uintptr_t)b->p);
 	return k;
@@ -469,13 +474,16 @@ int batch_reserve(pars_scanner* sc, uint64_t max_bytes, uint64_t max_docs) {
 
 int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint64_t* offsets,
 				  const uint32_t* lengths, uint64_t n_docs, hipStream_t s, bool run_pairs, BatchPtrs* out) {
-	if (!n_docs || n > sc->cap_struct || n > sc->cap_br || n_docs * 8 > sc->h_off_cap)
+	if (!n_docs || n > sc->cap_struct || n > sc->cap_br || n_docs * 12 > sc->h_tab_cap
+		|| batch_pack_words(n_docs) * 4 > sc->doc_pack.cap)
 		return fail(PARS_ERR_ARGUMENT, "batch exceeds the reserved capacity");
-	std::memcpy(sc->h_doc_off, offsets, n_docs * 8);
-	std::memcpy(sc->h_doc_len, lengths, n_docs * 4);
+	std::memcpy(tab_off(sc), offsets, n_docs * 8);
+	std::memcpy(tab_len(sc, n_docs), lengths, n_docs * 4);
 	int r;
-	if ((r = enqueue_batch(sc, d_in, n, n_docs, s, false, run_pairs, false)))
+	if ((r = enqueue_batch(sc, d_in, n, n_docs, s, false, run_pairs, false, true)))
 		return r;
+	out->pack		   = sc->doc_pack.as<uint32_t>();
+	out->pack_bytes	   = batch_pack_words(n_docs) * 4;
 	out->structural	   = sc->positions.as<uint32_t>();
 	out->st_start	   = sc->st_start.as<uint32_t>();
 	out->bracket_pos   = sc->bpos.as<uint32_t>();
@@ -490,8 +498,12 @@ int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint6
 	return PARS_OK;
 }
 
+bool batch_failed_word(uint32_t overflow) {
+	return (overflow & 0x8000000Du) != 0;
+}
+
 bool batch_failed(const void* h_summary) {
-	return (static_cast<const DevSummary*>(h_summary)->overflow & 0x8000000Du) != 0;
+	return batch_failed_word(static_cast<const DevSummary*>(h_summary)->overflow);
 }
 
 }  // namespace pars_hip
@@ -706,8 +718,8 @@ int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const p
 	if ((r = ensure_batch(sc, n, n_docs, cap_br, want_pairs != 0)))
 		return r;
 	for (size_t d = 0; d < n_docs; ++d) {
-		sc->h_doc_off[d] = docs[d].offset;
-		sc->h_doc_len[d] = (uint32_t)docs[d].size;
+		tab_off(sc)[d]			= docs[d].offset;
+		tab_len(sc, n_docs)[d] = (uint32_t)docs[d].size;
 	}
 	const bool graph = sc->use_graph && s != nullptr;
 	for (int attempt = 0; attempt < 2; ++attempt) {

@@ -447,9 +447,11 @@ hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
 	if (a.n_tiles == 0)
 		return hipSuccess;
 	if constexpr (BATCH) {
-		const hipError_t e = launch_tile_doc(a.b, a.n_tiles, tbytes, s);
-		if (e != hipSuccess)
-			return e;
+		if (a.b.tile_rows != (uint32_t)W) {	 // the batch prep looked up another tile height
+			const hipError_t e = launch_tile_doc(a.b, a.n_tiles, tbytes, s);
+			if (e != hipSuccess)
+				return e;
+		}
 	}
 	const uint32_t grid = a.n_tiles < (uint32_t)resident ? a.n_tiles : (uint32_t)resident;
 	// a chaser pays off once tiles outnumber the resident blocks several times

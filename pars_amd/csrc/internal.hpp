@@ -22,6 +22,11 @@ struct BatchPtrs {
 	const void*		summary;	  // device summary (DevSummary)
 	void*			h_summary;	  // its pinned host copy (filled by the caller's D2H)
 	size_t			summary_bytes;
+	// the per-document results packed in one block for one copy (u32, n = n_docs):
+	// [0, n] structural starts, [n+1, 2n+1] bracket starts, [2n+2, 3n+1] max
+	// depths, [3n+2, 4n+1] flags, [4n+2] the summary's overflow word
+	const uint32_t* pack;
+	size_t			pack_bytes;
 };
 
 // Size every buffer for batches of up to max_bytes / max_docs documents with
@@ -37,5 +42,12 @@ int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint6
 // True when the summary copied to h_summary reports a failed look-back or an
 // output past its capacity.
 bool batch_failed(const void* h_summary);
+// The same from the overflow word of a packed result block.
+bool batch_failed_word(uint32_t overflow);
+
+// Bytes of the packed result block of a batch of n_docs documents.
+inline uint64_t batch_pack_bytes(uint64_t n_docs) {
+	return (4 * n_docs + 3) * 4;
+}
 
 }  // namespace pars_hip

@@ -44,7 +44,8 @@ struct DevSummary {
 	// (never reset itself), so records of earlier launches - also of earlier
 	// replays of one captured graph - read as unpublished.
 	uint32_t epoch;
-	uint32_t pad[2];
+	uint32_t small_done;  // R6 was completed by the one-block path (k_pair_small)
+	uint32_t pad;
 };
 
 // Batch mode: the input holds n_docs documents in ascending order, each
@@ -67,6 +68,8 @@ struct BatchArgs {
 	int32_t*		doc_max;	  // [n_docs] max depth after a bracket (starts at 0)
 	int32_t*		doc_last;	  // [n_docs] depth after the document
 	uint32_t*		bdoc;		  // [per bracket] document index
+	uint32_t		tile_rows;	  // tile_doc was filled for tiles of this many rows (0: not yet)
+	uint32_t*		pack;		  // optional: per-document results packed for one copy (k_batch_finish)
 };
 
 struct ScanArgs {
@@ -118,15 +121,28 @@ hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
 hipError_t launch_batch_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 hipError_t launch_batch_escape(const ScanArgs& a, int rows, hipStream_t s);
 hipError_t launch_batch_parity(const ScanArgs& a, int rows, hipStream_t s);
-// Batch preparation: clear and set the head bitmap, and reset the per-document
-// depth statistics.
-hipError_t launch_batch_heads(const BatchArgs& b, hipStream_t s);
+// Batch preparation in two launches: reset the summary (as launch_summary_reset
+// with a fresh document), clear the head bitmap, reset the per-document
+// statistics and doc_bad, fill tile_doc for tiles of `tile_rows` rows; then set
+// the head bits.
+hipError_t launch_batch_prep(const BatchArgs& b, uint64_t n, DevSummary* sum, uint32_t* doc_bad, uint32_t tile_rows,
+							 hipStream_t s);
+// Rows per tile the batch launchers will use for this plan and options (the
+// launch recomputes tile_doc itself if it differs).
+uint32_t batch_tile_rows(const DevPlan& p, int path, int rows);
 // tile_doc for tiles of `tile_bytes` (called by the batch launchers).
 hipError_t launch_tile_doc(const BatchArgs& b, uint32_t n_tiles, uint64_t tile_bytes, hipStream_t s);
 // Per-document flags once R6 has run: bit0 balanced, bit1 well_formed, bit2
-// unclosed_mode; start_packed = the plan's start states.
+// unclosed_mode; start_packed = the plan's start states.  With b.pack, also the
+// per-document results in one block of u32 (n = n_docs): [0, n] structural
+// starts, [n+1, 2n+1] bracket starts, [2n+2, 3n+1] max depths, [3n+2, 4n+1]
+// flags, [4n+2] the summary's overflow word.
 hipError_t launch_batch_finish(const BatchArgs& b, const uint32_t* doc_bad, uint32_t start_mask,
-							   uint32_t start_packed, int run_pairs, uint8_t* doc_flags, hipStream_t s);
+							   uint32_t start_packed, int run_pairs, uint8_t* doc_flags, const DevSummary* sum,
+							   hipStream_t s);
+inline uint64_t batch_pack_words(uint64_t n_docs) {
+	return 4 * n_docs + 3;	// batch_pack_bytes (internal.hpp) / 4
+}
 // Initial summary values: a fresh document starts all at 0; a chunk of a
 // longer document starts from the running values its carry holds.
 struct SummaryInit {
@@ -165,7 +181,22 @@ struct PairArgs {
 	uint32_t		n_docs;
 	uint32_t*		doc_bad;  // [n_docs] a sorted pair of the document does not match
 	uint32_t		max_digit_bits;	 // radix digit width cap per pass (1..8; 0 = 8)
+	// One-block R6 for at most kSmallBrackets brackets: 0 = not tried, 1 = tried
+	// first (the radix path then runs only if it did not apply), 2 = the only
+	// path (the caller guarantees it applies: see small_pairing_mode)
+	uint32_t		small;
 };
+
+constexpr uint32_t kSmallBrackets = 8192;
+// The R6 path for n input bytes in n_docs documents (1 for a single document):
+// the one-block path is certain when n bounds the bracket count to
+// kSmallBrackets and the (document, key) range to 19 bits, worth trying up to
+// 1 MiB, and skipped above.
+inline uint32_t small_pairing_mode(uint64_t n, uint64_t n_docs) {
+	if (n <= kSmallBrackets && n_docs * (n + 2) < (1ull << 19))
+		return 2;
+	return n <= (1ull << 20) ? 1u : 0u;
+}
 
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
 

@@ -39,7 +39,12 @@ __device__ __forceinline__ uint32_t n_brackets(const PairArgs& a) {
 __global__ void k_sort_prep(PairArgs a) {
 	if (threadIdx.x != 0 || blockIdx.x != 0)
 		return;
-	DevSummary*	   s	= a.sum;
+	DevSummary* s = a.sum;
+	if (s->small_done) {  // the one-block path paired everything: no pass runs
+		s->sort_passes = 0;
+		s->n_chunks	   = 0;
+		return;
+	}
 	const uint32_t nb	= n_brackets(a);
 	const int32_t  kmin = s->min_bal - 1;  // open keys are depth-1, depths >= min_bal
 	const int32_t  kmax = s->max_bal;
@@ -238,6 +243,8 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(PairArgs a) {
 // Walked in document order: bracket i sits at sorted position rank[i]; its
 // neighbour is at rank[i] ^ 1.
 __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
+	if (a.sum->small_done)
+		return;
 	const uint32_t	nb	  = n_brackets(a);
 	const uint32_t	pass  = a.sum->sort_passes;
 	const uint32_t* order = a.order[(pass - 1) & 1];
@@ -276,6 +283,94 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 		atomicOr(&a.sum->mismatched, 1u);
 }
 
+// R6 in one block for at most kSmallBrackets brackets whose (document, key)
+// range fits 19 bits: every bracket becomes (key << 13) | index, so sorting
+// these distinct values is the stable sort by key; a bitonic sort in LDS, then
+// sorted neighbours pair exactly as in k_pair.  Small batches (the serving
+// case) and small documents then cost one launch instead of the radix passes.
+// Sets small_done when it applied; otherwise leaves everything to them.
+constexpr int kSmallBlock = 1024;
+
+__global__ __launch_bounds__(kSmallBlock) void k_pair_small(PairArgs a) {
+	__shared__ uint32_t s_v[kSmallBrackets];
+	__shared__ uint32_t s_ok;
+	DevSummary*			sum = a.sum;
+	const uint32_t		nb	= n_brackets(a);
+	const int			tid = threadIdx.x;
+	const int32_t		kmin = sum->min_bal - 1, kmax = sum->max_bal;
+	const uint64_t		span = (uint64_t)(kmax - kmin) + 1u;
+	const uint64_t		range = a.bdoc ? (uint64_t)a.n_docs * span - 1u : (uint64_t)(kmax - kmin);
+	if (nb > kSmallBrackets || nb == 0 || range >= (1ull << 19))
+		return;	 // not this path (nb == 0: nothing to pair, flags need no sort)
+	uint32_t N = 1;
+	while (N < nb) N <<= 1;
+	for (uint32_t i = tid; i < N; i += kSmallBlock) {
+		uint32_t v = 0xFFFFFFFFu;  // padding sorts last
+		if (i < nb) {
+			const uint32_t k = (uint32_t)(a.bkey[i] - kmin);
+			v				 = ((a.bdoc ? a.bdoc[i] * (uint32_t)span + k : k) << 13) | i;
+		}
+		s_v[i] = v;
+	}
+	__syncthreads();
+	for (uint32_t k = 2; k <= N; k <<= 1)
+		for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+			for (uint32_t i = tid; i < N; i += kSmallBlock) {
+				const uint32_t l = i ^ j;
+				if (l > i) {
+					const uint32_t x = s_v[i], y = s_v[l];
+					if ((x > y) == ((i & k) == 0)) {
+						s_v[i] = y;
+						s_v[l] = x;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	const DevPlan* P   = a.plan;
+	bool		   bad = false;
+	for (uint32_t m = tid; m < nb; m += kSmallBlock) {
+		const uint32_t v = s_v[m] & 0x1FFFu;
+		uint32_t	   d = 0, b0 = 0, n = nb, s0 = 0;
+		if (a.bdoc) {
+			d  = a.bdoc[v];
+			b0 = a.br_start[d];
+			n  = a.br_start[d + 1] - b0;
+			s0 = a.st_start[d];
+		}
+		const uint32_t p = m - b0, q = p ^ 1u;
+		int32_t		   partner = -1;
+		if (q < n) {
+			const uint32_t j  = s_v[b0 + q] & 0x1FFFu;
+			const uint32_t ia = (p & 1u) ? j : v, ib = (p & 1u) ? v : j;
+			const uint8_t  ca = a.bchar[ia], cb = a.bchar[ib];
+			const bool ok = (P->br_flags[ca] & 1u) && (P->br_flags[cb] & 2u) && P->br_close_of[ca] == cb;
+			if (ok)
+				partner = (int32_t)j;
+			else if (!(p & 1u)) {
+				bad = true;
+				if (a.doc_bad)
+					a.doc_bad[d] = 1u;
+			}
+		}
+		if (a.pair)
+			a.pair[v] = partner >= 0 ? partner - (int32_t)b0 : -1;
+		if (a.pair_pos)
+			a.pair_pos[a.bsidx[v]] = partner >= 0 ? (int32_t)(a.bsidx[partner] - s0) : -1;
+	}
+	if (tid == 0)
+		s_ok = 0;
+	__syncthreads();
+	if (bad)
+		s_ok = 1;
+	__syncthreads();
+	if (tid == 0) {
+		if (s_ok)
+			atomicOr(&sum->mismatched, 1u);
+		sum->small_done = 1u;
+	}
+}
+
 template<int PASS>
 hipError_t launch_pass(const PairArgs& a, unsigned grid, hipStream_t s) {
 	hipLaunchKernelGGL((k_sort_hist<PASS>), dim3(grid), dim3(kSortBlock), 0, s, a);
@@ -291,6 +386,11 @@ hipError_t launch_pass(const PairArgs& a, unsigned grid, hipStream_t s) {
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
 	if (a.cap_br == 0)
 		return hipSuccess;
+	if (a.small) {
+		hipLaunchKernelGGL(k_pair_small, dim3(1), dim3(kSmallBlock), 0, s, a);
+		if (a.small == 2)
+			return hipGetLastError();
+	}
 	const uint64_t chunks = (a.cap_br + kChunk - 1) / kChunk;
 	const unsigned grid	  = (unsigned)std::min<uint64_t>(std::max<uint64_t>(chunks, 1), 2048);
 	hipLaunchKernelGGL(k_sort_prep, dim3(1), dim3(64), 0, s, a);

@@ -114,11 +114,8 @@ struct Slot {
 	uint32_t			  outstanding = 0;
 	uint64_t			  launch_ns	  = 0;
 	BatchPtrs			  ptrs {};
-	// pinned per-document results of the batch
-	uint32_t* h_st	  = nullptr;
-	uint32_t* h_br	  = nullptr;
-	uint8_t*  h_flags = nullptr;
-	int32_t*  h_maxd  = nullptr;
+	// pinned per-document results of the batch, packed (BatchPtrs::pack)
+	uint32_t* h_pack = nullptr;
 	std::vector<uint64_t> offs;
 	std::vector<uint32_t> lens;
 };
@@ -141,6 +138,8 @@ struct pars_server {
 	std::vector<PlanState>		  plans;
 	std::mutex					  mu;
 	std::condition_variable		  cv_done;
+	std::condition_variable		  cv_work;	// a request was submitted or released (wakes an idle worker)
+	std::atomic<uint64_t>		  work_seq {0};
 	std::unordered_map<uint64_t, Request> reqs;
 	std::deque<uint64_t>		  completed;
 	uint64_t					  next_id  = 1;
@@ -175,7 +174,11 @@ int shipfail(hipError_t e, const char* what) {
 }  // namespace
 
 pars_server::~pars_server() {
-	stop = true;
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		stop = true;
+	}
+	cv_work.notify_all();
 	if (worker.joinable())
 		worker.join();
 	for (PlanState& p : plans) {
@@ -188,9 +191,8 @@ pars_server::~pars_server() {
 				(void)hipEventDestroy(s.h2d);
 			if (s.done)
 				(void)hipEventDestroy(s.done);
-			for (void* h : {(void*)s.h_st, (void*)s.h_br, (void*)s.h_flags, (void*)s.h_maxd})
-				if (h)
-					(void)hipHostFree(h);
+			if (s.h_pack)
+				(void)hipHostFree(s.h_pack);
 		}
 		if (p.stage)
 			(void)hipHostFree(p.stage);
@@ -213,24 +215,27 @@ bool pars_server::retire() {
 			hipError_t e = hipEventQuery(s.done);
 			if (e == hipErrorNotReady)
 				break;
-			const uint64_t t	  = now_ns();
-			const bool	   failed = e != hipSuccess || batch_failed(s.ptrs.h_summary);
+			const uint64_t	t	   = now_ns();
+			const size_t	nd	   = s.reqs.size();
+			const uint32_t* pk	   = s.h_pack;	// packed: starts, bracket starts, depths, flags, overflow
+			const bool		failed = e != hipSuccess || batch_failed_word(pk[4 * nd + 2]);
 			std::lock_guard<std::mutex> lk(mu);
-			for (size_t d = 0; d < s.reqs.size(); ++d) {
+			for (size_t d = 0; d < nd; ++d) {
 				Request&			 q = reqs[s.reqs[d]];
 				pars_request_result& r = q.res;
 				r.id				   = q.id;
 				r.status			   = failed ? PARS_ERR_DEVICE : PARS_OK;
 				r.plan				   = p;
 				r.n_bytes			   = q.n;
-				r.n_struct			   = s.h_st[d + 1] - s.h_st[d];
-				r.n_brackets		   = s.h_br[d + 1] - s.h_br[d];
-				r.structural		   = s.ptrs.structural + s.h_st[d];
-				r.bracket_pos		   = s.ptrs.bracket_pos + s.h_br[d];
-				r.bracket_char		   = s.ptrs.bracket_char + s.h_br[d];
-				r.bracket_pair		   = s.ptrs.bracket_pair ? s.ptrs.bracket_pair + s.h_br[d] : nullptr;
-				r.flags				   = s.h_flags[d];
-				r.max_depth			   = s.h_maxd[d];
+				const uint32_t		 st0 = pk[d], br0 = pk[nd + 1 + d];
+				r.n_struct				 = pk[d + 1] - st0;
+				r.n_brackets			 = pk[nd + 2 + d] - br0;
+				r.structural			 = s.ptrs.structural + st0;
+				r.bracket_pos			 = s.ptrs.bracket_pos + br0;
+				r.bracket_char			 = s.ptrs.bracket_char + br0;
+				r.bracket_pair			 = s.ptrs.bracket_pair ? s.ptrs.bracket_pair + br0 : nullptr;
+				r.flags					 = pk[3 * nd + 2 + d];
+				r.max_depth				 = (int32_t)pk[2 * nd + 2 + d];
 				r.t_submit_ns		   = q.t_submit;
 				r.t_launch_ns		   = q.t_launch;
 				r.t_done_ns			   = t;
@@ -328,17 +333,8 @@ bool pars_server::launch(uint32_t p) {
 		r = batch_enqueue(s.sc, d_in, bytes, s.offs.data(), s.lens.data(), take.size(), ps.compute,
 						  cfg.run_pairs != 0, &s.ptrs);
 	const size_t nd = take.size();
-	if (r == PARS_OK) {
-		e = hipMemcpyAsync(s.h_st, s.ptrs.st_start, (nd + 1) * 4, hipMemcpyDeviceToHost, ps.compute);
-		if (e == hipSuccess)
-			e = hipMemcpyAsync(s.h_br, s.ptrs.br_start, (nd + 1) * 4, hipMemcpyDeviceToHost, ps.compute);
-		if (e == hipSuccess)
-			e = hipMemcpyAsync(s.h_flags, s.ptrs.doc_flags, nd, hipMemcpyDeviceToHost, ps.compute);
-		if (e == hipSuccess)
-			e = hipMemcpyAsync(s.h_maxd, s.ptrs.max_depth, nd * 4, hipMemcpyDeviceToHost, ps.compute);
-		if (e == hipSuccess)
-			e = hipMemcpyAsync(s.ptrs.h_summary, s.ptrs.summary, s.ptrs.summary_bytes, hipMemcpyDeviceToHost,
-							   ps.compute);
+	if (r == PARS_OK) {	 // one copy brings every per-request result back
+		e = hipMemcpyAsync(s.h_pack, s.ptrs.pack, s.ptrs.pack_bytes, hipMemcpyDeviceToHost, ps.compute);
 		if (e != hipSuccess)
 			r = PARS_ERR_DEVICE;
 	}
@@ -371,13 +367,21 @@ bool pars_server::launch(uint32_t p) {
 void pars_server::run() {
 	(void)hipSetDevice(dev);
 	while (!stop) {
-		bool busy = retire();
+		const uint64_t seen = work_seq.load();
+		bool		   busy = retire();
 		for (uint32_t p = 0; p < plans.size(); ++p)
 			while (launch(p)) busy = true;
 		bool running = false;
 		for (const PlanState& ps : plans) running = running || !ps.running.empty();
-		if (!busy)
-			std::this_thread::sleep_for(std::chrono::microseconds(running ? 5 : 50));
+		if (busy)
+			continue;
+		if (running) {	// batches on the device: poll their events
+			std::this_thread::sleep_for(std::chrono::microseconds(5));
+			continue;
+		}
+		// idle: sleep until a submit or a release changes what can launch
+		std::unique_lock<std::mutex> lk(mu);
+		cv_work.wait_for(lk, std::chrono::microseconds(200), [this, seen] { return stop || work_seq.load() != seen; });
 	}
 }
 
@@ -441,13 +445,7 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 			if (e == hipSuccess)
 				e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
 			if (e == hipSuccess)
-				e = hipHostMalloc((void**)&s.h_st, (nd + 1) * 4, hipHostMallocDefault);
-			if (e == hipSuccess)
-				e = hipHostMalloc((void**)&s.h_br, (nd + 1) * 4, hipHostMallocDefault);
-			if (e == hipSuccess)
-				e = hipHostMalloc((void**)&s.h_flags, nd, hipHostMallocDefault);
-			if (e == hipSuccess)
-				e = hipHostMalloc((void**)&s.h_maxd, nd * 4, hipHostMallocDefault);
+				e = hipHostMalloc((void**)&s.h_pack, batch_pack_bytes(nd), hipHostMallocDefault);
 			if (e != hipSuccess) {
 				r = shipfail(e, "server slot");
 				break;
@@ -500,6 +498,8 @@ int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, 
 	ps.pending.push_back(q.id);
 	sv->st.submitted++;
 	*id = q.id;
+	sv->work_seq++;
+	sv->cv_work.notify_one();
 	return PARS_OK;
 }
 
@@ -538,6 +538,8 @@ int pars_hip_server_release(pars_server* sv, uint64_t id) {
 	sv->reserved -= std::min(sv->reserved, q.reserve);
 	sv->st.released++;
 	sv->reqs.erase(it);
+	sv->work_seq++;
+	sv->cv_work.notify_one();
 	return PARS_OK;
 }
 

@@ -177,6 +177,32 @@ def bench_batch(torch, n_docs: int, iters: int, plain_sample: int, tune: bool = 
     }
 
 
+def bench_stream(sc, doc, stream: int, reps: int = 3) -> dict:
+    """The host boundary: the same document handed over in host memory and
+    streamed through pinned staging (64 MiB chunks, 8 copy threads), uploads and
+    scans overlapped; results on the device as in the main measurement."""
+    import numpy as np
+
+    from pars_amd import _native as N
+
+    st = N.StageTimes()
+    walls = []
+    for i in range(reps + 1):
+        t0 = time.perf_counter()
+        r = sc.scan_stream(doc, chunk_bytes=64 << 20, copy_threads=8, stream=stream, stage_times=st)
+        if i:
+            walls.append(time.perf_counter() - t0)
+    w = float(np.median(walls))
+    return {
+        "boundary": "host bytes -> pinned chunks -> PCIe -> scan (pars_hip_scan_stream)",
+        "chunk_bytes": 64 << 20,
+        "copy_threads": 8,
+        "ms": round(w * 1e3, 3),
+        "gb_per_s": round(doc.size / w / 1e9, 3),
+        "n_struct": int(r.n_struct),
+    }
+
+
 def bench_serve(n_req: int, seed: int = 5) -> dict:
     """Serving: Poisson arrivals of JSON / CSV / C-like requests (60/25/15,
     log-normal sizes, median 4 KiB, sigma 1.85, clipped to [200 B, 4 MiB])
@@ -256,6 +282,7 @@ def main() -> int:
     ap.add_argument("--only-serve", action="store_true", help="run only the serving measurement")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs (no autotuner)")
+    ap.add_argument("--no-stream", action="store_true", help="skip the host-boundary (streaming) measurement")
     ap.add_argument("--knobs", default="",
                     help="fixed launch knobs for the document scan instead of tuning, e.g. lb_window=16,rows=4 "
                          "(rows, lb_window, blocks_per_cu, radix_bits); profiles use this")
@@ -393,6 +420,10 @@ def main() -> int:
     }
     if cpu is not None:
         line["cpu_baseline"] = cpu
+    if not args.no_stream:
+        line["host_stream"] = bench_stream(sc, doc, sp)
+        assert line["host_stream"]["n_struct"] == k, "streamed scan disagrees with the device-resident one"
+        log(f"host stream: {line['host_stream']}")
     sc.close()
     del d_doc
     torch.cuda.empty_cache()

@@ -174,6 +174,19 @@ int pars_hip_scan_chunk(
 	size_t byte_base, pars_chunk_carry* out
 );
 
+/* Streaming host boundary (the reference's scan_host, pipeline.cuh:1176-1212,
+ * copies and uploads the whole input before scanning): the host bytes are
+ * staged chunk by chunk (chunk_bytes, default 64 MiB, rounded to 64 KiB) into
+ * two pinned buffers by copy_threads threads (default 8), uploaded on a copy
+ * stream, and scanned on `stream` with the state carried from chunk to chunk
+ * on the device, so host copies, PCIe transfers and kernels overlap; R6 then
+ * pairs the whole document.  Results as pars_hip_scan_device (device memory,
+ * whole-document positions); st->p1 = first chunk to last scan, p4 = R6. */
+int pars_hip_scan_stream(
+	pars_scanner* sc, const uint8_t* h_in, size_t n, size_t chunk_bytes, int copy_threads, pars_stream stream,
+	int want_pairs, pars_stage_times* st, pars_device_result* out
+);
+
 /* R6 over everything the chunks of one document emitted (the reference has
  * no pairing across chunks, docs/model/model.md:355-359). */
 int pars_hip_pair_chunks(

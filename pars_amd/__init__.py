@@ -239,6 +239,20 @@ class Scanner:
                                             int(want_pairs), C.byref(r)))
         return HostScan(r)
 
+    def scan_stream(self, data, chunk_bytes: int = 0, copy_threads: int = 0, stream: int | None = None,
+                    want_pairs: bool = False, stage_times: N.StageTimes | None = None) -> DeviceScan:
+        """Host bytes streamed through pinned staging in chunks, scanned as they
+        arrive (pars_hip_scan_stream); results on the device as scan_device."""
+        import numpy as np
+
+        b = data if isinstance(data, np.ndarray) else np.frombuffer(bytes(data), dtype=np.uint8)
+        b = np.ascontiguousarray(b, dtype=np.uint8)
+        r = N.DeviceResult()
+        _check(self._lib.pars_hip_scan_stream(
+            self._h, C.c_void_p(b.ctypes.data), b.size, chunk_bytes, copy_threads, C.c_void_p(stream or 0),
+            int(want_pairs), C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
+        return DeviceScan(r, b.size)
+
     def reserve_struct(self, cap: int) -> None:
         _check(self._lib.pars_hip_reserve_struct(self._h, cap))
 

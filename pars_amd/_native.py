@@ -181,6 +181,11 @@ PROTOTYPES = {
     "pars_hip_scan_host": (
         C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_int, C.POINTER(HostResult)]
     ),
+    "pars_hip_scan_stream": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int, C.c_void_p, C.c_int, C.POINTER(StageTimes),
+         C.POINTER(DeviceResult)],
+    ),
     "pars_hip_reserve_struct": (C.c_int, [C.c_void_p, C.c_size_t]),
     "pars_hip_scan_chunk": (
         C.c_int,

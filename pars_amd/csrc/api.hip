@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -113,6 +114,13 @@ struct pars_scanner {
 	uint8_t*  h_doctab	 = nullptr;
 	size_t	  h_tab_cap	 = 0;
 	int		  use_graph	 = 0;
+	// streaming host boundary: two pinned staging buffers, two device chunks,
+	// a copy stream, their events, and the chained carry
+	uint8_t*	  h_chunk[2]  = {nullptr, nullptr};
+	size_t		  h_chunk_cap = 0;
+	DevBuf		  d_chunk[2], carry;
+	hipStream_t	  copy_stream = nullptr;
+	hipEvent_t	  ev_h2d[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
 	hipGraphExec_t			gexec = nullptr;
 	std::vector<uintptr_t>	gkey;  // shape and buffers the captured graph was made for
 
@@ -126,6 +134,18 @@ struct pars_scanner {
 			b->release();
 		if (h_doctab)
 			(void)hipHostFree(h_doctab);
+		for (int i = 0; i < 2; ++i) {
+			if (h_chunk[i])
+				(void)hipHostFree(h_chunk[i]);
+			d_chunk[i].release();
+			if (ev_h2d[i])
+				(void)hipEventDestroy(ev_h2d[i]);
+			if (ev_done[i])
+				(void)hipEventDestroy(ev_done[i]);
+		}
+		carry.release();
+		if (copy_stream)
+			(void)hipStreamDestroy(copy_stream);
 		if (h_sum)
 			(void)hipHostFree(h_sum);
 		if (h_in)
@@ -977,6 +997,135 @@ int pars_hip_autotune(pars_scanner* sc, const uint8_t* d_in, size_t n, const par
 	out->blocks_per_cu	= sc->blocks_per_cu;
 	out->radix_bits		= sc->radix_bits;
 	return r;
+}
+
+
+// Streaming host boundary.  Chunk k's bytes go host -> pinned staging (k % 2)
+// by `copy_threads` threads, then up to device chunk (k % 2) on the copy
+// stream, and are scanned on the call's stream with the carry chained on the
+// device; the host only waits for staging buffer k % 2 to be read (chunk
+// k - 2's upload), so host copies, PCIe transfers and kernels overlap.  R6
+// runs once over the whole document at the end.
+int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t chunk_bytes, int copy_threads,
+						 pars_stream stream, int want_pairs, pars_stage_times* st, pars_device_result* out) {
+	if (!sc || !out || (n && !h_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if ((uint64_t)n > 0xFFFFFFFFull)
+		return fail(PARS_ERR_ARGUMENT, "documents are limited to 2^32-1 bytes (uint32 positions)");
+	if (st)
+		*st = pars_stage_times {0, 0, 0, 0};
+	if (n == 0) {
+		std::memset(out, 0, sizeof *out);
+		out->balanced = out->well_formed = 1;
+		return PARS_OK;
+	}
+	hipStream_t	   s	 = (hipStream_t)stream;
+	const uint64_t unit	 = 64 << 10;  // chunks start on lane and bitmap-word boundaries
+	uint64_t	   chunk = chunk_bytes ? chunk_bytes : (64ull << 20);
+	chunk				 = std::max<uint64_t>(unit, (chunk + unit - 1) / unit * unit);
+	chunk				 = std::min<uint64_t>(chunk, (n + unit - 1) / unit * unit);
+	const int	   thr	 = std::min(std::max(copy_threads ? copy_threads : 8, 1), 16);
+	const uint64_t n_chunks = (n + chunk - 1) / chunk;
+	int			   r;
+	hipError_t	   e;
+	if (!sc->copy_stream && (e = hipStreamCreateWithFlags(&sc->copy_stream, hipStreamNonBlocking)) != hipSuccess)
+		return hip_fail(e, "copy stream");
+	for (int i = 0; i < 2; ++i) {
+		if (!sc->ev_h2d[i] && (e = hipEventCreateWithFlags(&sc->ev_h2d[i], hipEventDisableTiming)) != hipSuccess)
+			return hip_fail(e, "event");
+		if (!sc->ev_done[i] && (e = hipEventCreateWithFlags(&sc->ev_done[i], hipEventDisableTiming)) != hipSuccess)
+			return hip_fail(e, "event");
+	}
+	if (chunk > sc->h_chunk_cap) {
+		for (int i = 0; i < 2; ++i) {
+			if (sc->h_chunk[i])
+				(void)hipHostFree(sc->h_chunk[i]);
+			sc->h_chunk[i] = nullptr;
+		}
+		sc->h_chunk_cap = 0;
+		for (int i = 0; i < 2; ++i)
+			if ((e = hipHostMalloc((void**)&sc->h_chunk[i], chunk, hipHostMallocDefault)) != hipSuccess)
+				return fail(PARS_ERR_MEMORY, "hipHostMalloc(%llu): %s", (unsigned long long)chunk, hipGetErrorString(e));
+		sc->h_chunk_cap = chunk;
+	}
+	if ((r = sc->d_chunk[0].ensure(chunk)) || (r = sc->d_chunk[1].ensure(chunk)) || (r = sc->carry.ensure(64)))
+		return r;
+	const uint64_t n_tiles = (chunk + kTile - 1) / kTile;
+	if (!sc->br_hint)
+		sc->br_hint = std::max<uint64_t>(n / 8, 1u << 16);
+	uint64_t cap_br = std::min<uint64_t>(std::max<uint64_t>(sc->br_hint, sc->cap_br), n);
+	if ((r = ensure_outputs(sc, n_tiles, std::max<uint64_t>(n, sc->cap_struct), cap_br, want_pairs != 0)))
+		return r;
+	if (sc->want_bitmap && (r = sc->bitmap.ensure(((n + 63) / 64) * 8)))
+		return r;
+	// host copy of one chunk into pinned staging, split over threads
+	auto stage = [&](uint8_t* dst, const uint8_t* src, uint64_t len) {
+		const uint64_t per = ((len + thr - 1) / thr + 4095) / 4096 * 4096;
+		if (thr == 1 || len <= per) {
+			std::memcpy(dst, src, len);
+			return;
+		}
+		std::vector<std::thread> ts;
+		for (uint64_t off = per; off < len; off += per)
+			ts.emplace_back([=] { std::memcpy(dst + off, src + off, std::min<uint64_t>(per, len - off)); });
+		std::memcpy(dst, src, std::min<uint64_t>(per, len));
+		for (std::thread& t : ts) t.join();
+	};
+	for (int attempt = 0; attempt < 2; ++attempt) {
+		if ((e = launch_summary_reset(sc->summary.as<DevSummary>(), SummaryInit {}, s)) != hipSuccess)
+			return hip_fail(e, "summary reset");
+		if (st && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+			return hip_fail(e, "hipEventRecord");
+		for (uint64_t k = 0; k < n_chunks; ++k) {
+			const int	   slot = (int)(k & 1);
+			const uint64_t off = k * chunk, len = std::min<uint64_t>(chunk, n - off);
+			if (k >= 2 && (e = hipEventSynchronize(sc->ev_h2d[slot])) != hipSuccess)  // staging slot read
+				return hip_fail(e, "staging wait");
+			stage(sc->h_chunk[slot], h_in + off, len);
+			if (k >= 2 && (e = hipStreamWaitEvent(sc->copy_stream, sc->ev_done[slot], 0)) != hipSuccess)
+				return hip_fail(e, "chunk wait");
+			if ((e = hipMemcpyAsync(sc->d_chunk[slot].p, sc->h_chunk[slot], len, hipMemcpyHostToDevice,
+									sc->copy_stream))
+					!= hipSuccess
+				|| (e = hipEventRecord(sc->ev_h2d[slot], sc->copy_stream)) != hipSuccess
+				|| (e = hipStreamWaitEvent(s, sc->ev_h2d[slot], 0)) != hipSuccess)
+				return hip_fail(e, "chunk upload");
+			if (k > 0 && (e = launch_carry_next(sc->summary.as<DevSummary>(), sc->carry.as<DevCarry>(), s)) != hipSuccess)
+				return hip_fail(e, "carry");
+			ScanArgs a	= make_scan_args(sc, sc->d_chunk[slot].as<uint8_t>(), len, want_pairs != 0);
+			a.byte_base = off;
+			a.bitmap	= sc->want_bitmap ? sc->bitmap.as<uint64_t>() + off / 64 : nullptr;
+			a.carry		= k > 0 ? sc->carry.as<DevCarry>() : nullptr;
+			if ((e = launch_scan(sc, a, s)) != hipSuccess)
+				return hip_fail(e, "scan kernel launch");
+			if ((e = hipEventRecord(sc->ev_done[slot], s)) != hipSuccess)
+				return hip_fail(e, "hipEventRecord");
+		}
+		if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+			return hip_fail(e, "hipEventRecord");
+		PairArgs p = make_pair_args(sc, want_pairs != 0);
+		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, 1);
+		if ((e = launch_pairing(p, s)) != hipSuccess)
+			return hip_fail(e, "pairing launch");
+		if (st && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+			return hip_fail(e, "hipEventRecord");
+		if ((r = read_summary(sc, s)))
+			return r;
+		if (sc->h_sum->n_brackets <= sc->cap_br)
+			break;
+		sc->br_hint = sc->h_sum->n_brackets;  // more brackets than the optimistic capacity: grow, stream again
+		if ((r = ensure_outputs(sc, n_tiles, sc->cap_struct, sc->h_sum->n_brackets, want_pairs != 0)))
+			return r;
+	}
+	if (st) {
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
+			st->p1 = ms;
+		if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
+			st->p4 = ms;
+	}
+	fill_result(sc, n, want_pairs != 0, true, out);
+	return PARS_OK;
 }
 
 }  // extern "C"

@@ -72,6 +72,15 @@ struct BatchArgs {
 	uint32_t*		pack;		  // optional: per-document results packed for one copy (k_batch_finish)
 };
 
+// The state a chunk of a longer document starts from, kept on the device when
+// chunks are chained without the host (pars_hip_scan_stream).
+struct DevCarry {
+	uint32_t states;	// packed per-descriptor states before the chunk
+	uint32_t out_base;	// structurals before the chunk
+	uint32_t br_base;	// brackets before the chunk
+	int32_t	 depth;		// depth before the chunk
+};
+
 struct ScanArgs {
 	const uint8_t*	data;
 	uint64_t		n;
@@ -103,6 +112,7 @@ struct ScanArgs {
 	uint32_t		lb_window;
 	uint32_t		blocks_per_cu;
 	uint32_t		chase;		 // set by the launcher: the last block chases aggregates (one chain)
+	const DevCarry* carry;		 // when set, in_states / out_base / br_base / depth_base come from here
 	BatchArgs		b;			 // batch mode only
 };
 
@@ -154,6 +164,10 @@ struct SummaryInit {
 };
 
 hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStream_t s);
+// Between two chained chunks: the next chunk's carry from the summary the last
+// one left (its running counts, depth extremes and error bits carry on), and
+// a new look-back epoch.
+hipError_t launch_carry_next(DevSummary* sum, DevCarry* carry, hipStream_t s);
 
 // R6: stable sort of bracket indices by depth key, then adjacent pairing.
 struct PairArgs {

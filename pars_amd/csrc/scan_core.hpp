@@ -572,6 +572,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	a.epoch		  = a.sum->epoch;  // set by the summary reset that precedes every launch
+	if (a.carry) {	// a chained chunk: where the previous one left off
+		a.in_states	 = a.carry->states;
+		a.out_base	 = a.carry->out_base;
+		a.br_base	 = a.carry->br_base;
+		a.depth_base = a.carry->depth;
+	}
 	E::load(s_lds, a.plan);
 	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
 	// bracket bytes from the walk's ordinal bits instead of the input

@@ -25,7 +25,23 @@ __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 	}
 }
 
+__global__ void k_carry_next(DevSummary* s, DevCarry* c) {
+	if (threadIdx.x == 0) {
+		c->states		 = s->final_states;
+		c->out_base		 = s->n_struct;
+		c->br_base		 = s->n_brackets;
+		c->depth		 = s->last_bal;
+		const uint32_t e = (s->epoch + 1u) & 0x3FFFFFFFu;
+		s->epoch		 = e ? e : 1u;
+	}
+}
+
 }  // namespace
+
+hipError_t launch_carry_next(DevSummary* sum, DevCarry* carry, hipStream_t s) {
+	hipLaunchKernelGGL(k_carry_next, dim3(1), dim3(64), 0, s, sum, carry);
+	return hipGetLastError();
+}
 
 namespace {
 thread_local int t_last_occupancy = 0;

@@ -24,6 +24,7 @@
 #include "grammars/xml.hpp"
 
 #include "grammars/clike6.hpp"
+#include "grammars/compiler_cases.hpp"
 
 #include <chrono>
 #include <cstdint>
@@ -45,6 +46,16 @@ constexpr auto kToml   = compile::make_scan_plan<bench::toml_grammar>();
 constexpr auto kIni	   = compile::make_scan_plan<bench::ini_grammar>();
 constexpr auto kXml	   = compile::make_scan_plan<bench::xml_grammar>();
 constexpr auto kClike6 = compile::make_scan_plan<pars_amd::grammars::clike6_grammar>();
+
+// the compiler probe grammars (grammars/compiler_cases.hpp)
+namespace cc	  = pars_amd::compiler_cases;
+constexpr auto kSqlCase	  = compile::make_scan_plan<cc::sql_case>();
+constexpr auto kEscCase	  = compile::make_scan_plan<cc::esc_case>();
+constexpr auto kManyCase  = compile::make_scan_plan<cc::many_case>();
+constexpr auto kCapsCase  = compile::make_scan_plan<cc::caps_case>();
+constexpr auto kTinyCase  = compile::make_scan_plan<cc::tiny_case>();
+constexpr auto kUnsupCase = compile::make_scan_plan<cc::unsup_case>();
+constexpr auto kPropCase  = compile::make_scan_plan<cc::prop_case>();
 
 static_assert(sizeof(scan_plan) == 3177, "plan layout changed");
 
@@ -153,6 +164,20 @@ int main(int argc, char** argv) {
 		return dispatch<kXml>(argc, argv);
 	if (g == "clike6")
 		return dispatch<kClike6>(argc, argv);
+	if (g == "sql_case")
+		return dispatch<kSqlCase>(argc, argv);
+	if (g == "esc_case")
+		return dispatch<kEscCase>(argc, argv);
+	if (g == "many_case")
+		return dispatch<kManyCase>(argc, argv);
+	if (g == "caps_case")
+		return dispatch<kCapsCase>(argc, argv);
+	if (g == "tiny_case")
+		return dispatch<kTinyCase>(argc, argv);
+	if (g == "unsup_case")
+		return dispatch<kUnsupCase>(argc, argv);
+	if (g == "prop_case")
+		return dispatch<kPropCase>(argc, argv);
 	std::fprintf(stderr, "unknown grammar %s\n", g.c_str());
 	return 2;
 }

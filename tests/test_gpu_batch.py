@@ -73,8 +73,8 @@ def run_batch(torch, sc: Scanner, docs, want_pairs=True, run_pairs=True, stream=
     return out
 
 
-def check_batch(g: str, docs, got, label=""):
-    plan = Plan.load(g).blob
+def check_batch(g: str, docs, got, label="", plan: bytes | None = None):
+    plan = plan if plan is not None else Plan.load(g).blob
     ss, bs = got["struct_start"], got["bracket_start"]
     assert ss[0] == 0 and bs[0] == 0, label
     assert ss[-1] == got["r"].n_struct and bs[-1] == got["r"].n_brackets, label

@@ -307,6 +307,38 @@ int pars_hip_autotune(
 /* pipeline.cuh:1225 scanner::struct_ptr. */
 const uint32_t* pars_hip_struct_ptr(const pars_scanner* sc);
 
+/* The JSON tape: one node per value the reference's JSON DOM hook
+ * (bench/bench_parse.cu:41-333, driven through model::replay, visit.hpp:35-48,
+ * by pars_parse, bench_parse.cu:336-348) pushes, in the same order with the
+ * same (kind, begin, end), plus the bytes each string decodes to.  All device
+ * memory, valid until the scanner's next call. */
+typedef struct {
+	const uint8_t*	kind;	   /* 0 object, 1 array, 2 string, 3 number, 4 true, 5 false, 6 null */
+	const uint32_t* begin;	   /* byte offset of the value's first byte (a container: its open) */
+	const uint32_t* end;	   /* one past its last byte; a container: its close's offset; 0 while unclosed */
+	const uint32_t* str_off;   /* decoded bytes so far: node k's string is arena[str_off[k], str_off[k+1]) */
+	const uint8_t*	arena;	   /* every string's decoded bytes, in node order */
+	const uint32_t* structural; /* the structural index the tape was built from */
+	uint64_t		n_nodes;
+	uint64_t		arena_bytes;
+	uint64_t		n_struct;
+	int64_t			fail_event; /* the first event the hook rejected, -1 if none: 2i = the run of
+								 * bytes before structural i, 2i+1 = structural i, 2 n_struct =
+								 * the run after the last one */
+	int32_t			ok;		   /* the hook's verdict after its final checks (bench_parse.cu:196-201) */
+	int32_t			pad;
+} pars_json_tape;
+
+/* Scan a JSON document on the device (R1-R6) and build its tape there: the
+ * reference's stage-2 JSON hook (bench_parse.cu:336-348: scan_host, replay into
+ * json_parser, finish) without the host replay.  The scanner's plan must be a
+ * JSON plan: punctuation within ',' ':', brackets within {} [] (else
+ * PARS_ERR_ARGUMENT).  Blocks until done; st->p1 = R1-R5, p4 = R6, p2 = the
+ * tape kernels (ms), when st is not NULL. */
+int pars_hip_json_tape(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, pars_stage_times* st, pars_json_tape* out
+);
+
 /* Message for the last failing call on this thread ("" if none). */
 const char* pars_hip_last_error(void);
 

@@ -24,6 +24,7 @@ __all__ = [
     "Plan",
     "Scanner",
     "DeviceScan",
+    "DeviceTape",
     "HostScan",
     "ChunkCarry",
     "scan_device",
@@ -122,6 +123,30 @@ class DeviceScan:
             f"DeviceScan(n_struct={self.n_struct}, n_brackets={self.n_brackets}, balanced={self.balanced}, "
             f"well_formed={self.well_formed}, max_depth={self.max_depth}, unclosed={self.unclosed_mode})"
         )
+
+
+class DeviceTape:
+    """The JSON tape (pars_hip_json_tape): device pointers into the scanner's
+    arena, valid until its next call; `fail_event` is -1 when the hook
+    accepted every event."""
+
+    def __init__(self, r: N.JsonTape, n: int):
+        self.n = n
+        self.kind = r.kind or 0
+        self.begin = r.begin or 0
+        self.end = r.end or 0
+        self.str_off = r.str_off or 0
+        self.arena = r.arena or 0
+        self.structural = r.structural or 0
+        self.n_nodes = r.n_nodes
+        self.arena_bytes = r.arena_bytes
+        self.n_struct = r.n_struct
+        self.fail_event = r.fail_event
+        self.ok = bool(r.ok)
+
+    def __repr__(self) -> str:
+        return (f"DeviceTape(n_nodes={self.n_nodes}, arena_bytes={self.arena_bytes}, ok={self.ok}, "
+                f"fail_event={self.fail_event})")
 
 
 class HostScan:
@@ -252,6 +277,17 @@ class Scanner:
             self._h, C.c_void_p(b.ctypes.data), b.size, chunk_bytes, copy_threads, C.c_void_p(stream or 0),
             int(want_pairs), C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
         return DeviceScan(r, b.size)
+
+    def json_tape(self, d_data: int, n: int, stream: int | None = None,
+                  stage_times: N.StageTimes | None = None) -> DeviceTape:
+        """R1-R6 and the JSON tape of one device-resident document
+        (pars_hip_json_tape): the reference's replay-driven JSON DOM hook
+        (bench/bench_parse.cu:336-348) built on the device."""
+        r = N.JsonTape()
+        _check(self._lib.pars_hip_json_tape(
+            self._h, C.c_void_p(d_data), n, C.c_void_p(stream or 0),
+            C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
+        return DeviceTape(r, n)
 
     def reserve_struct(self, cap: int) -> None:
         _check(self._lib.pars_hip_reserve_struct(self._h, cap))

@@ -54,6 +54,23 @@ class DeviceResult(C.Structure):
     ]
 
 
+class JsonTape(C.Structure):
+    _fields_ = [
+        ("kind", C.c_void_p),
+        ("begin", C.c_void_p),
+        ("end", C.c_void_p),
+        ("str_off", C.c_void_p),
+        ("arena", C.c_void_p),
+        ("structural", C.c_void_p),
+        ("n_nodes", C.c_uint64),
+        ("arena_bytes", C.c_uint64),
+        ("n_struct", C.c_uint64),
+        ("fail_event", C.c_int64),
+        ("ok", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
 class HostResult(C.Structure):
     _fields_ = [
         ("structural", C.POINTER(C.c_uint32)),
@@ -220,6 +237,9 @@ PROTOTYPES = {
          C.POINTER(TuneResult)],
     ),
     "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
+    "pars_hip_json_tape": (
+        C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(StageTimes), C.POINTER(JsonTape)]
+    ),
     "pars_hip_last_error": (C.c_char_p, []),
     "pars_hip_version": (C.c_char_p, []),
 }

@@ -22,7 +22,7 @@ OBJ = os.path.join(HERE, "_obj")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_SOURCES = ["scan_kernels.hip", "batch_kernels.hip", "pair_kernels.hip", "api.hip", "server.hip"]
+HIP_SOURCES = ["scan_kernels.hip", "batch_kernels.hip", "pair_kernels.hip", "dom_kernels.hip", "api.hip", "server.hip"]
 CXX_SOURCES = ["plan_dev.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -123,6 +123,12 @@ struct pars_scanner {
 	hipEvent_t	  ev_h2d[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr};
 	hipGraphExec_t			gexec = nullptr;
 	std::vector<uintptr_t>	gkey;  // shape and buffers the captured graph was made for
+	// the JSON tape: per-token and per-bracket scratch, the tape, its summary
+	int			force_radix = 0;  // R6 through the radix passes (their sorted order is kept)
+	DevBuf		dom_tinfo, dom_gbeg, dom_gend, dom_gdec, dom_nx, dom_ax, dom_bx, dom_top, dom_tblk, dom_hblk,
+		dom_parent, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
+	DomSummary* h_dom = nullptr;  // pinned
+	size_t		h_dom_cap = 0;
 
 	~pars_scanner() {
 		if (gexec)
@@ -130,8 +136,12 @@ struct pars_scanner {
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
 						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
-						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack})
+						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &dom_tinfo, &dom_gbeg, &dom_gend,
+						  &dom_gdec, &dom_nx, &dom_ax, &dom_bx, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_sum,
+						  &tape_kind, &tape_beg, &tape_end, &tape_soff, &tape_arena})
 			b->release();
+		if (h_dom)
+			(void)hipHostFree(h_dom);
 		if (h_doctab)
 			(void)hipHostFree(h_doctab);
 		for (int i = 0; i < 2; ++i) {
@@ -315,7 +325,8 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 		return hip_fail(e, "hipEventRecord");
 	if (run_pairs) {
 		PairArgs p = make_pair_args(sc, want_pairs);
-		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, 1);  // a set digit cap asks for the radix path
+		// a set digit cap, or a consumer of the sorted order, asks for the radix path
+		p.small = (sc->radix_bits || sc->force_radix) ? 0u : small_pairing_mode(n, 1);
 		if ((e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 	}
@@ -1125,6 +1136,118 @@ int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t
 			st->p4 = ms;
 	}
 	fill_result(sc, n, want_pairs != 0, true, out);
+	return PARS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The JSON tape.
+// ---------------------------------------------------------------------------
+int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, pars_stage_times* st,
+					   pars_json_tape* out) {
+	if (!sc || !out || (n && !d_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	for (unsigned i = 0; i < sc->plan.n_punctuation; ++i)
+		if (sc->plan.punctuation[i] != ',' && sc->plan.punctuation[i] != ':')
+			return fail(PARS_ERR_ARGUMENT, "the JSON tape needs a JSON plan (punctuation within ',' ':')");
+	for (unsigned i = 0; i < sc->plan.n_brackets; ++i) {
+		const char o = sc->plan.brackets[i].open, c = sc->plan.brackets[i].close;
+		if (!((o == '{' && c == '}') || (o == '[' && c == ']')))
+			return fail(PARS_ERR_ARGUMENT, "the JSON tape needs a JSON plan (brackets within {} [])");
+	}
+	std::memset(out, 0, sizeof *out);
+	out->fail_event = -1;
+	if (st)
+		*st = pars_stage_times {0, 0, 0, 0};
+	if (n == 0)
+		return PARS_OK;	 // no value: the hook's final check fails (ok = 0)
+	hipStream_t		   s = (hipStream_t)stream;
+	pars_device_result res;
+	sc->force_radix = 1;
+	int r			= pars_hip_scan_device(sc, d_in, n, stream, 0, 1, st, &res);
+	sc->force_radix = 0;
+	if (r)
+		return r;
+	const DevSummary& h = *sc->h_sum;
+	if (h.small_done || h.sort_passes < 1 || h.sort_passes > 4)
+		return fail(PARS_ERR_DEVICE, "the bracket sort did not run through the radix passes (internal error)");
+	const uint64_t ns = res.n_struct, nb = res.n_brackets, ntok = ns + 1;
+	const uint64_t n_tblk = (ntok + kDomTile - 1) / kDomTile, n_hblk = std::max<uint64_t>((nb + kDomTile - 1) / kDomTile, 1);
+	const uint64_t cap_nodes = std::min<uint64_t>(n, 2 * ns + 1);
+	for (DevBuf* b : {&sc->dom_tinfo, &sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec, &sc->dom_nx, &sc->dom_ax, &sc->dom_bx})
+		if ((r = b->ensure(ntok * 4)))
+			return r;
+	if ((r = sc->dom_top.ensure(ntok)) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
+		(r = sc->dom_hblk.ensure(n_hblk * 4)) || (r = sc->dom_parent.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
+		(r = sc->dom_sum.ensure(sizeof(DomSummary))) || (r = sc->tape_kind.ensure(cap_nodes)) ||
+		(r = sc->tape_beg.ensure(cap_nodes * 4)) || (r = sc->tape_end.ensure(cap_nodes * 4)) ||
+		(r = sc->tape_soff.ensure(cap_nodes * 4)) || (r = sc->tape_arena.ensure(n)))
+		return r;
+	if (!sc->h_dom && (r = ensure_pinned(reinterpret_cast<void**>(&sc->h_dom), sc->h_dom_cap, sizeof(DomSummary))))
+		return r;
+	DomArgs a {};
+	a.data		= d_in;
+	a.n			= n;
+	a.positions = sc->positions.as<uint32_t>();
+	a.n_struct	= ns;
+	a.nb		= nb;
+	a.plan		= sc->dplan.as<DevPlan>();
+	a.sum		= sc->summary.as<DevSummary>();
+	a.bchar		= sc->bchar.as<uint8_t>();
+	a.bkey		= sc->bkey.as<int32_t>();
+	a.bsidx		= sc->bsidx.as<uint32_t>();
+	a.rank		= sc->rank.as<uint32_t>();
+	a.order		= ((h.sort_passes - 1) & 1) ? sc->order1.as<uint32_t>() : sc->order0.as<uint32_t>();
+	a.tinfo		= sc->dom_tinfo.as<uint32_t>();
+	a.gbeg		= sc->dom_gbeg.as<uint32_t>();
+	a.gend		= sc->dom_gend.as<uint32_t>();
+	a.gdec		= sc->dom_gdec.as<uint32_t>();
+	a.nodes_x	= sc->dom_nx.as<uint32_t>();
+	a.arena_x	= sc->dom_ax.as<uint32_t>();
+	a.brk_x		= sc->dom_bx.as<uint32_t>();
+	a.topflag	= sc->dom_top.as<uint8_t>();
+	a.tblk		= sc->dom_tblk.as<uint32_t>();
+	a.n_tblk	= n_tblk;
+	a.hblk		= sc->dom_hblk.as<int32_t>();
+	a.n_hblk	= n_hblk;
+	a.parent	= sc->dom_parent.as<int32_t>();
+	a.ds		= sc->dom_sum.as<DomSummary>();
+	a.kind		= sc->tape_kind.as<uint8_t>();
+	a.nbeg		= sc->tape_beg.as<uint32_t>();
+	a.nend		= sc->tape_end.as<uint32_t>();
+	a.soff		= sc->tape_soff.as<uint32_t>();
+	a.arena		= sc->tape_arena.as<uint8_t>();
+	a.cap_nodes = cap_nodes;
+	a.cap_arena = n;
+	hipError_t e;
+	if (st && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = launch_json_tape(a, s)) != hipSuccess)
+		return hip_fail(e, "tape launch");
+	if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = hipMemcpyAsync(sc->h_dom, a.ds, sizeof(DomSummary), hipMemcpyDeviceToHost, s)) != hipSuccess)
+		return hip_fail(e, "tape summary readback");
+	if ((e = hipStreamSynchronize(s)) != hipSuccess)
+		return hip_fail(e, "hipStreamSynchronize");
+	const DomSummary& d = *sc->h_dom;
+	if (d.n_nodes > cap_nodes || d.arena_bytes > n)
+		return fail(PARS_ERR_DEVICE, "tape larger than its bound (internal error)");
+	if (st) {
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
+			st->p2 = ms;
+	}
+	out->kind		 = a.kind;
+	out->begin		 = a.nbeg;
+	out->end		 = a.nend;
+	out->str_off	 = a.soff;
+	out->arena		 = a.arena;
+	out->structural	 = a.positions;
+	out->n_nodes	 = d.n_nodes;
+	out->arena_bytes = d.arena_bytes;
+	out->n_struct	 = ns;
+	out->fail_event	 = d.fail_event == kDomNone ? -1 : (int64_t)d.fail_event;
+	out->ok			 = (int32_t)d.ok;
 	return PARS_OK;
 }
 

@@ -217,4 +217,61 @@ hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
 // Fill n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// The JSON tape (dom_kernels.hip): the reference's replay-driven JSON DOM hook
+// (bench/bench_parse.cu:41-348) computed from a document's structural index
+// and R6's depth-sorted brackets.  Events: the gap before structural i is 2i,
+// structural i is 2i+1, the gap after the last structural is 2 n_struct.
+constexpr uint64_t kDomNone = ~0ull;
+
+struct DomSummary {
+	unsigned long long fail_key;  // min over failing events of (event << 1 | partial node)
+	unsigned long long e1, e2;	  // the first and second top-level value events
+	unsigned long long fail_event;  // resolved: the event the hook stopped at, or kDomNone
+	uint64_t		   n_nodes, arena_bytes;
+	uint32_t		   ok, partial;
+};
+
+struct DomArgs {
+	const uint8_t*	data;
+	uint64_t		n;
+	const uint32_t* positions;
+	uint64_t		n_struct;  // tokens = n_struct + 1 (the last one has only a gap)
+	uint64_t		nb;		   // brackets
+	const DevPlan*	plan;
+	const DevSummary* sum;	   // R1-R6 summary of the same document (final depth)
+	const uint8_t*	bchar;
+	const int32_t*	bkey;
+	const uint32_t* bsidx;
+	const uint32_t* rank;	   // R6: sorted position of each bracket
+	const uint32_t* order;	   // R6: bracket at each sorted position
+	// per token
+	uint32_t* tinfo;
+	uint32_t* gbeg;
+	uint32_t* gend;
+	uint32_t* gdec;
+	uint32_t* nodes_x;	// exclusive prefixes: nodes, decoded bytes, brackets
+	uint32_t* arena_x;
+	uint32_t* brk_x;
+	uint8_t*  topflag;
+	uint32_t* tblk;	  // per token block: sums, then exclusive prefixes (3 each)
+	uint64_t  n_tblk;
+	// per sorted bracket position / per bracket
+	int32_t*  hblk;	  // per sorted block: last head, then the exclusive max
+	uint64_t  n_hblk;
+	int32_t*  parent;  // per bracket: the container that encloses its pair, or -1
+	DomSummary* ds;
+	// outputs
+	uint8_t*  kind;
+	uint32_t* nbeg;
+	uint32_t* nend;
+	uint32_t* soff;
+	uint8_t*  arena;
+	uint64_t  cap_nodes, cap_arena;
+};
+
+constexpr uint64_t kDomTile = 4096;	 // tokens or sorted brackets per block
+
+hipError_t launch_json_tape(const DomArgs& a, hipStream_t s);
+
 }  // namespace pars_hip

@@ -1,0 +1,133 @@
+"""GPU parity of the JSON tape (pars_hip_json_tape) with the reference's
+replay-driven JSON DOM hook as restated in oracle/json_dom.py: the verdict,
+the event the hook stopped at, every node (kind, begin, end), every decoded
+string and the hook's FNV-1a hash, bit-exact, on valid documents (generated,
+the reference's own bench payload, deep nesting, sizes across the kernels'
+4096-token blocks), on malformed ones (hand-made cases, random byte soup,
+corrupted documents) and on edge cases (empty, whitespace, one value).
+"""
+from __future__ import annotations
+
+import random
+
+import numpy as np
+import pytest
+
+from oracle import json_dom as J
+from oracle import oracle as O
+from pars_amd import Plan, Scanner
+from pars_amd import workloads as W
+from tests.gpu_util import d2h, say, to_device
+from tests.test_json_dom import KAT
+
+pytestmark = pytest.mark.gpu
+
+PLAN = Plan.load("json").blob
+_sc: dict = {}
+
+
+def scanner() -> Scanner:
+    if "json" not in _sc:
+        _sc["json"] = Scanner("json")
+    return _sc["json"]
+
+
+def gpu_tape(torch, d: bytes) -> dict:
+    arr = np.frombuffer(d, np.uint8)
+    t, ptr = to_device(torch, arr)
+    r = scanner().json_tape(ptr, arr.size, torch.cuda.current_stream().cuda_stream)
+    n = r.n_nodes
+    out = {"ok": r.ok, "fail_event": r.fail_event, "nodes": n,
+           "kind": d2h(r.kind, n, np.uint8), "begin": d2h(r.begin, n, np.uint32), "end": d2h(r.end, n, np.uint32),
+           "str_off": d2h(r.str_off, n, np.uint32), "arena": d2h(r.arena, r.arena_bytes, np.uint8).tobytes(),
+           "arena_bytes": r.arena_bytes}
+    del t
+    return out
+
+
+def check(torch, d: bytes, tag: str = ""):
+    exp = J.json_tape(d, O.scan(PLAN, np.frombuffer(d, np.uint8))["positions"])
+    got = gpu_tape(torch, d)
+    tag = tag or repr(d[:60])
+    assert (got["ok"], got["fail_event"], got["nodes"]) == (exp["ok"], exp["fail_event"], exp["nodes"]), tag
+    assert np.array_equal(got["kind"], exp["kind"]), tag
+    assert np.array_equal(got["begin"], exp["begin"]), tag
+    assert np.array_equal(got["end"], exp["end"]), tag
+    so = got["str_off"]
+    bounds = list(so) + [got["arena_bytes"]]
+    strings = [got["arena"][bounds[k]:bounds[k + 1]] for k in range(got["nodes"]) if got["kind"][k] == J.J_STR]
+    assert strings == exp["decoded"], tag
+    for k in range(got["nodes"]):  # non-strings decode to nothing
+        if got["kind"][k] != J.J_STR:
+            assert bounds[k + 1] == bounds[k], tag
+    assert J.tape_hash(d, got["kind"], got["begin"], got["end"], strings) == exp["hash"], tag
+    return got
+
+
+def test_known_answers(torch_cuda):
+    for d, ok, fe, nodes, h in KAT:
+        say(f"[gpu] tape KAT {d[:40]!r}")
+        got = check(torch_cuda, d)
+        assert (got["ok"], got["fail_event"], got["nodes"]) == (ok, fe, nodes)
+    check(torch_cuda, b'[{"a": [1, 2}]')
+
+
+@pytest.mark.parametrize("seed,n,depth,sf", [(1, 4096, 8, None), (2, 65536 + 13, 8, None),
+                                             (3, 1 << 20, 12, None), (4, 3 << 20, 32, 0.5)])
+def test_generated_documents(torch_cuda, seed, n, depth, sf):
+    d = W.json(seed, n, depth, sf).tobytes()
+    say(f"[gpu] tape json seed={seed} n={len(d)}")
+    got = check(torch_cuda, d)
+    assert got["ok"]
+
+
+def test_reference_bench_payload(torch_cuda):
+    rec = b'{"id":123,"name":"x \\"y\\"","a":[1,2,3],"n":{"z":null}}'  # bench_parse.cu:405-412
+    d = b"[" + b",".join([rec] * 40000) + b"]"
+    say(f"[gpu] tape bench payload n={len(d)}")
+    got = check(torch_cuda, d)
+    assert got["ok"] and got["nodes"] == 1 + 40000 * 14
+
+
+@pytest.mark.parametrize("depth", [1, 4095, 4097, 30000])
+def test_deep_nesting(torch_cuda, depth):
+    for d in (b"[" * depth + b"]" * depth, b"[" * depth + b"]" * (depth - 1),
+              b'{"k":' * depth + b"1" + b"}" * depth, b"[" * depth + b"}" + b"]" * depth):
+        say(f"[gpu] tape nesting depth={depth} n={len(d)}")
+        check(torch_cuda, d, f"depth {depth} {d[:8]!r}")
+
+
+def test_edge_cases(torch_cuda):
+    for d in (b"", b" ", b"1", b'"', b"[]", b"{}", b"  true  ", b"[" + b" " * 10000 + b"]",
+              b'"' + b"a" * 100000 + b'"', b"[" + b"1," * 20000 + b"1]"):
+        say(f"[gpu] tape edge n={len(d)}")
+        check(torch_cuda, d)
+
+
+def test_random_byte_soup(torch_cuda):
+    rng = random.Random(11)
+    parts = [b"{", b"}", b"[", b"]", b",", b":", b'"a"', b'"', b"1", b"-2.5e3", b"true", b"null", b" ", b"x",
+             b'\\"', b"01", b'"\\u00e9"', b"fals", b"\n"]
+    say("[gpu] tape random soup x400")
+    for _ in range(400):
+        check(torch_cuda, b"".join(rng.choice(parts) for _ in range(rng.randint(0, 40))))
+
+
+def test_corrupted_documents(torch_cuda):
+    rng = random.Random(12)
+    base = W.json(21, 200_000, 10).tobytes()
+    say("[gpu] tape corrupted x24")
+    for _ in range(24):
+        i = rng.randrange(len(base))
+        d = base[:i] + bytes([rng.choice(b'{}[],:"x 1\\')]) + base[i + 1:]
+        check(torch_cuda, d, f"corrupt at {i}")
+
+
+def test_repeated_calls_and_refusal(torch_cuda):
+    a, b = W.json(31, 300_000, 8).tobytes(), b'{"x": [1, 2, {"y": "z"}]}'
+    check(torch_cuda, a)
+    check(torch_cuda, b)  # a smaller document after a larger one
+    check(torch_cuda, a)
+    with pytest.raises(Exception):
+        t, ptr = to_device(torch_cuda, np.frombuffer(b, np.uint8))
+        Scanner("csv").json_tape(ptr, len(b))

@@ -82,12 +82,38 @@ struct Gap {
 	uint32_t kind, ok, beg, end, dec;
 };
 
+// The input read through a 16-byte register window: one vector load per
+// aligned 16 bytes, byte loads where a window would reach past the buffer.
+// A lane walks its gap forwards, so most bytes come from registers.
+struct Bytes {
+	const uint8_t* d;
+	uint64_t	   n;
+	uint64_t	   base;
+	uint4		   v;
+
+	__device__ Bytes(const uint8_t* d_, uint64_t n_) : d(d_), n(n_), base(~0ull), v(make_uint4(0, 0, 0, 0)) {}
+
+	__device__ uint32_t operator[](uint64_t i) {
+		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
+		const uint64_t a   = i - mis;
+		if (a != base) {
+			if (i < mis || a + 16u > n)
+				return d[i];
+			base = a;
+			v	 = *reinterpret_cast<const uint4*>(d + a);
+		}
+		const uint32_t k = (uint32_t)mis;
+		const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
+		return (w >> (8u * (k & 3u))) & 0xFFu;
+	}
+};
+
 // The value in [b, e) as the hook parses it (bench_parse.cu:86-130, 219-332):
 // kind, whether it parsed with only whitespace after it, its first byte, one
 // past its last byte (0 for a string that did not close), and the decoded
 // bytes a string mixed into the hash.  EMIT writes those bytes to `out`.
 template<bool EMIT>
-__device__ Gap parse_gap(const uint8_t* d, uint64_t b, uint64_t e, uint8_t* out) {
+__device__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
 	Gap		 g {G_EMPTY, 0u, 0u, 0u, 0u};
 	uint64_t i = b;
 	while (i < e && is_ws(d[i])) ++i;
@@ -348,7 +374,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 			break;
 		uint64_t b, e;
 		gap_bounds(a, t, b, e);
-		const Gap	   g   = parse_gap<false>(a.data, b, e, nullptr);
+		Bytes		   rd(a.data, a.n);
+		const Gap	   g   = parse_gap<false>(rd, b, e, nullptr);
 		const uint32_t sym = t < a.n_struct ? sym_class(a.data[a.positions[t]]) : SYM_END;
 		const uint32_t w   = sym | (g.kind << 3) | (g.ok << 6);
 		a.tinfo[t]		   = w;
@@ -638,7 +665,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 				if (gk == G_STR && (uint64_t)a.arena_x[t] + a.gdec[t] <= a.cap_arena) {
 					uint64_t b, e;
 					gap_bounds(a, t, b, e);
-					(void)parse_gap<true>(a.data, b, e, a.arena + a.arena_x[t]);
+					Bytes rd(a.data, a.n);
+					(void)parse_gap<true>(rd, b, e, a.arena + a.arena_x[t]);
 				}
 			}
 			++idx;

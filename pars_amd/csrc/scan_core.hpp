@@ -523,8 +523,16 @@ __device__ void chase_fused(uint64_t* recs, uint32_t n_tiles, uint32_t epoch, ui
 //   classes(S, OC, OP)      structural / bracket / open bits
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRowBytes = kBlock * kBPL;  // 16 KiB
-constexpr int	   kCapP	 = 4096;		   // staged structural outputs per row
-constexpr int	   kCapB	 = 1536;		   // staged bracket outputs per row
+// Outputs of one row staged in LDS before the coalesced write-back (a denser
+// row stores directly, lane by lane).  Single documents at three or four rows
+// per tile stage denser rows (small-record JSON reaches ~4900 structural and
+// ~1700 bracket bytes per 16 KiB row): the extra 7.5 KB still leaves room for
+// the three blocks per CU their registers allow.  The other shapes keep the
+// block count their registers allow with the smaller staging.
+template<int W, bool BATCH>
+constexpr int kCapP = (!BATCH && W >= 3) ? 6144 : 4096;  // staged structural outputs per row
+template<int W, bool BATCH>
+constexpr int kCapB = (!BATCH && W >= 3) ? 2048 : 1536;  // staged bracket outputs per row
 
 // Waves per SIMD the register allocation must leave room for.  Many-descriptor
 // plans with wide descriptors need most of the register file; they are held to
@@ -548,6 +556,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 	using Map			 = typename M::type;
 	using KeyT			 = typename std::conditional<BATCH, int32_t, int16_t>::type;
 	constexpr uint64_t kTileBytes = (uint64_t)W * kRowBytes;
+	constexpr int	   CAPP		  = kCapP<W, BATCH>;
+	constexpr int	   CAPB		  = kCapB<W, BATCH>;
 
 	__shared__ typename E::Lds s_lds;
 	__shared__ Map			   s_wagg[W][4][NDA];
@@ -557,14 +567,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 	__shared__ int32_t		   s_vpost[4];			   // one-chain batch: per-wave depth after the last head
 	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
 	__shared__ int32_t		   s_mm[4][2];
-	__shared__ uint16_t		   s_pos[kCapP];	// row-local byte offset
-	__shared__ uint16_t		   s_bpos[kCapB];	// row-local byte offset
-	__shared__ uint16_t		   s_bsidx[kCapB];	// row-local structural index
-	__shared__ KeyT			   s_bkey[kCapB];	// key: relative to the row's entry depth / document-relative
+	__shared__ uint16_t		   s_pos[CAPP];	 // row-local byte offset
+	__shared__ uint16_t		   s_bpos[CAPB];	 // row-local byte offset
+	__shared__ uint16_t		   s_bsidx[CAPB];	 // row-local structural index
+	__shared__ KeyT			   s_bkey[CAPB];	 // key: relative to the row's entry depth / document-relative
 	// bit-parallel evaluators: each lane's bracket ordinal bits per row (from the
 	// walk) and the staged bracket bytes rebuilt from them
 	__shared__ uint64_t		   s_bo[E::kBitParallel ? W : 1][E::kBitParallel ? kBlock : 1];
-	__shared__ uint8_t		   s_bch[E::kBitParallel ? kCapB : 1];
+	__shared__ uint8_t		   s_bch[E::kBitParallel ? CAPB : 1];
 	__shared__ uint64_t		   s_hw[BATCH ? NHW : 1];		// the tile's head bits
 	__shared__ uint32_t		   s_hpre[BATCH ? NHW : 1];		// heads in the words before
 	__shared__ int32_t		   s_dhead[BATCH ? NL : 1];		// tile-relative depth at each head, by slot
@@ -1045,7 +1055,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				}
 			}
 			int32_t		   rmin = INT_MAX, rmax = INT_MIN;	// this lane, this row
-			const bool	   staged = rtot[r][0] <= (uint32_t)kCapP && rtot[r][1] <= (uint32_t)kCapB;
+			const bool	   staged = rtot[r][0] <= (uint32_t)CAPP && rtot[r][1] <= (uint32_t)CAPB;
 			if (staged) {  // block-uniform
 				{  // both 32-bit halves of the mask at once: half the iterations
 					uint32_t mlo = (uint32_t)sm[r], mhi = (uint32_t)(sm[r] >> 32);

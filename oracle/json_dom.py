@@ -12,7 +12,8 @@ never by the product.
   JsonTape          bench/bench_parse.cu:41-333 — the JSON DOM hook the
                     reference drives through replay: a tape of
                     (kind, begin, end) nodes, every value decoded and folded
-                    into an FNV-1a 64 hash, `ok` cleared at the first event
+                    into an FNV-1a style hash (the hook's own seed, HOOK_SEED),
+                    `ok` cleared at the first event
                     the grammar rejects (nothing is consumed after it)
 
 Events are numbered as the device tape reports them: the run of bytes before
@@ -28,8 +29,12 @@ import numpy as np
 # node kinds, in the order of the reference's jkind (bench_parse.cu:33)
 J_OBJ, J_ARR, J_STR, J_NUM, J_TRUE, J_FALSE, J_NULL = range(7)
 
-FNV_OFFSET = 0xCBF29CE484222325
+FNV_OFFSET = 0xCBF29CE484222325  # the published FNV-1a 64 offset basis
 FNV_PRIME = 0x100000001B3
+# The hook seeds its hash with 1469598103934665603 (bench_parse.cu:57, 209), a
+# 19-digit constant rather than the 20-digit offset basis above; the mixing
+# step is FNV-1a's.
+HOOK_SEED = 1469598103934665603
 _M64 = (1 << 64) - 1
 _WS = b" \t\n\r"
 _ESC = {ord('"'): ord('"'), ord("\\"): ord("\\"), ord("/"): ord("/"), ord("b"): 8, ord("f"): 12,
@@ -104,7 +109,7 @@ class JsonTape:
         self.stack: list = []         # [open byte, phase]
         self.ok = True
         self.top_done = False
-        self.hash = FNV_OFFSET
+        self.hash = HOOK_SEED
         self.fail_event = -1
         self._ev = -1
 
@@ -344,7 +349,7 @@ def tape_hash(data, kind, begin, end, strings) -> int:
     'F' or '0' for the literals; containers mix nothing (bench_parse.cu:219-332).
     `strings` yields each string node's decoded bytes in node order."""
     base = bytes(np.asarray(data, dtype=np.uint8)) if not isinstance(data, (bytes, bytearray)) else bytes(data)
-    h = FNV_OFFSET
+    h = HOOK_SEED
     it = iter(strings)
     for k, b, e in zip(kind, begin, end):
         k = int(k)

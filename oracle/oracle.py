@@ -37,11 +37,40 @@ def lib():
         L.oracle_final_states.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
         L.oracle_fnv1a.restype = C.c_uint64
         L.oracle_fnv1a.argtypes = [C.c_uint64, C.c_void_p, C.c_size_t]
+        L.oracle_json_tape.restype = C.c_int
+        L.oracle_json_tape.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.POINTER(TapeInfo)]
         _lib = L
     return _lib
 
 
 FNV0 = 0xCBF29CE484222325
+
+
+class TapeInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_uint64), ("dec_bytes", C.c_uint64), ("hash", C.c_uint64),
+                ("fail_event", C.c_int64), ("ok", C.c_int32), ("pad", C.c_int32)]
+
+
+def json_tape(data, positions) -> dict:
+    """json_dom.c: the JSON hook (bench_parse.cu:41-348) replayed over a
+    structural index; the same dict as oracle/json_dom.py json_tape, with the
+    decoded strings as one buffer (`dec`) instead of a list."""
+    d = _u8(data)
+    pos = np.ascontiguousarray(positions, dtype=np.uint32)
+    cap = 2 * pos.size + 1
+    kind = np.zeros(cap, np.uint8)
+    beg = np.zeros(cap, np.uint32)
+    end = np.zeros(cap, np.uint32)
+    dec = np.zeros(max(d.size, 1), np.uint8)
+    info = TapeInfo()
+    rc = lib().oracle_json_tape(d.ctypes.data, d.size, pos.ctypes.data, pos.size, kind.ctypes.data, beg.ctypes.data,
+                                end.ctypes.data, cap, dec.ctypes.data, dec.size, C.byref(info))
+    if rc:
+        raise MemoryError("oracle_json_tape")
+    k = int(info.n_nodes)
+    return {"ok": bool(info.ok), "nodes": k, "hash": int(info.hash), "fail_event": int(info.fail_event),
+            "kind": kind[:k], "begin": beg[:k], "end": end[:k], "dec": dec[:int(info.dec_bytes)].tobytes()}
 
 
 def fnv(arr) -> int:

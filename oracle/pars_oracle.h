@@ -7,6 +7,8 @@
  *   oracle_classifier  include/pars/model/classify.hpp:39-55
  *   oracle_scan        include/pars/model/scan.hpp:27-73
  *   oracle_pair        include/pars/model/nest.hpp:45-92
+ *   oracle_json_tape   bench/bench_parse.cu:41-348 through visit.hpp:35-48
+ *                      (json_dom.c; the C twin of oracle/json_dom.py)
  *
  * Pinning (what checks each function against something other than itself):
  *   - golden vectors written by the reference compiled from its own sources
@@ -55,6 +57,22 @@ void oracle_final_states(
 /* 64-bit FNV-1a over a byte range, chained from `h` (start with
  * 0xcbf29ce484222325). */
 uint64_t oracle_fnv1a(uint64_t h, const void* p, size_t n);
+
+/* The JSON hook's results (json_dom.c). */
+typedef struct {
+	uint64_t n_nodes;	 /* nodes pushed (the arrays hold the first cap_nodes) */
+	uint64_t dec_bytes;	 /* decoded string bytes (dec holds the first cap_dec) */
+	uint64_t hash;		 /* the hook's FNV-1a 64 */
+	int64_t	 fail_event; /* 2i: the run before structural i; 2i+1: structural i; -1: none */
+	int32_t	 ok;
+	int32_t	 pad;
+} oracle_tape_info;
+
+/* bench_parse.cu:41-348: replay the structural index into the JSON hook.
+ * Returns 0, or -1 when its stack cannot be allocated. */
+int oracle_json_tape(const uint8_t* data, size_t n, const uint32_t* positions, size_t n_struct, uint8_t* kind,
+					 uint32_t* begin, uint32_t* end, size_t cap_nodes, uint8_t* dec, size_t cap_dec,
+					 oracle_tape_info* info);
 
 #ifdef __cplusplus
 }

@@ -93,7 +93,7 @@ struct Bytes {
 
 	__device__ Bytes(const uint8_t* d_, uint64_t n_) : d(d_), n(n_), base(~0ull), v(make_uint4(0, 0, 0, 0)) {}
 
-	__device__ uint32_t operator[](uint64_t i) {
+	__device__ __forceinline__ uint32_t operator[](uint64_t i) {
 		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
 		const uint64_t a   = i - mis;
 		if (a != base) {
@@ -106,6 +106,36 @@ struct Bytes {
 		const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
 		return (w >> (8u * (k & 3u))) & 0xFFu;
 	}
+
+	// Bytes of a string body that decode to themselves - not '"', '\\' or a
+	// control byte - starting at i, counted to the end of i's window or to e.
+	// Byte masks by the zero-byte test on 32-bit words: every real match is
+	// flagged; a false flag can only sit above a real one, so the first flag
+	// is never late (a false one just hands that byte to the caller's check).
+	__device__ __forceinline__ uint32_t plain_run(uint64_t i, uint64_t e) {
+		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
+		const uint64_t a   = i - mis;
+		if (i < mis || a + 16u > n)
+			return 0;
+		if (a != base) {
+			base = a;
+			v	 = *reinterpret_cast<const uint4*>(d + a);
+		}
+		auto special = [](uint32_t w) {
+			const uint32_t q = w ^ 0x22222222u, b = w ^ 0x5C5C5C5Cu;
+			const uint32_t m = ((q - 0x01010101u) & ~q) | ((b - 0x01010101u) & ~b) | ((w - 0x20202020u) & ~w);
+			const uint32_t t = m & 0x80808080u;
+			return ((t >> 7) & 1u) | ((t >> 14) & 2u) | ((t >> 21) & 4u) | ((t >> 28) & 8u);
+		};
+		uint32_t mask = special(v.x) | (special(v.y) << 4) | (special(v.z) << 8) | (special(v.w) << 12);
+		const uint32_t k = (uint32_t)mis;
+		mask &= ~((1u << k) - 1u);
+		uint32_t stop = mask ? (uint32_t)__builtin_ctz(mask) : 16u;
+		const uint64_t lim = e - a;	 // bytes of the window inside [a, e)
+		if (lim < stop)
+			stop = (uint32_t)lim;
+		return stop > k ? stop - k : 0u;
+	}
 };
 
 // The value in [b, e) as the hook parses it (bench_parse.cu:86-130, 219-332):
@@ -113,7 +143,7 @@ struct Bytes {
 // past its last byte (0 for a string that did not close), and the decoded
 // bytes a string mixed into the hash.  EMIT writes those bytes to `out`.
 template<bool EMIT>
-__device__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
+__device__ __forceinline__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
 	Gap		 g {G_EMPTY, 0u, 0u, 0u, 0u};
 	uint64_t i = b;
 	while (i < e && is_ws(d[i])) ++i;
@@ -133,6 +163,14 @@ __device__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
 		};
 		bool fail = false;
 		while (j < e && !fail && !parsed) {
+			const uint32_t run = d.plain_run(j, e);
+			if (run) {
+				if constexpr (EMIT)
+					for (uint32_t k = 0; k < run; ++k) out[dec + k] = (uint8_t)d[j + k];
+				dec += run;
+				j += run;
+				continue;
+			}
 			const uint32_t x = d[j++];
 			if (x == '"') {
 				g.end  = (uint32_t)j;
@@ -232,22 +270,18 @@ __device__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
 			parsed = true;
 		}
 	} else if (c == 't' || c == 'f' || c == 'n') {
-		auto word = [&](const char* w, uint32_t len) {
-			if (i + len > e)
-				return false;
-			for (uint32_t k = 0; k < len; ++k)
-				if (d[i + k] != (uint8_t)w[k])
-					return false;
-			return true;
-		};
+		// "true", "false", "null" (bench_parse.cu:308-332)
+		const uint64_t avail = e - i;
+		uint32_t	   w4	  = 0;	// the first four bytes, little endian
+		for (uint32_t k = 0; k < 4u && k < avail; ++k) w4 |= d[i + k] << (8u * k);
 		g.kind = G_BAD;
-		if (word("true", 4)) {
+		if (avail >= 4u && w4 == 0x65757274u) {	 // "true"
 			g.kind = G_TRUE;
 			g.end  = (uint32_t)(i + 4);
-		} else if (word("false", 5)) {
+		} else if (avail >= 5u && w4 == 0x736C6166u && d[i + 4] == 'e') {	// "fals" "e"
 			g.kind = G_FALSE;
 			g.end  = (uint32_t)(i + 5);
-		} else if (word("null", 4)) {
+		} else if (avail >= 4u && w4 == 0x6C6C756Eu) {  // "null"
 			g.kind = G_NULL;
 			g.end  = (uint32_t)(i + 4);
 		}
@@ -362,16 +396,14 @@ __global__ void k_dom_reset(DomSummary* ds) {
 	}
 }
 
-// 1. Per token: symbol class and the gap's value; block sums of the counts.
+// 1. Per token (one per thread): symbol class and the gap's value; the counts
+// summed per 4096-token group (the group sums start at zero).
 __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 	__shared__ uint32_t s[kDomBlock / 64][3];
 	const uint64_t		ntok = a.n_struct + 1;
-	const uint64_t		base = (uint64_t)blockIdx.x * kDomTile;
+	const uint64_t		t	 = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
 	uint32_t			sum[3] = {0u, 0u, 0u};
-	for (int r = 0; r < kDomRows; ++r) {
-		const uint64_t t = base + (uint64_t)r * kDomBlock + threadIdx.x;
-		if (t >= ntok)
-			break;
+	if (t < ntok) {
 		uint64_t b, e;
 		gap_bounds(a, t, b, e);
 		Bytes		   rd(a.data, a.n);
@@ -382,15 +414,18 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 		a.gbeg[t]		   = g.beg;
 		a.gend[t]		   = g.end;
 		a.gdec[t]		   = g.dec;
-		sum[0] += t_nodes(w);
-		sum[1] += g.dec;
-		sum[2] += t_bracket(w) ? 1u : 0u;
+		sum[0]			   = t_nodes(w);
+		sum[1]			   = g.dec;
+		sum[2]			   = t_bracket(w) ? 1u : 0u;
 	}
 	uint32_t tot[3];
 	block_excl3(sum, tot, s);
+	const uint64_t grp = ((uint64_t)blockIdx.x * kDomBlock) / kDomTile;
 	if (threadIdx.x == 0)
 #pragma unroll
-		for (int f = 0; f < 3; ++f) a.tblk[blockIdx.x * 3 + f] = tot[f];
+		for (int f = 0; f < 3; ++f)
+			if (tot[f])
+				atomicAdd(&a.tblk[grp * 3 + f], tot[f]);
 }
 
 // 2. Exclusive scan of the block sums (one block; each thread a contiguous run).
@@ -520,8 +555,8 @@ __device__ __forceinline__ void wave_atomic_min(unsigned long long* p, unsigned 
 // top_done), resolved by k_dom_top2.
 __global__ __launch_bounds__(kDomBlock) void k_dom_check(DomArgs a) {
 	const uint64_t ntok = a.n_struct + 1;
-	for (uint64_t t = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x; t - threadIdx.x < ntok;
-		 t += (uint64_t)gridDim.x * kDomBlock) {
+	{
+		const uint64_t	   t  = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
 		unsigned long long fk = kDomNone, top = kDomNone;
 		if (t < ntok) {
 			const uint32_t w = a.tinfo[t];
@@ -593,8 +628,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_check(DomArgs a) {
 __global__ __launch_bounds__(kDomBlock) void k_dom_top2(DomArgs a) {
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long e1	  = a.ds->e1;
-	for (uint64_t t = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x; t - threadIdx.x < ntok;
-		 t += (uint64_t)gridDim.x * kDomBlock) {
+	{
+		const uint64_t	   t = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
 		unsigned long long v = kDomNone;
 		if (t < ntok && e1 != kDomNone) {
 			const uint32_t f = a.topflag[t];
@@ -647,10 +682,11 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long fe	  = a.ds->fail_event;  // kDomNone: none
 	const uint32_t			 part = a.ds->partial;
-	for (uint64_t t = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x; t < ntok; t += (uint64_t)gridDim.x * kDomBlock) {
+	const uint64_t			 t	  = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
+	if (t < ntok) {
 		const unsigned long long eg = 2ull * t;
 		if (eg > fe)
-			continue;
+			return;
 		const uint32_t w  = a.tinfo[t];
 		const uint32_t gk = t_gap(w);
 		uint64_t	   idx = a.nodes_x[t];
@@ -694,16 +730,17 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	}
 }
 
-unsigned grid_for(uint64_t items, uint64_t per_block, uint64_t cap) {
-	return (unsigned)std::min<uint64_t>(std::max<uint64_t>((items + per_block - 1) / per_block, 1), cap);
-}
 
 }  // namespace
 
 hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 	const uint64_t ntok = a.n_struct + 1;
+	const unsigned gt	= (unsigned)((ntok + kDomBlock - 1) / kDomBlock);	// one token per thread
 	hipLaunchKernelGGL(k_dom_reset, dim3(1), dim3(64), 0, s, a.ds);
-	hipLaunchKernelGGL(k_dom_tok, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
+	hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.tblk), a.n_tblk * 3, 0, s);
+	if (e != hipSuccess)
+		return e;
+	hipLaunchKernelGGL(k_dom_tok, dim3(gt), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tok_top, dim3(1), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tok_apply, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
 	if (a.nb) {
@@ -711,11 +748,10 @@ hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 		hipLaunchKernelGGL(k_dom_heads_top, dim3(1), dim3(kDomBlock), 0, s, a);
 		hipLaunchKernelGGL(k_dom_parent, dim3((unsigned)a.n_hblk), dim3(kDomBlock), 0, s, a);
 	}
-	const unsigned g = grid_for(ntok, kDomBlock, 16384);
-	hipLaunchKernelGGL(k_dom_check, dim3(g), dim3(kDomBlock), 0, s, a);
-	hipLaunchKernelGGL(k_dom_top2, dim3(g), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_check, dim3(gt), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_top2, dim3(gt), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_final, dim3(1), dim3(64), 0, s, a);
-	hipLaunchKernelGGL(k_dom_tape, dim3(g), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_tape, dim3(gt), dim3(kDomBlock), 0, s, a);
 	return hipGetLastError();
 }
 

@@ -35,6 +35,9 @@ def run(d: bytes) -> dict:
 
 
 def test_fnv_vectors():
+    """The mixing step against the published FNV-1a 64 vectors; the hook's own
+    seed is its 19-digit constant (bench_parse.cu:57)."""
+    assert J.HOOK_SEED == 1469598103934665603 and J.HOOK_SEED != J.FNV_OFFSET
     assert J.fnv1a64(b"") == 0xCBF29CE484222325
     assert J.fnv1a64(b"a") == 0xAF63DC4C8601EC8C
     assert J.fnv1a64(b"foobar") == 0x85944171F73967E8
@@ -151,41 +154,41 @@ def test_hook_matches_json_module(i):
 
 
 def _h(*parts: bytes) -> int:
-    return J.fnv1a64(b"".join(parts))
+    return J.fnv1a64(b"".join(parts), J.HOOK_SEED)
 
 
 # (document, ok, fail_event, nodes, hash), worked out by hand
 KAT = [
-    (b"", False, -1, 0, J.FNV_OFFSET),
-    (b"   ", False, -1, 0, J.FNV_OFFSET),
+    (b"", False, -1, 0, J.HOOK_SEED),
+    (b"   ", False, -1, 0, J.HOOK_SEED),
     (b"[1,]", True, -1, 2, _h(b"N1")),                 # a trailing comma is accepted
     (b'{"a":1,}', True, -1, 3, _h(b"Sa\0", b"N1")),
     (b'{"a" 1}', False, 2, 2, _h(b"Sa\0")),            # key then a value in one gap
     (b"[1 2]", False, 2, 2, _h(b"N1")),
-    (b"{} {}", False, 5, 1, J.FNV_OFFSET),             # a second top-level value
+    (b"{} {}", False, 5, 1, J.HOOK_SEED),             # a second top-level value
     (b"1 2", False, 0, 1, _h(b"N1")),
-    (b"[}", False, 3, 1, J.FNV_OFFSET),
+    (b"[}", False, 3, 1, J.HOOK_SEED),
     (b'"abc', False, 0, 1, _h(b"Sabc")),               # unclosed: no terminating 0 mixed
     (b'{"a":1,"b"}', False, 7, 4, _h(b"Sa\0", b"N1", b"Sb\0")),
     (b"[1]]", False, 5, 2, _h(b"N1")),
-    (b"[", False, -1, 1, J.FNV_OFFSET),                # still open at the end
+    (b"[", False, -1, 1, J.HOOK_SEED),                # still open at the end
     (b'  "x"  ', True, -1, 1, _h(b"Sx\0")),
-    (b"tru", False, 0, 0, J.FNV_OFFSET),
+    (b"tru", False, 0, 0, J.HOOK_SEED),
     (b"[01]", False, 2, 2, _h(b"N0")),
     (b'{"a":"\\q"}', False, 4, 3, _h(b"Sa\0", b"S")),  # an unknown escape
-    (b"{1:2}", False, 2, 1, J.FNV_OFFSET),             # a key must be a string
-    (b"[:]", False, 3, 1, J.FNV_OFFSET),
+    (b"{1:2}", False, 2, 1, J.HOOK_SEED),             # a key must be a string
+    (b"[:]", False, 3, 1, J.HOOK_SEED),
     (b'{"a"}', False, 3, 2, _h(b"Sa\0")),
     (b'{"a":}', False, 5, 2, _h(b"Sa\0")),
-    (b"[,1]", False, 3, 1, J.FNV_OFFSET),
+    (b"[,1]", False, 3, 1, J.HOOK_SEED),
     (b'["\\u00e9"]', True, -1, 2, _h(b"S\xc3\xa9\0")),
     (b'"\\ud800"', True, -1, 1, _h(b"S\xed\xa0\x80\0")),  # each \u unit encoded alone
     (b'"a\x01"', False, 0, 1, _h(b"Sa")),              # a control byte
-    (b"{}}", False, 5, 1, J.FNV_OFFSET),
-    (b"nul", False, 0, 0, J.FNV_OFFSET),
-    (b"[-]", False, 2, 1, J.FNV_OFFSET),
-    (b"[1.]", False, 2, 1, J.FNV_OFFSET),
-    (b"[1e+]", False, 2, 1, J.FNV_OFFSET),
+    (b"{}}", False, 5, 1, J.HOOK_SEED),
+    (b"nul", False, 0, 0, J.HOOK_SEED),
+    (b"[-]", False, 2, 1, J.HOOK_SEED),
+    (b"[1.]", False, 2, 1, J.HOOK_SEED),
+    (b"[1e+]", False, 2, 1, J.HOOK_SEED),
     (b"[true false]", False, 2, 2, _h(b"T")),
 ]
 
@@ -217,3 +220,25 @@ def test_hook_random_documents_hash_from_tape():
         r = run(d)
         assert J.tape_hash(d, r["kind"], r["begin"], r["end"], r["decoded"]) == r["hash"]
         assert r["nodes"] == len(r["kind"])
+
+
+# ---------------------------------------------------------------------------
+def _same_as_python(d: bytes):
+    pos = positions(d)
+    a, b = run(d), O.json_tape(d, pos)
+    assert (a["ok"], a["nodes"], a["hash"], a["fail_event"]) == (b["ok"], b["nodes"], b["hash"], b["fail_event"]), d[:60]
+    assert np.array_equal(a["kind"], b["kind"]) and np.array_equal(a["begin"], b["begin"])
+    assert np.array_equal(a["end"], b["end"]) and b"".join(a["decoded"]) == b["dec"]
+
+
+def test_c_twin_matches_restatement():
+    """oracle/json_dom.c (the fast twin) against oracle/json_dom.py."""
+    for d, *_ in KAT:
+        _same_as_python(d)
+    for d in _valid_docs():
+        _same_as_python(d)
+    rng = random.Random(4)
+    parts = [b"{", b"}", b"[", b"]", b",", b":", b'"a"', b'"', b"1", b"-2.5e3", b"true", b"null", b" ", b"x",
+             b'\\"', b"01", b'"\\u00e9"', b"fals", b"\\u12"]
+    for _ in range(500):
+        _same_as_python(b"".join(rng.choice(parts) for _ in range(rng.randint(0, 30))))

@@ -203,6 +203,67 @@ def bench_stream(sc, doc, stream: int, reps: int = 3) -> dict:
     }
 
 
+def bench_tape(d_ptr: int, doc, stream: int, reps: int, cpu_sample: int, no_cpu: bool) -> dict:
+    """The stage-2 JSON hook (bench_parse.cu:336-348: scan, replay into the JSON
+    DOM builder) as one device call, pars_hip_json_tape: R1-R6 and the tape of
+    a device-resident document.  Measured on the 2 GiB document and on the
+    reference bench's own payload (its record repeated to 64 MiB,
+    bench_parse.cu:403-412).  CPU baseline: the hook restated in C
+    (oracle/json_dom.c), one core, replaying the oracle's structural index of
+    the document's first `cpu_sample` bytes (the replay + hook stage only)."""
+    import numpy as np
+    import torch
+
+    from pars_amd import Scanner
+    from pars_amd import _native as N
+
+    def run(sc, ptr, n):
+        st, walls, last = N.StageTimes(), [], None
+        parts = []
+        for i in range(reps + 1):
+            t0 = time.perf_counter()
+            last = sc.json_tape(ptr, n, stream, st)
+            if i:
+                walls.append(time.perf_counter() - t0)
+                parts.append((st.p1, st.p4, st.p2))
+        dev = float(np.median([a + b + c for a, b, c in parts]))
+        p = np.median(np.array(parts), axis=0)
+        return {"bytes": n, "nodes": int(last.n_nodes), "ok": bool(last.ok),
+                "device_ms": round(dev, 3), "gb_per_s": round(n / dev / 1e6, 2),
+                "stage_ms": {"r1_r5": round(float(p[0]), 3), "r6": round(float(p[1]), 3), "tape": round(float(p[2]), 3)},
+                "call_ms": round(float(np.median(walls)) * 1e3, 3)}
+
+    sc = Scanner("json")
+    out = {"call": "pars_hip_json_tape (R1-R6 + tape, device-resident)"}
+    out["json_2gib"] = run(sc, d_ptr, doc.size)
+    rec = b'{"id":123,"name":"x \\"y\\"","a":[1,2,3],"n":{"z":null}}'
+    k = (64 << 20) // (len(rec) + 1)
+    pay = np.frombuffer(b"[" + b",".join([rec] * k) + b"]", np.uint8)
+    t = torch.from_numpy(pay.copy()).to("cuda")
+    torch.cuda.synchronize()
+    out["record_payload_64mib"] = run(sc, t.data_ptr(), pay.size)
+    del t
+    sc.close()
+    if not no_cpu:
+        from oracle import oracle as O
+
+        sample = np.ascontiguousarray(doc[:cpu_sample])
+        pos = O.scan(Plan_json(), sample)["positions"]
+        t0 = time.perf_counter()
+        r = O.json_tape(sample, pos)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sample.size / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+                               "stage": "replay + JSON hook over a given structural index (oracle/json_dom.c)",
+                               "sample": f"first {sample.size} bytes of the 2 GiB document, {r['nodes']} nodes"}
+    return out
+
+
+def Plan_json() -> bytes:
+    from pars_amd import Plan
+
+    return Plan.load("json").blob
+
+
 def bench_serve(n_req: int, seed: int = 5) -> dict:
     """Serving: Poisson arrivals of JSON / CSV / C-like requests (60/25/15,
     log-normal sizes, median 4 KiB, sigma 1.85, clipped to [200 B, 4 MiB])
@@ -283,6 +344,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs (no autotuner)")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-boundary (streaming) measurement")
+    ap.add_argument("--no-tape", action="store_true", help="skip the JSON tape (stage-2 hook) measurement")
+    ap.add_argument("--tape-cpu-mib", type=int, default=256, help="CPU baseline sample of the JSON tape")
     ap.add_argument("--knobs", default="",
                     help="fixed launch knobs for the document scan instead of tuning, e.g. lb_window=16,rows=4 "
                          "(rows, lb_window, blocks_per_cu, radix_bits); profiles use this")
@@ -424,6 +487,9 @@ def main() -> int:
         line["host_stream"] = bench_stream(sc, doc, sp)
         assert line["host_stream"]["n_struct"] == k, "streamed scan disagrees with the device-resident one"
         log(f"host stream: {line['host_stream']}")
+    if not args.no_tape:
+        line["json_tape"] = bench_tape(d_doc.data_ptr(), doc, sp, 5, args.tape_cpu_mib << 20, args.no_cpu)
+        log(f"json tape: {line['json_tape']}")
     sc.close()
     del d_doc
     torch.cuda.empty_cache()

@@ -2,7 +2,9 @@
 // backend's template API (include/pars/backend/cuda/pipeline.cuh:1164-1281):
 //
 //   pars::hip::scanner<S>      scan_device / scan_host / reserve_struct /
-//                              scan_chunk / struct_ptr
+//                              scan_chunk / struct_ptr, and json_tape (the
+//                              reference's stage-2 JSON hook, bench_parse.cu:
+//                              336-348, built on the device)
 //   pars::hip::scan_device<S>, scan_host<S>, scan_chunk<S>
 //                              one scanner per plan per thread (:1260-1281)
 //
@@ -74,6 +76,10 @@ using chunk_carry = pars_chunk_carry;
 // pipeline.cuh:745-748
 using stage_times = pars_stage_times;
 
+// The JSON tape (pars_hip_json_tape): device pointers, valid until the
+// scanner's next call.
+using json_tape = pars_json_tape;
+
 // A scanner for a plan known at run time.
 class runtime_scanner {
 public:
@@ -132,6 +138,16 @@ public:
 	}
 
 	const std::int32_t* struct_ptr() const { return reinterpret_cast<const std::int32_t*>(pars_hip_struct_ptr(h_)); }
+
+	// R1-R6 and the JSON tape of a device-resident document: what the
+	// reference's pars_parse (bench_parse.cu:336-348) computes on the host by
+	// replaying the structural index into its JSON DOM hook.
+	json_tape json_tape_device(const std::uint8_t* d_data, std::size_t n, hipStream_t s = nullptr,
+								stage_times* st = nullptr) const {
+		json_tape t {};
+		check(pars_hip_json_tape(h_, d_data, n, s, st, &t));
+		return t;
+	}
 
 	pars_scanner* handle() const noexcept { return h_; }
 

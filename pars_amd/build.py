@@ -86,7 +86,7 @@ def build_facade_check() -> str:
     src = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
     out = os.path.join(ROOT, "tests", "cpp", "facade_check")
     _run(["g++", "-std=c++20", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-          "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lpars_hip",
+          "-I" + os.path.join(ROOT, "include"), src, "-L" + HERE, "-lpars_hip", "-L/opt/rocm/lib", "-lamdhip64",
           "-Wl,-rpath,$ORIGIN/../../pars_amd", "-o", out])
     return out
 

@@ -4,6 +4,9 @@
 //
 //   facade_check <plan file> <input file>   prints "n_struct balanced well_formed max_depth"
 //                                           then the structural positions, one per line
+//   facade_check <plan file> <input file> tape
+//                                           prints "n_nodes ok fail_event arena_bytes" of the
+//                                           JSON tape, then "kind begin end" per node
 #include "pars_hip.hpp"
 
 #include <cstdio>
@@ -63,6 +66,24 @@ int main(int argc, char** argv) {
 		pars_plan p;
 		std::memcpy(&p, plan_bytes.data(), sizeof p);
 		pars::hip::runtime_scanner sc(p);
+		if (argc > 3 && std::strcmp(argv[3], "tape") == 0) {
+			void* d = nullptr;
+			if (hipMalloc(&d, input.size() + 1) != hipSuccess ||
+				hipMemcpy(d, input.data(), input.size(), hipMemcpyHostToDevice) != hipSuccess)
+				return 1;
+			const auto t = sc.json_tape_device(static_cast<const std::uint8_t*>(d), input.size());
+			std::vector<std::uint8_t>  kind(t.n_nodes);
+			std::vector<std::uint32_t> b(t.n_nodes), e(t.n_nodes);
+			if (t.n_nodes && (hipMemcpy(kind.data(), t.kind, t.n_nodes, hipMemcpyDeviceToHost) != hipSuccess ||
+							  hipMemcpy(b.data(), t.begin, 4 * t.n_nodes, hipMemcpyDeviceToHost) != hipSuccess ||
+							  hipMemcpy(e.data(), t.end, 4 * t.n_nodes, hipMemcpyDeviceToHost) != hipSuccess))
+				return 1;
+			(void)hipFree(d);
+			std::printf("%llu %d %lld %llu\n", (unsigned long long)t.n_nodes, t.ok, (long long)t.fail_event,
+						(unsigned long long)t.arena_bytes);
+			for (std::size_t k = 0; k < t.n_nodes; ++k) std::printf("%u %u %u\n", kind[k], b[k], e[k]);
+			return 0;
+		}
 		const auto				   r = sc.scan_host(input.data(), input.size(), nullptr, true);
 		std::printf("%zu %d %d %d\n", r.structural.size(), r.balanced ? 1 : 0, r.well_formed ? 1 : 0, r.max_depth);
 		for (std::int32_t v : r.structural) std::printf("%d\n", v);

@@ -131,3 +131,26 @@ def test_repeated_calls_and_refusal(torch_cuda):
     with pytest.raises(Exception):
         t, ptr = to_device(torch_cuda, np.frombuffer(b, np.uint8))
         Scanner("csv").json_tape(ptr, len(b))
+
+
+def test_cpp_facade_tape(torch_cuda, tmp_path):
+    """runtime_scanner::json_tape_device (include/pars_hip.hpp) from a C++
+    program equals the oracle."""
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(__file__))
+    d = W.json(72, 80_000, 12).tobytes() + b' {"tail": [1, }'  # valid prefix, then a rejected event
+    inp = tmp_path / "in.json"
+    inp.write_bytes(d)
+    say("[gpu] C++ facade tape")
+    r = subprocess.run([os.path.join(root, "tests", "cpp", "facade_check"), os.path.join(root, "pars_amd", "plans",
+                        "json.plan"), str(inp), "tape"], stdout=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0
+    rows = r.stdout.split("\n")
+    n, ok, fe, _ = (int(v) for v in rows[0].split())
+    exp = O.json_tape(d, O.scan(PLAN, np.frombuffer(d, np.uint8))["positions"])
+    assert (n, bool(ok), fe) == (exp["nodes"], exp["ok"], exp["fail_event"])
+    got = np.array([[int(v) for v in row.split()] for row in rows[1:1 + n]], dtype=np.int64).reshape(-1, 3)
+    assert np.array_equal(got[:, 0], exp["kind"]) and np.array_equal(got[:, 1], exp["begin"])
+    assert np.array_equal(got[:, 2], exp["end"])

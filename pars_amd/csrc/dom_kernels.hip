@@ -71,13 +71,6 @@ __device__ __forceinline__ uint32_t sym_class(uint32_t c) {
 	}
 }
 
-__device__ __forceinline__ bool is_ws(uint32_t c) {
-	return c == ' ' || c == '\t' || c == '\n' || c == '\r';
-}
-__device__ __forceinline__ bool is_digit(uint32_t c) {
-	return c - '0' < 10u;
-}
-
 struct Gap {
 	uint32_t kind, ok, beg, end, dec;
 };
@@ -91,9 +84,9 @@ struct Bytes {
 	uint64_t	   base;
 	uint4		   v;
 
-	__device__ Bytes(const uint8_t* d_, uint64_t n_) : d(d_), n(n_), base(~0ull), v(make_uint4(0, 0, 0, 0)) {}
+	__host__ __device__ Bytes(const uint8_t* d_, uint64_t n_) : d(d_), n(n_), base(~0ull), v(make_uint4(0, 0, 0, 0)) {}
 
-	__device__ __forceinline__ uint32_t operator[](uint64_t i) {
+	__host__ __device__ __forceinline__ uint32_t operator[](uint64_t i) {
 		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
 		const uint64_t a   = i - mis;
 		if (a != base) {
@@ -106,194 +99,242 @@ struct Bytes {
 		const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
 		return (w >> (8u * (k & 3u))) & 0xFFu;
 	}
-
-	// Bytes of a string body that decode to themselves - not '"', '\\' or a
-	// control byte - starting at i, counted to the end of i's window or to e.
-	// Byte masks by the zero-byte test on 32-bit words: every real match is
-	// flagged; a false flag can only sit above a real one, so the first flag
-	// is never late (a false one just hands that byte to the caller's check).
-	__device__ __forceinline__ uint32_t plain_run(uint64_t i, uint64_t e) {
-		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
-		const uint64_t a   = i - mis;
-		if (i < mis || a + 16u > n)
-			return 0;
-		if (a != base) {
-			base = a;
-			v	 = *reinterpret_cast<const uint4*>(d + a);
-		}
-		auto special = [](uint32_t w) {
-			const uint32_t q = w ^ 0x22222222u, b = w ^ 0x5C5C5C5Cu;
-			const uint32_t m = ((q - 0x01010101u) & ~q) | ((b - 0x01010101u) & ~b) | ((w - 0x20202020u) & ~w);
-			const uint32_t t = m & 0x80808080u;
-			return ((t >> 7) & 1u) | ((t >> 14) & 2u) | ((t >> 21) & 4u) | ((t >> 28) & 8u);
-		};
-		uint32_t mask = special(v.x) | (special(v.y) << 4) | (special(v.z) << 8) | (special(v.w) << 12);
-		const uint32_t k = (uint32_t)mis;
-		mask &= ~((1u << k) - 1u);
-		uint32_t stop = mask ? (uint32_t)__builtin_ctz(mask) : 16u;
-		const uint64_t lim = e - a;	 // bytes of the window inside [a, e)
-		if (lim < stop)
-			stop = (uint32_t)lim;
-		return stop > k ? stop - k : 0u;
-	}
 };
 
-// The value in [b, e) as the hook parses it (bench_parse.cu:86-130, 219-332):
+// The value in [b, e) as the hook parses it (bench_parse.cu:86-130, 219-332)
+// — leading whitespace, one string / number / literal, trailing whitespace —
+// as one DFA over byte classes, so every lane of a wave runs the same loop body
+// whatever its gap holds (a branch per value kind made a wave run each kind's
+// loop in turn).  Transition entries: next state in bits 0-4; bit 5: a value
+// completes here; bit 6: the completing byte belongs to the value (a closing
+// quote, a literal's last letter) rather than following it (a number ends at
+// the first byte that cannot continue it).  The gap's end is one more class,
+// K_END, stepped once after the last byte: a number ends there, anything else
+// unfinished fails (the end is handled inside the automaton; a separate
+// fix-up after the loop lost its store of the end offset in device code).
+enum : uint8_t {
+	K_SP, K_WSC, K_CTL, K_QUOTE, K_BSL, K_MINUS, K_PLUS, K_DOT, K_ZERO, K_D19, K_e, K_E, K_a, K_b, K_cd, K_f, K_HEXU,
+	K_l, K_n, K_r, K_s, K_t, K_u, K_SLASH, K_OTHER, K_END, K_N
+};
+enum : uint8_t {
+	S_LEAD, S_STR, S_ESC, S_U1, S_U2, S_U3, S_U4, S_NMIN, S_NZERO, S_NINT, S_NDOT, S_NFRAC, S_NE, S_NESIGN, S_NEXP,
+	S_T1, S_T2, S_T3, S_F1, S_F2, S_F3, S_F4, S_N1, S_N2, S_N3, S_TRAIL, S_BADVAL, S_BADTRAIL, S_N
+};
+constexpr uint8_t kDone = 32, kIncl = 64;
+
+struct GapTables {
+	uint8_t cls[256];
+	uint8_t tr[S_N * K_N];
+	uint8_t esc[256];  // byte an escape decodes to (0 = not an escape letter)
+};
+
+constexpr GapTables make_gap_tables() {
+	GapTables g {};
+	for (int c = 0; c < 256; ++c) {
+		uint8_t k = K_OTHER;
+		if (c == ' ')
+			k = K_SP;
+		else if (c == '\t' || c == '\n' || c == '\r')
+			k = K_WSC;
+		else if (c < 0x20)
+			k = K_CTL;
+		else if (c == '"')
+			k = K_QUOTE;
+		else if (c == '\\')
+			k = K_BSL;
+		else if (c == '-')
+			k = K_MINUS;
+		else if (c == '+')
+			k = K_PLUS;
+		else if (c == '.')
+			k = K_DOT;
+		else if (c == '0')
+			k = K_ZERO;
+		else if (c >= '1' && c <= '9')
+			k = K_D19;
+		else if (c == 'e')
+			k = K_e;
+		else if (c == 'E')
+			k = K_E;
+		else if (c == 'a')
+			k = K_a;
+		else if (c == 'b')
+			k = K_b;
+		else if (c == 'c' || c == 'd')
+			k = K_cd;
+		else if (c == 'f')
+			k = K_f;
+		else if (c == 'A' || c == 'B' || c == 'C' || c == 'D' || c == 'F')
+			k = K_HEXU;
+		else if (c == 'l')
+			k = K_l;
+		else if (c == 'n')
+			k = K_n;
+		else if (c == 'r')
+			k = K_r;
+		else if (c == 's')
+			k = K_s;
+		else if (c == 't')
+			k = K_t;
+		else if (c == 'u')
+			k = K_u;
+		else if (c == '/')
+			k = K_SLASH;
+		g.cls[c] = k;
+	}
+	auto set = [&](int s, int k, int v) { g.tr[s * K_N + k] = (uint8_t)v; };
+	for (int s = 0; s < S_N; ++s)
+		for (int k = 0; k < K_N; ++k) set(s, k, s == S_BADTRAIL ? S_BADTRAIL : S_BADVAL);
+	auto ws		= [](int k) { return k == K_SP || k == K_WSC; };
+	auto digit	= [](int k) { return k == K_ZERO || k == K_D19; };
+	auto hex	= [](int k) {
+		   return k == K_ZERO || k == K_D19 || k == K_a || k == K_b || k == K_cd || k == K_e || k == K_E || k == K_f ||
+				  k == K_HEXU;
+	};
+	for (int k = 0; k < K_N; ++k) {
+		// before the value
+		set(S_LEAD, k, ws(k) ? S_LEAD
+					   : k == K_QUOTE ? S_STR
+					   : k == K_MINUS ? S_NMIN
+					   : k == K_ZERO  ? S_NZERO
+					   : k == K_D19   ? S_NINT
+					   : k == K_t	  ? S_T1
+					   : k == K_f	  ? S_F1
+					   : k == K_n	  ? S_N1
+									  : S_BADVAL);
+		// strings (bench_parse.cu:231-284)
+		set(S_STR, k, k == K_QUOTE ? (S_TRAIL | kDone | kIncl)
+				  : k == K_BSL	   ? S_ESC
+				  : (k == K_WSC || k == K_CTL) ? S_BADVAL
+											   : S_STR);
+		const bool esc_letter = k == K_QUOTE || k == K_BSL || k == K_SLASH || k == K_b || k == K_f || k == K_n ||
+								k == K_r || k == K_t;
+		set(S_ESC, k, esc_letter ? S_STR : k == K_u ? S_U1 : S_BADVAL);
+		set(S_U1, k, hex(k) ? S_U2 : S_BADVAL);
+		set(S_U2, k, hex(k) ? S_U3 : S_BADVAL);
+		set(S_U3, k, hex(k) ? S_U4 : S_BADVAL);
+		set(S_U4, k, hex(k) ? S_STR : S_BADVAL);
+		// numbers (bench_parse.cu:286-306): a number ends before the first byte
+		// that cannot continue it; that byte must then be whitespace
+		const int after = ws(k) ? (S_TRAIL | kDone) : (S_BADTRAIL | kDone);
+		set(S_NMIN, k, k == K_ZERO ? S_NZERO : k == K_D19 ? S_NINT : S_BADVAL);
+		set(S_NZERO, k, k == K_DOT ? S_NDOT : (k == K_e || k == K_E) ? S_NE : after);
+		set(S_NINT, k, digit(k) ? S_NINT : k == K_DOT ? S_NDOT : (k == K_e || k == K_E) ? S_NE : after);
+		set(S_NDOT, k, digit(k) ? S_NFRAC : S_BADVAL);
+		set(S_NFRAC, k, digit(k) ? S_NFRAC : (k == K_e || k == K_E) ? S_NE : after);
+		set(S_NE, k, (k == K_PLUS || k == K_MINUS) ? S_NESIGN : digit(k) ? S_NEXP : S_BADVAL);
+		set(S_NESIGN, k, digit(k) ? S_NEXP : S_BADVAL);
+		set(S_NEXP, k, digit(k) ? S_NEXP : after);
+		// literals (bench_parse.cu:308-332)
+		set(S_T1, k, k == K_r ? S_T2 : S_BADVAL);
+		set(S_T2, k, k == K_u ? S_T3 : S_BADVAL);
+		set(S_T3, k, k == K_e ? (S_TRAIL | kDone | kIncl) : S_BADVAL);
+		set(S_F1, k, k == K_a ? S_F2 : S_BADVAL);
+		set(S_F2, k, k == K_l ? S_F3 : S_BADVAL);
+		set(S_F3, k, k == K_s ? S_F4 : S_BADVAL);
+		set(S_F4, k, k == K_e ? (S_TRAIL | kDone | kIncl) : S_BADVAL);
+		set(S_N1, k, k == K_u ? S_N2 : S_BADVAL);
+		set(S_N2, k, k == K_l ? S_N3 : S_BADVAL);
+		set(S_N3, k, k == K_l ? (S_TRAIL | kDone | kIncl) : S_BADVAL);
+		// after the value: whitespace only
+		set(S_TRAIL, k, ws(k) ? S_TRAIL : S_BADTRAIL);
+	}
+	// the gap's end
+	for (int st = 0; st < S_N; ++st) set(st, K_END, st == S_BADTRAIL ? S_BADTRAIL : S_BADVAL);
+	set(S_LEAD, K_END, S_LEAD);
+	set(S_TRAIL, K_END, S_TRAIL);
+	for (int st : {S_NZERO, S_NINT, S_NFRAC, S_NEXP}) set(st, K_END, S_TRAIL | kDone);
+	g.esc['"']	= '"';
+	g.esc['\\'] = '\\';
+	g.esc['/']	= '/';
+	g.esc['b']	= '\b';
+	g.esc['f']	= '\f';
+	g.esc['n']	= '\n';
+	g.esc['r']	= '\r';
+	g.esc['t']	= '\t';
+	return g;
+}
+
+__constant__ GapTables kGapTables = make_gap_tables();
+
+// The tables in LDS (every block copies them first).
+__device__ __forceinline__ void load_gap_tables(GapTables& L) {
+	const uint32_t* src = reinterpret_cast<const uint32_t*>(&kGapTables);
+	uint32_t*		dst = reinterpret_cast<uint32_t*>(&L);
+	for (uint32_t i = threadIdx.x; i < sizeof(GapTables) / 4; i += blockDim.x) dst[i] = src[i];
+	__syncthreads();
+}
+
 // kind, whether it parsed with only whitespace after it, its first byte, one
 // past its last byte (0 for a string that did not close), and the decoded
 // bytes a string mixed into the hash.  EMIT writes those bytes to `out`.
 template<bool EMIT>
-__device__ __forceinline__ Gap parse_gap(Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
+__host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
 	Gap		 g {G_EMPTY, 0u, 0u, 0u, 0u};
-	uint64_t i = b;
-	while (i < e && is_ws(d[i])) ++i;
-	if (i >= e)
-		return g;
-	g.beg			 = (uint32_t)i;
-	const uint32_t c = d[i];
-	bool		   parsed = false;
-	if (c == '"') {
-		g.kind		 = G_STR;
-		uint64_t j	 = i + 1;
-		uint32_t dec = 0;
-		auto put = [&](uint32_t v) {
-			if constexpr (EMIT)
-				out[dec] = (uint8_t)v;
-			++dec;
-		};
-		bool fail = false;
-		while (j < e && !fail && !parsed) {
-			const uint32_t run = d.plain_run(j, e);
-			if (run) {
-				if constexpr (EMIT)
-					for (uint32_t k = 0; k < run; ++k) out[dec + k] = (uint8_t)d[j + k];
-				dec += run;
-				j += run;
-				continue;
-			}
-			const uint32_t x = d[j++];
-			if (x == '"') {
-				g.end  = (uint32_t)j;
-				parsed = true;
-			} else if (x == '\\') {
-				if (j >= e) {
-					fail = true;
-					break;
-				}
-				const uint32_t esc = d[j++];
-				if (esc == 'u') {
-					if (j + 4 > e) {
-						fail = true;
-						break;
-					}
-					uint32_t v = 0;
-					for (int k = 0; k < 4 && !fail; ++k) {
-						const uint32_t h = d[j++];
-						v <<= 4;
-						if (is_digit(h))
-							v |= h - '0';
-						else if (h - 'a' < 6u)
-							v |= h - 'a' + 10u;
-						else if (h - 'A' < 6u)
-							v |= h - 'A' + 10u;
-						else
-							fail = true;
-					}
-					if (fail)
-						break;
-					if (v < 0x80u) {
-						put(v);
-					} else if (v < 0x800u) {
-						put(0xC0u | (v >> 6));
-						put(0x80u | (v & 0x3Fu));
-					} else {
-						put(0xE0u | (v >> 12));
-						put(0x80u | ((v >> 6) & 0x3Fu));
-						put(0x80u | (v & 0x3Fu));
-					}
+	uint32_t s = S_LEAD, dec = 0, v = 0, first = K_OTHER;
+	bool	 done = false;
+	uint64_t j	  = b;
+	for (; j <= e && s < S_BADVAL; ++j) {
+		const uint32_t c = j < e ? d[j] : 0u;
+		const uint32_t k = j < e ? (uint32_t)L.cls[c] : (uint32_t)K_END;
+		const uint32_t x = L.tr[s * K_N + k];
+		const uint32_t t = x & 31u;
+		if (s == S_LEAD && t != S_LEAD) {
+			g.beg = (uint32_t)j;
+			first = k;
+		}
+		// decoded bytes: a plain string byte, an escape letter, a \u unit
+		uint32_t w = 0, nw = 0;
+		if (t == S_STR) {
+			if (s == S_STR) {
+				w  = c;
+				nw = 1;
+			} else if (s == S_ESC) {
+				w  = L.esc[c];
+				nw = 1;
+			} else if (s == S_U4) {
+				v = (v << 4) | (c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u));
+				if (v < 0x80u) {
+					w  = v;
+					nw = 1;
+				} else if (v < 0x800u) {
+					w  = (0xC0u | (v >> 6)) | ((0x80u | (v & 0x3Fu)) << 8);
+					nw = 2;
 				} else {
-					uint32_t v;
-					switch (esc) {
-					case '"': v = '"'; break;
-					case '\\': v = '\\'; break;
-					case '/': v = '/'; break;
-					case 'b': v = '\b'; break;
-					case 'f': v = '\f'; break;
-					case 'n': v = '\n'; break;
-					case 'r': v = '\r'; break;
-					case 't': v = '\t'; break;
-					default: v = 256u; break;
-					}
-					if (v > 255u) {
-						fail = true;
-						break;
-					}
-					put(v);
+					w  = (0xE0u | (v >> 12)) | ((0x80u | ((v >> 6) & 0x3Fu)) << 8) | ((0x80u | (v & 0x3Fu)) << 16);
+					nw = 3;
 				}
-			} else if (x < 0x20u) {
-				fail = true;
-			} else {
-				put(x);
 			}
+		} else if (t >= S_U2 && t <= S_U4) {
+			v = (v << 4) | (c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u));
+		} else if (t == S_U1) {
+			v = 0;
 		}
-		g.dec = dec;
-	} else if (c == '-' || is_digit(c)) {
-		uint64_t j	 = i + (c == '-' ? 1 : 0);
-		bool	 bad = j >= e;
-		if (!bad) {
-			if (d[j] == '0')
-				++j;
-			else if (d[j] - '1' < 9u)
-				while (j < e && is_digit(d[j])) ++j;
-			else
-				bad = true;
+		if constexpr (EMIT)
+			for (uint32_t q = 0; q < nw; ++q) out[dec + q] = (uint8_t)(w >> (8 * q));
+		dec += nw;
+		if (x & kDone) {
+			done  = true;
+			g.end = (uint32_t)((x & kIncl) ? j + 1 : j);
 		}
-		if (!bad && j < e && d[j] == '.') {
-			++j;
-			if (j >= e || !is_digit(d[j]))
-				bad = true;
-			else
-				while (j < e && is_digit(d[j])) ++j;
-		}
-		if (!bad && j < e && (d[j] == 'e' || d[j] == 'E')) {
-			++j;
-			if (j < e && (d[j] == '+' || d[j] == '-'))
-				++j;
-			if (j >= e || !is_digit(d[j]))
-				bad = true;
-			else
-				while (j < e && is_digit(d[j])) ++j;
-		}
-		g.kind = bad ? G_BAD : G_NUM;
-		if (!bad) {
-			g.end  = (uint32_t)j;
-			parsed = true;
-		}
-	} else if (c == 't' || c == 'f' || c == 'n') {
-		// "true", "false", "null" (bench_parse.cu:308-332)
-		const uint64_t avail = e - i;
-		uint32_t	   w4	  = 0;	// the first four bytes, little endian
-		for (uint32_t k = 0; k < 4u && k < avail; ++k) w4 |= d[i + k] << (8u * k);
-		g.kind = G_BAD;
-		if (avail >= 4u && w4 == 0x65757274u) {	 // "true"
-			g.kind = G_TRUE;
-			g.end  = (uint32_t)(i + 4);
-		} else if (avail >= 5u && w4 == 0x736C6166u && d[i + 4] == 'e') {	// "fals" "e"
-			g.kind = G_FALSE;
-			g.end  = (uint32_t)(i + 5);
-		} else if (avail >= 4u && w4 == 0x6C6C756Eu) {  // "null"
-			g.kind = G_NULL;
-			g.end  = (uint32_t)(i + 4);
-		}
-		parsed = g.kind != G_BAD;
-	} else {
-		g.kind = G_BAD;
+		s = t;
 	}
-	if (parsed) {
-		uint64_t k = g.end;
-		while (k < e && is_ws(d[k])) ++k;
-		g.ok = k == e;
+	if (first == K_OTHER && s == S_LEAD)
+		return g;  // whitespace only
+	switch (first) {
+	case K_QUOTE: g.kind = G_STR; break;
+	case K_MINUS:
+	case K_ZERO:
+	case K_D19: g.kind = done ? G_NUM : G_BAD; break;
+	case K_t: g.kind = done ? G_TRUE : G_BAD; break;
+	case K_f: g.kind = done ? G_FALSE : G_BAD; break;
+	case K_n: g.kind = done ? G_NULL : G_BAD; break;
+	default: g.kind = G_BAD; break;
 	}
+	if (!done)
+		g.end = 0;
+	g.dec = dec;
+	g.ok  = done && s == S_TRAIL;
 	return g;
 }
 
@@ -399,7 +440,9 @@ __global__ void k_dom_reset(DomSummary* ds) {
 // 1. Per token (one per thread): symbol class and the gap's value; the counts
 // summed per 4096-token group (the group sums start at zero).
 __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
-	__shared__ uint32_t s[kDomBlock / 64][3];
+	__shared__ uint32_t	 s[kDomBlock / 64][3];
+	__shared__ GapTables L;
+	load_gap_tables(L);
 	const uint64_t		ntok = a.n_struct + 1;
 	const uint64_t		t	 = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
 	uint32_t			sum[3] = {0u, 0u, 0u};
@@ -407,7 +450,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 		uint64_t b, e;
 		gap_bounds(a, t, b, e);
 		Bytes		   rd(a.data, a.n);
-		const Gap	   g   = parse_gap<false>(rd, b, e, nullptr);
+		const Gap	   g   = parse_gap<false>(L, rd, b, e, nullptr);
 		const uint32_t sym = t < a.n_struct ? sym_class(a.data[a.positions[t]]) : SYM_END;
 		const uint32_t w   = sym | (g.kind << 3) | (g.ok << 6);
 		a.tinfo[t]		   = w;
@@ -679,6 +722,8 @@ __global__ void k_dom_final(DomArgs a) {
 
 // 10. The tape, in document order, and the decoded string bytes.
 __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
+	__shared__ GapTables L;
+	load_gap_tables(L);
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long fe	  = a.ds->fail_event;  // kDomNone: none
 	const uint32_t			 part = a.ds->partial;
@@ -702,7 +747,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 					uint64_t b, e;
 					gap_bounds(a, t, b, e);
 					Bytes rd(a.data, a.n);
-					(void)parse_gap<true>(rd, b, e, a.arena + a.arena_x[t]);
+					(void)parse_gap<true>(L, rd, b, e, a.arena + a.arena_x[t]);
 				}
 			}
 			++idx;

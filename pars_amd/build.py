@@ -91,11 +91,21 @@ def build_facade_check() -> str:
     return out
 
 
+def build_gap_check() -> str:
+    """tests/cpp/gap_check: the JSON tape's gap parser (device code) built for
+    the host, checked against the oracle by tests/test_gap_parser.py."""
+    src = os.path.join(ROOT, "tests", "cpp", "gap_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "gap_check")
+    _run([HIPCC, *HIP_FLAGS, src, "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_oracle()
     build_hip()
     build_facade_check()
+    build_gap_check()
 
 
 if __name__ == "__main__":

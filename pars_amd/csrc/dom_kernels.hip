@@ -743,11 +743,20 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 					a.nend[idx] = a.gend[t];
 					a.soff[idx] = a.arena_x[t];
 				}
-				if (gk == G_STR && (uint64_t)a.arena_x[t] + a.gdec[t] <= a.cap_arena) {
-					uint64_t b, e;
-					gap_bounds(a, t, b, e);
-					Bytes rd(a.data, a.n);
-					(void)parse_gap<true>(L, rd, b, e, a.arena + a.arena_x[t]);
+				const uint32_t dec = a.gdec[t];
+				if (gk == G_STR && (uint64_t)a.arena_x[t] + dec <= a.cap_arena) {
+					Bytes		   rd(a.data, a.n);
+					uint8_t*	   out = a.arena + a.arena_x[t];
+					const uint32_t gb = a.gbeg[t], ge = a.gend[t];
+					if (ge && ge - gb - 2u == dec) {
+						// closed, and as long decoded as between its quotes: no
+						// escape (every escape decodes shorter), the bytes as they are
+						for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[(uint64_t)gb + 1u + q];
+					} else {
+						uint64_t b, e;
+						gap_bounds(a, t, b, e);
+						(void)parse_gap<true>(L, rd, b, e, out);
+					}
 				}
 			}
 			++idx;

@@ -154,3 +154,19 @@ def test_cpp_facade_tape(torch_cuda, tmp_path):
     got = np.array([[int(v) for v in row.split()] for row in rows[1:1 + n]], dtype=np.int64).reshape(-1, 3)
     assert np.array_equal(got[:, 0], exp["kind"]) and np.array_equal(got[:, 1], exp["begin"])
     assert np.array_equal(got[:, 2], exp["end"])
+
+
+def test_large_document_against_c_twin(torch_cuda):
+    """256 MiB of the benchmark's JSON: every node and every decoded byte
+    against the hook restated in C (oracle/json_dom.c)."""
+    d = W.json(2, 256 << 20, 32)
+    say(f"[gpu] tape 256 MiB n={d.size}")
+    t, ptr = to_device(torch_cuda, d)
+    r = scanner().json_tape(ptr, d.size, torch_cuda.cuda.current_stream().cuda_stream)
+    exp = O.json_tape(d, O.scan(PLAN, d)["positions"])
+    assert (r.ok, r.n_nodes, r.fail_event) == (exp["ok"], exp["nodes"], exp["fail_event"])
+    assert np.array_equal(d2h(r.kind, r.n_nodes, np.uint8), exp["kind"])
+    assert np.array_equal(d2h(r.begin, r.n_nodes, np.uint32), exp["begin"])
+    assert np.array_equal(d2h(r.end, r.n_nodes, np.uint32), exp["end"])
+    assert d2h(r.arena, r.arena_bytes, np.uint8).tobytes() == exp["dec"]
+    del t

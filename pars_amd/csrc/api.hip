@@ -126,7 +126,7 @@ struct pars_scanner {
 	// the JSON tape: per-token and per-bracket scratch, the tape, its summary
 	int			force_radix = 0;  // R6 through the radix passes (their sorted order is kept)
 	DevBuf		dom_tinfo, dom_gbeg, dom_gend, dom_gdec, dom_nx, dom_ax, dom_bx, dom_top, dom_tblk, dom_hblk,
-		dom_parent, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
+		dom_parent, dom_close, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
 	DomSummary* h_dom = nullptr;  // pinned
 	size_t		h_dom_cap = 0;
 
@@ -137,7 +137,7 @@ struct pars_scanner {
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
 						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
 						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &dom_tinfo, &dom_gbeg, &dom_gend,
-						  &dom_gdec, &dom_nx, &dom_ax, &dom_bx, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_sum,
+						  &dom_gdec, &dom_nx, &dom_ax, &dom_bx, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum,
 						  &tape_kind, &tape_beg, &tape_end, &tape_soff, &tape_arena})
 			b->release();
 		if (h_dom)
@@ -1178,6 +1178,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 			return r;
 	if ((r = sc->dom_top.ensure(ntok)) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
 		(r = sc->dom_hblk.ensure(n_hblk * 4)) || (r = sc->dom_parent.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
+		(r = sc->dom_close.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
 		(r = sc->dom_sum.ensure(sizeof(DomSummary))) || (r = sc->tape_kind.ensure(cap_nodes)) ||
 		(r = sc->tape_beg.ensure(cap_nodes * 4)) || (r = sc->tape_end.ensure(cap_nodes * 4)) ||
 		(r = sc->tape_soff.ensure(cap_nodes * 4)) || (r = sc->tape_arena.ensure(n)))
@@ -1195,7 +1196,6 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	a.bchar		= sc->bchar.as<uint8_t>();
 	a.bkey		= sc->bkey.as<int32_t>();
 	a.bsidx		= sc->bsidx.as<uint32_t>();
-	a.rank		= sc->rank.as<uint32_t>();
 	a.order		= ((h.sort_passes - 1) & 1) ? sc->order1.as<uint32_t>() : sc->order0.as<uint32_t>();
 	a.tinfo		= sc->dom_tinfo.as<uint32_t>();
 	a.gbeg		= sc->dom_gbeg.as<uint32_t>();
@@ -1210,6 +1210,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	a.hblk		= sc->dom_hblk.as<int32_t>();
 	a.n_hblk	= n_hblk;
 	a.parent	= sc->dom_parent.as<int32_t>();
+	a.close		= sc->dom_close.as<uint32_t>();
 	a.ds		= sc->dom_sum.as<DomSummary>();
 	a.kind		= sc->tape_kind.as<uint8_t>();
 	a.nbeg		= sc->tape_beg.as<uint32_t>();

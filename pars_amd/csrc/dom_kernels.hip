@@ -75,29 +75,36 @@ struct Gap {
 	uint32_t kind, ok, beg, end, dec;
 };
 
-// The input read through a 16-byte register window: one vector load per
-// aligned 16 bytes, byte loads where a window would reach past the buffer.
-// A lane walks its gap forwards, so most bytes come from registers.
+// The input read through a 32-byte register window: two vector loads issued
+// together per aligned 32 bytes, so a typical gap (mean ~10 bytes) costs one
+// load round trip, not one per 16-byte boundary it crosses (a wave waits for
+// every lane's reload, and lanes cross boundaries at different bytes).  Byte
+// loads serve the buffer's edges, where a window would reach outside it.
 struct Bytes {
 	const uint8_t* d;
-	uint64_t	   n;
+	uint64_t	   lo, hi;	// windows [a, a + 32) with lo <= a and a + 32 <= hi lie in the buffer
 	uint64_t	   base;
-	uint4		   v;
+	uint4		   v0, v1;
 
-	__host__ __device__ Bytes(const uint8_t* d_, uint64_t n_) : d(d_), n(n_), base(~0ull), v(make_uint4(0, 0, 0, 0)) {}
+	__host__ __device__ Bytes(const uint8_t* d_, uint64_t n_)
+		: d(d_), lo((16u - ((uintptr_t)d_ & 15u)) & 15u), hi(n_), base(1ull << 63), v0(make_uint4(0, 0, 0, 0)),
+		  v1(make_uint4(0, 0, 0, 0)) {}  // no window yet: i - base >= 32 for every i < 2^63 - 32
 
 	__host__ __device__ __forceinline__ uint32_t operator[](uint64_t i) {
-		const uint64_t mis = (uint64_t)((uintptr_t)(d + i) & 15u);
-		const uint64_t a   = i - mis;
-		if (a != base) {
-			if (i < mis || a + 16u > n)
+		if (i - base >= 32u) {	// outside the window (i < base wraps)
+			const uint64_t a = i - (uint64_t)((uintptr_t)(d + i) & 15u);
+			if (i < lo || a + 32u > hi)
 				return d[i];
 			base = a;
-			v	 = *reinterpret_cast<const uint4*>(d + a);
+			v0	 = *reinterpret_cast<const uint4*>(d + a);
+			v1	 = *reinterpret_cast<const uint4*>(d + a + 16u);
 		}
-		const uint32_t k = (uint32_t)mis;
-		const uint32_t w = k < 8u ? (k < 4u ? v.x : v.y) : (k < 12u ? v.z : v.w);
-		return (w >> (8u * (k & 3u))) & 0xFFu;
+		// selects of values, not of fields (a select of fields can become a
+		// select of addresses, which keeps the window in scratch)
+		const uint32_t k  = (uint32_t)(i - base);
+		const uint64_t h0 = (k & 8u) ? ((uint64_t)v0.w << 32 | v0.z) : ((uint64_t)v0.y << 32 | v0.x);
+		const uint64_t h1 = (k & 8u) ? ((uint64_t)v1.w << 32 | v1.z) : ((uint64_t)v1.y << 32 | v1.x);
+		return (uint32_t)(((k & 16u) ? h1 : h0) >> (8u * (k & 7u))) & 0xFFu;
 	}
 };
 
@@ -279,45 +286,38 @@ __host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, 
 		const uint32_t k = j < e ? (uint32_t)L.cls[c] : (uint32_t)K_END;
 		const uint32_t x = L.tr[s * K_N + k];
 		const uint32_t t = x & 31u;
-		if (s == S_LEAD && t != S_LEAD) {
-			g.beg = (uint32_t)j;
-			first = k;
-		}
-		// decoded bytes: a plain string byte, an escape letter, a \u unit
-		uint32_t w = 0, nw = 0;
-		if (t == S_STR) {
-			if (s == S_STR) {
-				w  = c;
-				nw = 1;
-			} else if (s == S_ESC) {
-				w  = L.esc[c];
-				nw = 1;
-			} else if (s == S_U4) {
-				v = (v << 4) | (c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u));
-				if (v < 0x80u) {
-					w  = v;
-					nw = 1;
-				} else if (v < 0x800u) {
-					w  = (0xC0u | (v >> 6)) | ((0x80u | (v & 0x3Fu)) << 8);
-					nw = 2;
-				} else {
-					w  = (0xE0u | (v >> 12)) | ((0x80u | ((v >> 6) & 0x3Fu)) << 8) | ((0x80u | (v & 0x3Fu)) << 16);
-					nw = 3;
-				}
+		// every step computes every case and selects (no branches: the lanes
+		// of a wave are in different states)
+		const bool lead = s == S_LEAD && t != S_LEAD;
+		g.beg			= lead ? (uint32_t)j : g.beg;
+		first			= lead ? k : first;
+		// \u units: four hex digits accumulate from S_U1 on; the fourth
+		// (S_U4 → S_STR) decodes to 1-3 UTF-8 bytes
+		const uint32_t hv = c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u);
+		const uint32_t vn = (v << 4) | hv;
+		const bool	   u4 = s == S_U4 && t == S_STR;
+		const uint32_t nu = vn < 0x80u ? 1u : vn < 0x800u ? 2u : 3u;
+		// decoded bytes: a plain string byte or an escape letter (one), a \u
+		// unit (nu)
+		const uint32_t nw = u4 ? nu : (t == S_STR && (s == S_STR || s == S_ESC)) ? 1u : 0u;
+		if constexpr (EMIT) {
+			uint32_t w = s == S_ESC ? (uint32_t)L.esc[c] : c;
+			if (u4) {
+				if (vn < 0x80u)
+					w = vn;
+				else if (vn < 0x800u)
+					w = (0xC0u | (vn >> 6)) | ((0x80u | (vn & 0x3Fu)) << 8);
+				else
+					w = (0xE0u | (vn >> 12)) | ((0x80u | ((vn >> 6) & 0x3Fu)) << 8) | ((0x80u | (vn & 0x3Fu)) << 16);
 			}
-		} else if (t >= S_U2 && t <= S_U4) {
-			v = (v << 4) | (c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u));
-		} else if (t == S_U1) {
-			v = 0;
-		}
-		if constexpr (EMIT)
 			for (uint32_t q = 0; q < nw; ++q) out[dec + q] = (uint8_t)(w >> (8 * q));
-		dec += nw;
-		if (x & kDone) {
-			done  = true;
-			g.end = (uint32_t)((x & kIncl) ? j + 1 : j);
 		}
-		s = t;
+		v = t == S_U1 ? 0u : (s >= S_U1 && s <= S_U3) ? vn : v;
+		dec += nw;
+		const bool fin = (x & kDone) != 0u;
+		done		   = done || fin;
+		g.end		   = fin ? (uint32_t)((x & kIncl) ? j + 1 : j) : g.end;
+		s			   = t;
 	}
 	if (first == K_OTHER && s == S_LEAD)
 		return g;  // whitespace only
@@ -437,33 +437,105 @@ __global__ void k_dom_reset(DomSummary* ds) {
 	}
 }
 
-// 1. Per token (one per thread): symbol class and the gap's value; the counts
-// summed per 4096-token group (the group sums start at zero).
+// The per-gap kernels take kSortTok tokens per block and hand them to lanes
+// in order of their work (gap bytes to walk), so the 64 lanes of a wave walk
+// gaps of about one length: a wave lasts as long as its longest gap, and in
+// document order that is ~4.5× the mean on JSON.
+constexpr int kSortTok = 1024;
+constexpr int kSortPer = kSortTok / kDomBlock;
+constexpr int kBins	   = 64;
+static_assert(kDomTile % kSortTok == 0, "a block's tokens lie in one sum group");
+
+// work → bin: exact below 32 bytes, 8-byte bins to 280, one bin beyond
+__device__ __forceinline__ uint32_t work_bin(uint32_t w) {
+	return w < 32u ? w : (w < 32u + 31u * 8u ? 32u + (w - 32u) / 8u : 63u);
+}
+
+// Counting sort of the block's tokens (local index i * kDomBlock + thread)
+// by bin: perm[slot] = local index.  Order within a bin is arbitrary, which
+// changes no result: every token's outputs depend on that token alone.
+__device__ __forceinline__ void sort_by_work(const uint32_t (&work)[kSortPer], uint16_t* perm, uint32_t* bins) {
+	if (threadIdx.x < kBins)
+		bins[threadIdx.x] = 0;
+	__syncthreads();
+	uint32_t bin[kSortPer], rank[kSortPer];
+#pragma unroll
+	for (int i = 0; i < kSortPer; ++i) {
+		bin[i]	= work_bin(work[i]);
+		rank[i] = atomicAdd(&bins[bin[i]], 1u);
+	}
+	__syncthreads();
+	if (threadIdx.x < kBins) {	// wave 0: kBins == 64 lanes
+		const uint32_t c = bins[threadIdx.x];
+		bins[threadIdx.x] = wave_incl_sum(c) - c;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < kSortPer; ++i) perm[bins[bin[i]] + rank[i]] = (uint16_t)(i * kDomBlock + threadIdx.x);
+	__syncthreads();
+}
+
+// 1. Per token: symbol class and the gap's value; the counts summed per
+// 4096-token group (the group sums start at zero).  Gaps are parsed in work
+// order and their results go back to document order through LDS.
 __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 	__shared__ uint32_t	 s[kDomBlock / 64][3];
 	__shared__ GapTables L;
+	__shared__ uint16_t	 perm[kSortTok];
+	__shared__ uint32_t	 bins[kBins];
+	__shared__ uint32_t	 r_beg[kSortTok], r_end[kSortTok], r_dec[kSortTok];
+	__shared__ uint8_t	 r_gap[kSortTok];
 	load_gap_tables(L);
-	const uint64_t		ntok = a.n_struct + 1;
-	const uint64_t		t	 = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
-	uint32_t			sum[3] = {0u, 0u, 0u};
-	if (t < ntok) {
-		uint64_t b, e;
-		gap_bounds(a, t, b, e);
-		Bytes		   rd(a.data, a.n);
-		const Gap	   g   = parse_gap<false>(L, rd, b, e, nullptr);
-		const uint32_t sym = t < a.n_struct ? sym_class(a.data[a.positions[t]]) : SYM_END;
-		const uint32_t w   = sym | (g.kind << 3) | (g.ok << 6);
-		a.tinfo[t]		   = w;
-		a.gbeg[t]		   = g.beg;
-		a.gend[t]		   = g.end;
-		a.gdec[t]		   = g.dec;
-		sum[0]			   = t_nodes(w);
-		sum[1]			   = g.dec;
-		sum[2]			   = t_bracket(w) ? 1u : 0u;
+	const uint64_t ntok = a.n_struct + 1;
+	const uint64_t base = (uint64_t)blockIdx.x * kSortTok;
+	uint32_t	   work[kSortPer];
+#pragma unroll
+	for (int i = 0; i < kSortPer; ++i) {
+		const uint32_t j = (uint32_t)(i * kDomBlock) + threadIdx.x;
+		const uint64_t t = base + j;
+		work[i]			 = 0;
+		if (t < ntok) {
+			uint64_t b, e;
+			gap_bounds(a, t, b, e);
+			work[i]	 = (uint32_t)(e - b);
+			r_beg[j] = (uint32_t)b;	 // the gap's bounds, replaced by the results
+			r_end[j] = (uint32_t)e;
+		}
+	}
+	sort_by_work(work, perm, bins);
+#pragma unroll 1
+	for (int i = 0; i < kSortPer; ++i) {
+		const uint32_t j = perm[i * kDomBlock + threadIdx.x];
+		if (base + j < ntok) {
+			Bytes	  rd(a.data, a.n);
+			const Gap g = parse_gap<false>(L, rd, r_beg[j], r_end[j], nullptr);
+			r_beg[j]	= g.beg;
+			r_end[j]	= g.end;
+			r_dec[j]	= g.dec;
+			r_gap[j]	= (uint8_t)((g.kind << 3) | (g.ok << 6));
+		}
+	}
+	__syncthreads();
+	uint32_t sum[3] = {0u, 0u, 0u};
+#pragma unroll
+	for (int i = 0; i < kSortPer; ++i) {
+		const uint32_t j = (uint32_t)(i * kDomBlock) + threadIdx.x;
+		const uint64_t t = base + j;
+		if (t < ntok) {
+			const uint32_t sym = t < a.n_struct ? sym_class(a.data[a.positions[t]]) : SYM_END;
+			const uint32_t w   = sym | r_gap[j];
+			a.tinfo[t]		   = w;
+			a.gbeg[t]		   = r_beg[j];
+			a.gend[t]		   = r_end[j];
+			a.gdec[t]		   = r_dec[j];
+			sum[0] += t_nodes(w);
+			sum[1] += r_dec[j];
+			sum[2] += t_bracket(w) ? 1u : 0u;
+		}
 	}
 	uint32_t tot[3];
 	block_excl3(sum, tot, s);
-	const uint64_t grp = ((uint64_t)blockIdx.x * kDomBlock) / kDomTile;
+	const uint64_t grp = base / kDomTile;
 	if (threadIdx.x == 0)
 #pragma unroll
 		for (int f = 0; f < 3; ++f)
@@ -559,7 +631,11 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_heads_top(DomArgs a) {
 	}
 }
 
-// 6. Every bracket's parent: the bracket before its run's start.
+// 6. Every bracket's parent: the bracket before its run's start; and every
+// opening bracket's partner: the next bracket in sorted order when it has the
+// same key (the structural index of that bracket, else ~0u).  That bracket is
+// a close: after an open at key d the depth is d + 1, and only a close brings
+// it back to d, so the next bracket at key d in document order closes.
 __global__ __launch_bounds__(kDomBlock) void k_dom_parent(DomArgs a) {
 	__shared__ int32_t s[kDomBlock / 64];
 	const uint64_t	   base	 = (uint64_t)blockIdx.x * kDomTile;
@@ -568,8 +644,19 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_parent(DomArgs a) {
 		const uint64_t p  = base + (uint64_t)r * kDomBlock + threadIdx.x;
 		const bool	   in = p < a.nb;
 		const int32_t  h  = block_incl_max(in ? head_at(a, p) : -1, carry, s);
-		if (in)
-			a.parent[a.order[p]] = h >= 0 ? (int32_t)a.order[h] - 1 : -1;
+		if (in) {
+			const uint32_t x = a.order[p];
+			a.parent[x]		 = h >= 0 ? (int32_t)a.order[h] - 1 : -1;
+			if (br_open(a, x)) {
+				uint32_t tc = ~0u;
+				if (p + 1u < a.nb) {
+					const uint32_t y = a.order[p + 1u];
+					if (a.bkey[y] == a.bkey[x])
+						tc = a.bsidx[y];
+				}
+				a.close[x] = tc;
+			}
+		}
 	}
 }
 
@@ -721,19 +808,32 @@ __global__ void k_dom_final(DomArgs a) {
 }
 
 // 10. The tape, in document order, and the decoded string bytes.
+// Node records in document order; strings' decoded bytes afterwards, in work
+// order (bytes to copy, or gap bytes to decode again).
+enum : uint8_t { H_NONE = 0, H_COPY, H_DECODE };
+
 __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	__shared__ GapTables L;
+	__shared__ uint16_t	 perm[kSortTok];
+	__shared__ uint32_t	 bins[kBins];
+	__shared__ uint8_t	 s_how[kSortTok];
 	load_gap_tables(L);
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long fe	  = a.ds->fail_event;  // kDomNone: none
 	const uint32_t			 part = a.ds->partial;
-	const uint64_t			 t	  = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
-	if (t < ntok) {
+	const uint64_t			 base = (uint64_t)blockIdx.x * kSortTok;
+	uint32_t				 work[kSortPer];
+#pragma unroll
+	for (int i = 0; i < kSortPer; ++i) {
+		const uint32_t j = (uint32_t)(i * kDomBlock) + threadIdx.x;
+		const uint64_t t = base + j;
+		work[i]			 = 0;
+		s_how[j]		 = H_NONE;
 		const unsigned long long eg = 2ull * t;
-		if (eg > fe)
-			return;
-		const uint32_t w  = a.tinfo[t];
-		const uint32_t gk = t_gap(w);
+		if (t >= ntok || eg > fe)
+			continue;
+		const uint32_t w   = a.tinfo[t];
+		const uint32_t gk  = t_gap(w);
 		uint64_t	   idx = a.nodes_x[t];
 		if (gk >= G_STR) {
 			if (eg < fe || part) {
@@ -745,17 +845,17 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 				}
 				const uint32_t dec = a.gdec[t];
 				if (gk == G_STR && (uint64_t)a.arena_x[t] + dec <= a.cap_arena) {
-					Bytes		   rd(a.data, a.n);
-					uint8_t*	   out = a.arena + a.arena_x[t];
 					const uint32_t gb = a.gbeg[t], ge = a.gend[t];
+					// closed, and as long decoded as between its quotes: no escape
+					// (every escape decodes shorter), the bytes as they are
 					if (ge && ge - gb - 2u == dec) {
-						// closed, and as long decoded as between its quotes: no
-						// escape (every escape decodes shorter), the bytes as they are
-						for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[(uint64_t)gb + 1u + q];
+						work[i]	 = dec + 1u;
+						s_how[j] = H_COPY;
 					} else {
 						uint64_t b, e;
 						gap_bounds(a, t, b, e);
-						(void)parse_gap<true>(L, rd, b, e, out);
+						work[i]	 = (uint32_t)(e - b) + 1u;
+						s_how[j] = H_DECODE;
 					}
 				}
 			}
@@ -765,21 +865,32 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 		if ((sym == SYM_OOPEN || sym == SYM_AOPEN) && eg + 1ull < fe && idx < a.cap_nodes) {
 			// the close that follows this open at the same key is its partner
 			// while the hook accepts it
-			const uint32_t b   = a.brk_x[t];
-			const uint32_t r   = a.rank[b];
-			uint32_t	   end = 0;
-			if ((uint64_t)r + 1u < a.nb) {
-				const uint32_t x = a.order[r + 1u];
-				if (a.bkey[x] == a.bkey[b] && (a.plan->br_flags[a.bchar[x]] & 2u)) {
-					const uint32_t tc = a.bsidx[x];
-					if (2ull * tc + 1ull < fe)
-						end = a.positions[tc];
-				}
-			}
+			const uint32_t tc  = a.close[a.brk_x[t]];
+			const uint32_t end = tc != ~0u && 2ull * tc + 1ull < fe ? a.positions[tc] : 0u;
 			a.kind[idx] = sym == SYM_OOPEN ? 0u : 1u;
 			a.nbeg[idx] = a.positions[t];
 			a.nend[idx] = end;
 			a.soff[idx] = a.arena_x[t] + a.gdec[t];
+		}
+	}
+	sort_by_work(work, perm, bins);
+#pragma unroll 1
+	for (int i = 0; i < kSortPer; ++i) {
+		const uint32_t j   = perm[i * kDomBlock + threadIdx.x];
+		const uint32_t how = s_how[j];
+		if (how == H_NONE)
+			continue;
+		const uint64_t t   = base + j;
+		uint8_t*	   out = a.arena + a.arena_x[t];
+		Bytes		   rd(a.data, a.n);
+		if (how == H_COPY) {
+			const uint64_t src = (uint64_t)a.gbeg[t] + 1u;
+			const uint32_t dec = a.gdec[t];
+			for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[src + q];
+		} else {
+			uint64_t b, e;
+			gap_bounds(a, t, b, e);
+			(void)parse_gap<true>(L, rd, b, e, out);
 		}
 	}
 }
@@ -790,11 +901,12 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 	const uint64_t ntok = a.n_struct + 1;
 	const unsigned gt	= (unsigned)((ntok + kDomBlock - 1) / kDomBlock);	// one token per thread
+	const unsigned gs	= (unsigned)((ntok + kSortTok - 1) / kSortTok);		// kSortTok tokens per block
 	hipLaunchKernelGGL(k_dom_reset, dim3(1), dim3(64), 0, s, a.ds);
 	hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.tblk), a.n_tblk * 3, 0, s);
 	if (e != hipSuccess)
 		return e;
-	hipLaunchKernelGGL(k_dom_tok, dim3(gt), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_tok, dim3(gs), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tok_top, dim3(1), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tok_apply, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
 	if (a.nb) {
@@ -805,7 +917,7 @@ hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 	hipLaunchKernelGGL(k_dom_check, dim3(gt), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_top2, dim3(gt), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_final, dim3(1), dim3(64), 0, s, a);
-	hipLaunchKernelGGL(k_dom_tape, dim3(gt), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_tape, dim3(gs), dim3(kDomBlock), 0, s, a);
 	return hipGetLastError();
 }
 

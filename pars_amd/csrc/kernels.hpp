@@ -243,7 +243,6 @@ struct DomArgs {
 	const uint8_t*	bchar;
 	const int32_t*	bkey;
 	const uint32_t* bsidx;
-	const uint32_t* rank;	   // R6: sorted position of each bracket
 	const uint32_t* order;	   // R6: bracket at each sorted position
 	// per token
 	uint32_t* tinfo;
@@ -260,6 +259,7 @@ struct DomArgs {
 	int32_t*  hblk;	  // per sorted block: last head, then the exclusive max
 	uint64_t  n_hblk;
 	int32_t*  parent;  // per bracket: the container that encloses its pair, or -1
+	uint32_t* close;   // per opening bracket: the structural index of its partner, or ~0u
 	DomSummary* ds;
 	// outputs
 	uint8_t*  kind;

@@ -47,6 +47,15 @@ int main(int argc, char** argv) {
 		const Gap g2 = parse_gap<true>(T, rd2, shift, shift + len, out.data());
 		if (g.dec != g2.dec || g.kind != g2.kind || g.end != g2.end || g.ok != g2.ok)
 			return 3;
+		// the gap alone in a buffer of its own size at an odd address: every
+		// read near the buffer's edges goes through the byte path
+		std::vector<unsigned char> tight(len + 1), out3(g2.dec + 1);
+		std::memcpy(&tight[1], &in[off], len);
+		Bytes	  rd3(tight.data() + 1, len);
+		const Gap g3 = parse_gap<true>(T, rd3, 0, len, out3.data());
+		if (g3.dec != g2.dec || g3.kind != g2.kind || g3.ok != g2.ok || (g3.end ? g3.end + shift : 0) != g2.end ||
+			(g3.kind && g3.beg + shift != g2.beg) || std::memcmp(out3.data(), out.data(), g2.dec) != 0)
+			return 4;
 		std::printf("%u %u %lld %lld %u ", g.kind, g.ok, g.kind ? (long long)g.beg - (long long)shift : 0LL,
 					g.end ? (long long)g.end - (long long)shift : -1LL, g.dec);
 		for (uint32_t q = 0; q < g2.dec; ++q) std::printf("%02x", out[q]);

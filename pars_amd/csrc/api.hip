@@ -206,6 +206,8 @@ int ensure_outputs(pars_scanner* sc, uint64_t n_tiles, uint64_t cap_struct, uint
 			return r;
 	if ((r = sc->bchar.ensure(cap_br)))
 		return r;
+	if (sc->force_radix && (r = sc->dom_close.ensure(cap_br * 4)))  // the tape's bracket partners
+		return r;
 	const uint64_t chunks = pairing_max_chunks(cap_br);
 	if ((r = sc->hist.ensure(256 * std::max<uint64_t>(chunks, 1) * 4)))
 		return r;
@@ -261,6 +263,7 @@ PairArgs make_pair_args(pars_scanner* sc, bool want_pairs) {
 	p.max_chunks = pairing_max_chunks(sc->cap_br);
 	p.pair		 = sc->pair.as<int32_t>();
 	p.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	p.close		 = sc->force_radix ? sc->dom_close.as<uint32_t>() : nullptr;
 	p.max_digit_bits = (uint32_t)sc->radix_bits;
 	return p;
 }
@@ -1178,7 +1181,6 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 			return r;
 	if ((r = sc->dom_top.ensure(ntok)) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
 		(r = sc->dom_hblk.ensure(n_hblk * 4)) || (r = sc->dom_parent.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
-		(r = sc->dom_close.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
 		(r = sc->dom_sum.ensure(sizeof(DomSummary))) || (r = sc->tape_kind.ensure(cap_nodes)) ||
 		(r = sc->tape_beg.ensure(cap_nodes * 4)) || (r = sc->tape_end.ensure(cap_nodes * 4)) ||
 		(r = sc->tape_soff.ensure(cap_nodes * 4)) || (r = sc->tape_arena.ensure(n)))

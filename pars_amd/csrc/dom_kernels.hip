@@ -631,11 +631,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_heads_top(DomArgs a) {
 	}
 }
 
-// 6. Every bracket's parent: the bracket before its run's start; and every
-// opening bracket's partner: the next bracket in sorted order when it has the
-// same key (the structural index of that bracket, else ~0u).  That bracket is
-// a close: after an open at key d the depth is d + 1, and only a close brings
-// it back to d, so the next bracket at key d in document order closes.
+// 6. Every bracket's parent: the bracket before its run's start.
 __global__ __launch_bounds__(kDomBlock) void k_dom_parent(DomArgs a) {
 	__shared__ int32_t s[kDomBlock / 64];
 	const uint64_t	   base	 = (uint64_t)blockIdx.x * kDomTile;
@@ -644,19 +640,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_parent(DomArgs a) {
 		const uint64_t p  = base + (uint64_t)r * kDomBlock + threadIdx.x;
 		const bool	   in = p < a.nb;
 		const int32_t  h  = block_incl_max(in ? head_at(a, p) : -1, carry, s);
-		if (in) {
-			const uint32_t x = a.order[p];
-			a.parent[x]		 = h >= 0 ? (int32_t)a.order[h] - 1 : -1;
-			if (br_open(a, x)) {
-				uint32_t tc = ~0u;
-				if (p + 1u < a.nb) {
-					const uint32_t y = a.order[p + 1u];
-					if (a.bkey[y] == a.bkey[x])
-						tc = a.bsidx[y];
-				}
-				a.close[x] = tc;
-			}
-		}
+		if (in)
+			a.parent[a.order[p]] = h >= 0 ? (int32_t)a.order[h] - 1 : -1;
 	}
 }
 

@@ -794,14 +794,21 @@ __global__ void k_dom_final(DomArgs a) {
 
 // 10. The tape, in document order, and the decoded string bytes.
 // Node records in document order; strings' decoded bytes afterwards, in work
-// order (bytes to copy, or gap bytes to decode again).
+// order (bytes to copy, or gap bytes to decode again).  A block's strings own
+// one contiguous run of the arena: the lanes write their bytes into LDS and
+// the block stores the run with aligned 16-byte stores (byte stores from 64
+// lanes into 64 strings cost one memory request per byte).  A run longer
+// than the staging buffer is written directly.
 enum : uint8_t { H_NONE = 0, H_COPY, H_DECODE };
+constexpr uint32_t kStage = 12288;	// staged decoded bytes per block
 
 __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	__shared__ GapTables L;
 	__shared__ uint16_t	 perm[kSortTok];
 	__shared__ uint32_t	 bins[kBins];
 	__shared__ uint8_t	 s_how[kSortTok];
+	__shared__ __align__(16) uint8_t stage[kStage];
+	__shared__ uint64_t	 s_rng[2];
 	load_gap_tables(L);
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long fe	  = a.ds->fail_event;  // kDomNone: none
@@ -858,24 +865,57 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 			a.soff[idx] = a.arena_x[t] + a.gdec[t];
 		}
 	}
-	sort_by_work(work, perm, bins);
+	if (threadIdx.x == 0) {
+		// the block's run of the arena, cut at the tape's end (strings after
+		// the hook's first rejected event are not written)
+		const uint64_t last = (base + kSortTok < ntok ? base + kSortTok : ntok) - 1u;
+		const uint64_t a0	= a.arena_x[base];
+		const uint64_t a1	= (uint64_t)a.arena_x[last] + a.gdec[last];
+		const uint64_t lim	= a.ds->arena_bytes;
+		s_rng[0]			= a0;
+		s_rng[1]			= a1 < lim ? (a1 > a0 ? a1 : a0) : (lim > a0 ? lim : a0);
+	}
+	sort_by_work(work, perm, bins);	 // (its barriers publish s_rng)
+	const uint64_t a0 = s_rng[0], a1 = s_rng[1];
+	const uint64_t c0 = a0 & ~15ull;  // stage[g - c0] holds arena byte g: LDS and HBM agree mod 16
+	const bool	   staged = a1 - c0 <= kStage;	// block-uniform
+	// one loop per destination, so the staged one stores to LDS directly
+	// rather than through generic addresses
+	auto strings = [&](auto dst) {
 #pragma unroll 1
-	for (int i = 0; i < kSortPer; ++i) {
-		const uint32_t j   = perm[i * kDomBlock + threadIdx.x];
-		const uint32_t how = s_how[j];
-		if (how == H_NONE)
-			continue;
-		const uint64_t t   = base + j;
-		uint8_t*	   out = a.arena + a.arena_x[t];
-		Bytes		   rd(a.data, a.n);
-		if (how == H_COPY) {
-			const uint64_t src = (uint64_t)a.gbeg[t] + 1u;
-			const uint32_t dec = a.gdec[t];
-			for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[src + q];
-		} else {
-			uint64_t b, e;
-			gap_bounds(a, t, b, e);
-			(void)parse_gap<true>(L, rd, b, e, out);
+		for (int i = 0; i < kSortPer; ++i) {
+			const uint32_t j   = perm[i * kDomBlock + threadIdx.x];
+			const uint32_t how = s_how[j];
+			if (how == H_NONE)
+				continue;
+			const uint64_t t   = base + j;
+			uint8_t*	   out = dst(a.arena_x[t]);
+			Bytes		   rd(a.data, a.n);
+			if (how == H_COPY) {
+				const uint64_t src = (uint64_t)a.gbeg[t] + 1u;
+				const uint32_t dec = a.gdec[t];
+				for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[src + q];
+			} else {
+				uint64_t b, e;
+				gap_bounds(a, t, b, e);
+				(void)parse_gap<true>(L, rd, b, e, out);
+			}
+		}
+	};
+	if (!staged) {
+		strings([&](uint64_t off) { return a.arena + off; });
+	} else {
+		strings([&](uint64_t off) { return stage + (off - c0); });
+		__syncthreads();
+		const uint64_t c1 = (a1 + 15u) & ~15ull;
+		for (uint64_t c = c0 + 16u * threadIdx.x; c < c1; c += 16u * kDomBlock) {
+			const uint32_t so = (uint32_t)(c - c0);
+			if (c >= a0 && c + 16u <= a1)
+				*reinterpret_cast<uint4*>(a.arena + c) = *reinterpret_cast<const uint4*>(stage + so);
+			else  // the run's ends share 16 bytes with the neighbouring blocks' runs
+				for (uint32_t q = 0; q < 16u; ++q)
+					if (c + q >= a0 && c + q < a1)
+						a.arena[c + q] = stage[so + q];
 		}
 	}
 }

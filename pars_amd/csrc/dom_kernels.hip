@@ -564,37 +564,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok_top(DomArgs a) {
 		}
 }
 
-// 3. Per token exclusive prefixes: nodes, decoded bytes, brackets.
-__global__ __launch_bounds__(kDomBlock) void k_dom_tok_apply(DomArgs a) {
-	__shared__ uint32_t s[kDomBlock / 64][3];
-	const uint64_t		ntok = a.n_struct + 1;
-	const uint64_t		base = (uint64_t)blockIdx.x * kDomTile;
-	uint32_t			carry[3];
-#pragma unroll
-	for (int f = 0; f < 3; ++f) carry[f] = a.tblk[blockIdx.x * 3 + f];
-	for (int r = 0; r < kDomRows; ++r) {
-		const uint64_t t  = base + (uint64_t)r * kDomBlock + threadIdx.x;
-		const bool	   in = t < ntok;
-		uint32_t	   v[3] = {0u, 0u, 0u};
-		if (in) {
-			const uint32_t w = a.tinfo[t];
-			v[0]			 = t_nodes(w);
-			v[1]			 = a.gdec[t];
-			v[2]			 = t_bracket(w) ? 1u : 0u;
-		}
-		uint32_t tot[3];
-		block_excl3(v, tot, s);
-		if (in) {
-			a.nodes_x[t] = carry[0] + v[0];
-			a.arena_x[t] = carry[1] + v[1];
-			a.brk_x[t]	 = carry[2] + v[2];
-		}
-#pragma unroll
-		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
-	}
-}
-
-// 4. Run starts over the sorted brackets: block maxima.
+// 3. Run starts over the sorted brackets: block maxima.
 __device__ __forceinline__ int32_t head_at(const DomArgs& a, uint64_t p) {
 	const uint32_t x = a.order[p];
 	return (br_open(a, x) && (x == 0 || br_open(a, x - 1u))) ? (int32_t)p : -1;
@@ -615,7 +585,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_heads(DomArgs a) {
 		a.hblk[blockIdx.x] = carry;
 }
 
-// 5. Exclusive max over the block maxima (one block).
+// 4. Exclusive max over the block maxima (one block).
 __global__ __launch_bounds__(kDomBlock) void k_dom_heads_top(DomArgs a) {
 	__shared__ int32_t s[kDomBlock / 64];
 	const uint64_t	   nb  = a.n_hblk;
@@ -631,7 +601,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_heads_top(DomArgs a) {
 	}
 }
 
-// 6. Every bracket's parent: the bracket before its run's start.
+// 5. Every bracket's parent: the bracket before its run's start.
 __global__ __launch_bounds__(kDomBlock) void k_dom_parent(DomArgs a) {
 	__shared__ int32_t s[kDomBlock / 64];
 	const uint64_t	   base	 = (uint64_t)blockIdx.x * kDomTile;
@@ -665,81 +635,109 @@ __device__ __forceinline__ void wave_atomic_min(unsigned long long* p, unsigned 
 		atomicMin(p, v);
 }
 
-// 7. Every event against the hook's rules (bench_parse.cu:86-194).  Top-level
-// values are flagged: every one after the first is rejected (the hook's
-// top_done), resolved by k_dom_top2.
-__global__ __launch_bounds__(kDomBlock) void k_dom_check(DomArgs a) {
-	const uint64_t ntok = a.n_struct + 1;
-	{
-		const uint64_t	   t  = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
-		unsigned long long fk = kDomNone, top = kDomNone;
-		if (t < ntok) {
-			const uint32_t w = a.tinfo[t];
-			// the innermost open container before structural t
-			int64_t		   cont = -1;
-			const int64_t  b	= (int64_t)a.brk_x[t] - 1;
-			if (b >= 0)
-				cont = br_open(a, (uint64_t)b) ? b : (int64_t)a.parent[b];
-			const uint32_t cc	= cont >= 0 ? a.bchar[cont] : 0u;
-			const uint32_t pt	= t ? prev_of_symbol(t_sym(a.tinfo[t - 1])) : P_NONE;
-			const uint32_t gk	= t_gap(w);
-			uint32_t	   p	= pt;
-			uint8_t		   flag = 0;
-			auto reject = [&](unsigned long long ev, uint32_t partial) {
-				const unsigned long long k = (ev << 1) | partial;
-				fk						   = k < fk ? k : fk;
-			};
-			if (gk != G_EMPTY) {
-				const unsigned long long eg = 2ull * t;
-				const uint32_t			 node = gk >= G_STR ? 1u : 0u;
-				if (cc == '{' && (pt == P_NONE || pt == P_COMMA)) {	 // a key
-					if (gk != G_STR)
-						reject(eg, 0u);
-					else if (!t_gok(w))
-						reject(eg, 1u);
-					p = P_KEY;
-				} else {
-					if (cc == 0u)
-						flag |= 1u;
-					else if (!(cc == '{' ? pt == P_COLON : (pt == P_NONE || pt == P_COMMA)))
-						reject(eg, 0u);
-					if (gk == G_BAD)
-						reject(eg, 0u);
-					else if (!t_gok(w))
-						reject(eg, node);
-					p = P_VALUE;
-				}
-			}
-			if (t < a.n_struct) {
-				const unsigned long long es = 2ull * t + 1ull;
-				bool					 ok = true;
-				switch (t_sym(w)) {
-				case SYM_OOPEN:
-				case SYM_AOPEN:
-					if (cc == 0u)
-						flag |= 2u;
-					else
-						ok = cc == '{' ? p == P_COLON : (p == P_NONE || p == P_COMMA);
-					break;
-				case SYM_OCLOSE: ok = cc == '{' && (p == P_NONE || p == P_COMMA || p == P_VALUE); break;
-				case SYM_ACLOSE: ok = cc == '[' && (p == P_NONE || p == P_COMMA || p == P_VALUE); break;
-				case SYM_COLON: ok = cc == '{' && p == P_KEY; break;
-				case SYM_COMMA: ok = cc != 0u && p == P_VALUE; break;
-				default: ok = false; break;
-				}
-				if (!ok)
-					reject(es, 0u);
-			}
-			a.topflag[t] = flag;
-			if (flag)
-				top = (flag & 1u) ? 2ull * t : 2ull * t + 1ull;
+// 6. Per token exclusive prefixes (nodes, decoded bytes, brackets) and every
+// event against the hook's rules (bench_parse.cu:86-194), in one pass over a
+// 4096-token group per block: the check needs the bracket prefix the scan
+// has just produced.  Top-level values are flagged: every one after the
+// first is rejected (the hook's top_done), resolved by k_dom_top2.
+__device__ __forceinline__ void check_token(const DomArgs& a, uint64_t t, uint32_t w, uint32_t brk,
+											unsigned long long& fk, unsigned long long& top) {
+	// the innermost open container before structural t
+	int64_t		   cont = -1;
+	const int64_t  b	= (int64_t)brk - 1;
+	if (b >= 0)
+		cont = br_open(a, (uint64_t)b) ? b : (int64_t)a.parent[b];
+	const uint32_t cc	= cont >= 0 ? a.bchar[cont] : 0u;
+	const uint32_t pt	= t ? prev_of_symbol(t_sym(a.tinfo[t - 1])) : P_NONE;
+	const uint32_t gk	= t_gap(w);
+	uint32_t	   p	= pt;
+	uint8_t		   flag = 0;
+	auto reject = [&](unsigned long long ev, uint32_t partial) {
+		const unsigned long long k = (ev << 1) | partial;
+		fk						   = k < fk ? k : fk;
+	};
+	if (gk != G_EMPTY) {
+		const unsigned long long eg = 2ull * t;
+		const uint32_t			 node = gk >= G_STR ? 1u : 0u;
+		if (cc == '{' && (pt == P_NONE || pt == P_COMMA)) {	 // a key
+			if (gk != G_STR)
+				reject(eg, 0u);
+			else if (!t_gok(w))
+				reject(eg, 1u);
+			p = P_KEY;
+		} else {
+			if (cc == 0u)
+				flag |= 1u;
+			else if (!(cc == '{' ? pt == P_COLON : (pt == P_NONE || pt == P_COMMA)))
+				reject(eg, 0u);
+			if (gk == G_BAD)
+				reject(eg, 0u);
+			else if (!t_gok(w))
+				reject(eg, node);
+			p = P_VALUE;
 		}
-		wave_atomic_min(&a.ds->fail_key, fk);
-		wave_atomic_min(&a.ds->e1, top);
+	}
+	if (t < a.n_struct) {
+		const unsigned long long es = 2ull * t + 1ull;
+		bool					 ok = true;
+		switch (t_sym(w)) {
+		case SYM_OOPEN:
+		case SYM_AOPEN:
+			if (cc == 0u)
+				flag |= 2u;
+			else
+				ok = cc == '{' ? p == P_COLON : (p == P_NONE || p == P_COMMA);
+			break;
+		case SYM_OCLOSE: ok = cc == '{' && (p == P_NONE || p == P_COMMA || p == P_VALUE); break;
+		case SYM_ACLOSE: ok = cc == '[' && (p == P_NONE || p == P_COMMA || p == P_VALUE); break;
+		case SYM_COLON: ok = cc == '{' && p == P_KEY; break;
+		case SYM_COMMA: ok = cc != 0u && p == P_VALUE; break;
+		default: ok = false; break;
+		}
+		if (!ok)
+			reject(es, 0u);
+	}
+	a.topflag[t] = flag;
+	if (flag) {
+		const unsigned long long v = (flag & 1u) ? 2ull * t : 2ull * t + 1ull;
+		top						   = v < top ? v : top;
 	}
 }
 
-// 8. The second top-level value event.
+__global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
+	__shared__ uint32_t s[kDomBlock / 64][3];
+	const uint64_t		ntok = a.n_struct + 1;
+	const uint64_t		base = (uint64_t)blockIdx.x * kDomTile;
+	uint32_t			carry[3];
+#pragma unroll
+	for (int f = 0; f < 3; ++f) carry[f] = a.tblk[blockIdx.x * 3 + f];
+	unsigned long long fk = kDomNone, top = kDomNone;
+	for (int r = 0; r < kDomRows; ++r) {
+		const uint64_t t  = base + (uint64_t)r * kDomBlock + threadIdx.x;
+		const bool	   in = t < ntok;
+		uint32_t	   v[3] = {0u, 0u, 0u}, w = 0;
+		if (in) {
+			w	 = a.tinfo[t];
+			v[0] = t_nodes(w);
+			v[1] = a.gdec[t];
+			v[2] = t_bracket(w) ? 1u : 0u;
+		}
+		uint32_t tot[3];
+		block_excl3(v, tot, s);
+		if (in) {
+			a.nodes_x[t] = carry[0] + v[0];
+			a.arena_x[t] = carry[1] + v[1];
+			a.brk_x[t]	 = carry[2] + v[2];
+			check_token(a, t, w, carry[2] + v[2], fk, top);
+		}
+#pragma unroll
+		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
+	}
+	wave_atomic_min(&a.ds->fail_key, fk);
+	wave_atomic_min(&a.ds->e1, top);
+}
+
+// 7. The second top-level value event.
 __global__ __launch_bounds__(kDomBlock) void k_dom_top2(DomArgs a) {
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long e1	  = a.ds->e1;
@@ -757,7 +755,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_top2(DomArgs a) {
 	}
 }
 
-// 9. Where the hook stopped, how many nodes and decoded bytes it produced.
+// 8. Where the hook stopped, how many nodes and decoded bytes it produced.
 __global__ void k_dom_final(DomArgs a) {
 	if (threadIdx.x != 0)
 		return;
@@ -792,7 +790,7 @@ __global__ void k_dom_final(DomArgs a) {
 	}
 }
 
-// 10. The tape, in document order, and the decoded string bytes.
+// 9. The tape, in document order, and the decoded string bytes.
 // Node records in document order; strings' decoded bytes afterwards, in work
 // order (bytes to copy, or gap bytes to decode again).  A block's strings own
 // one contiguous run of the arena: the lanes write their bytes into LDS and
@@ -933,13 +931,12 @@ hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 		return e;
 	hipLaunchKernelGGL(k_dom_tok, dim3(gs), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tok_top, dim3(1), dim3(kDomBlock), 0, s, a);
-	hipLaunchKernelGGL(k_dom_tok_apply, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
 	if (a.nb) {
 		hipLaunchKernelGGL(k_dom_heads, dim3((unsigned)a.n_hblk), dim3(kDomBlock), 0, s, a);
 		hipLaunchKernelGGL(k_dom_heads_top, dim3(1), dim3(kDomBlock), 0, s, a);
 		hipLaunchKernelGGL(k_dom_parent, dim3((unsigned)a.n_hblk), dim3(kDomBlock), 0, s, a);
 	}
-	hipLaunchKernelGGL(k_dom_check, dim3(gt), dim3(kDomBlock), 0, s, a);
+	hipLaunchKernelGGL(k_dom_apply_check, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_top2, dim3(gt), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_final, dim3(1), dim3(64), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tape, dim3(gs), dim3(kDomBlock), 0, s, a);

@@ -1179,7 +1179,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	for (DevBuf* b : {&sc->dom_tinfo, &sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec, &sc->dom_nx, &sc->dom_ax, &sc->dom_bx})
 		if ((r = b->ensure(ntok * 4)))
 			return r;
-	if ((r = sc->dom_top.ensure(ntok)) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
+	if ((r = sc->dom_top.ensure(n_tblk * 2 * sizeof(unsigned long long))) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
 		(r = sc->dom_hblk.ensure(n_hblk * 4)) || (r = sc->dom_parent.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
 		(r = sc->dom_sum.ensure(sizeof(DomSummary))) || (r = sc->tape_kind.ensure(cap_nodes)) ||
 		(r = sc->tape_beg.ensure(cap_nodes * 4)) || (r = sc->tape_end.ensure(cap_nodes * 4)) ||
@@ -1206,7 +1206,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	a.nodes_x	= sc->dom_nx.as<uint32_t>();
 	a.arena_x	= sc->dom_ax.as<uint32_t>();
 	a.brk_x		= sc->dom_bx.as<uint32_t>();
-	a.topflag	= sc->dom_top.as<uint8_t>();
+	a.top2		= sc->dom_top.as<unsigned long long>();
 	a.tblk		= sc->dom_tblk.as<uint32_t>();
 	a.n_tblk	= n_tblk;
 	a.hblk		= sc->dom_hblk.as<int32_t>();

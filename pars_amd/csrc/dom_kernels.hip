@@ -635,13 +635,30 @@ __device__ __forceinline__ void wave_atomic_min(unsigned long long* p, unsigned 
 		atomicMin(p, v);
 }
 
+// The two smallest of a set of distinct events, kDomNone where missing.
+struct Two {
+	unsigned long long m1, m2;
+	__device__ __forceinline__ void add(unsigned long long v) {	 // v above every value added so far
+		if (m1 == kDomNone)
+			m1 = v;
+		else if (m2 == kDomNone)
+			m2 = v;
+	}
+	__device__ __forceinline__ void merge(unsigned long long b1, unsigned long long b2) {
+		const unsigned long long lo = m1 < b1 ? m1 : b1, hi = m1 < b1 ? b1 : m1, r2 = m2 < b2 ? m2 : b2;
+		m1							= lo;
+		m2							= hi < r2 ? hi : r2;
+	}
+};
+
 // 6. Per token exclusive prefixes (nodes, decoded bytes, brackets) and every
 // event against the hook's rules (bench_parse.cu:86-194), in one pass over a
 // 4096-token group per block: the check needs the bracket prefix the scan
-// has just produced.  Top-level values are flagged: every one after the
-// first is rejected (the hook's top_done), resolved by k_dom_top2.
+// has just produced.  Top-level value events are collected: every one after
+// the first is rejected (the hook's top_done), so the final step needs the
+// two smallest; each group keeps its own two.
 __device__ __forceinline__ void check_token(const DomArgs& a, uint64_t t, uint32_t w, uint32_t brk,
-											unsigned long long& fk, unsigned long long& top) {
+											unsigned long long& fk, Two& top) {
 	// the innermost open container before structural t
 	int64_t		   cont = -1;
 	const int64_t  b	= (int64_t)brk - 1;
@@ -697,11 +714,10 @@ __device__ __forceinline__ void check_token(const DomArgs& a, uint64_t t, uint32
 		if (!ok)
 			reject(es, 0u);
 	}
-	a.topflag[t] = flag;
-	if (flag) {
-		const unsigned long long v = (flag & 1u) ? 2ull * t : 2ull * t + 1ull;
-		top						   = v < top ? v : top;
-	}
+	if (flag & 1u)
+		top.add(2ull * t);
+	if (flag & 2u)
+		top.add(2ull * t + 1ull);
 }
 
 __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
@@ -711,7 +727,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 	uint32_t			carry[3];
 #pragma unroll
 	for (int f = 0; f < 3; ++f) carry[f] = a.tblk[blockIdx.x * 3 + f];
-	unsigned long long fk = kDomNone, top = kDomNone;
+	unsigned long long fk  = kDomNone;
+	Two				   top = {kDomNone, kDomNone};	// a thread's tokens come in increasing order
 	for (int r = 0; r < kDomRows; ++r) {
 		const uint64_t t  = base + (uint64_t)r * kDomBlock + threadIdx.x;
 		const bool	   in = t < ntok;
@@ -734,34 +751,42 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
 	}
 	wave_atomic_min(&a.ds->fail_key, fk);
-	wave_atomic_min(&a.ds->e1, top);
-}
-
-// 7. The second top-level value event.
-__global__ __launch_bounds__(kDomBlock) void k_dom_top2(DomArgs a) {
-	const uint64_t			 ntok = a.n_struct + 1;
-	const unsigned long long e1	  = a.ds->e1;
-	{
-		const uint64_t	   t = (uint64_t)blockIdx.x * kDomBlock + threadIdx.x;
-		unsigned long long v = kDomNone;
-		if (t < ntok && e1 != kDomNone) {
-			const uint32_t f = a.topflag[t];
-			if ((f & 1u) && 2ull * t > e1)
-				v = 2ull * t;
-			else if ((f & 2u) && 2ull * t + 1ull > e1)
-				v = 2ull * t + 1ull;
-		}
-		wave_atomic_min(&a.ds->e2, v);
+	__shared__ unsigned long long s_top[kDomBlock / 64][2];
+#pragma unroll
+	for (int o = 32; o >= 1; o >>= 1) top.merge(__shfl_xor(top.m1, o, 64), __shfl_xor(top.m2, o, 64));
+	if ((threadIdx.x & 63) == 0) {
+		s_top[threadIdx.x >> 6][0] = top.m1;
+		s_top[threadIdx.x >> 6][1] = top.m2;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < kDomBlock / 64; ++q) top.merge(s_top[q][0], s_top[q][1]);
+		a.top2[2u * blockIdx.x]		 = top.m1;
+		a.top2[2u * blockIdx.x + 1u] = top.m2;
 	}
 }
 
-// 8. Where the hook stopped, how many nodes and decoded bytes it produced.
-__global__ void k_dom_final(DomArgs a) {
+// 7. Where the hook stopped, how many nodes and decoded bytes it produced:
+// the first two top-level value events over every group, then one thread.
+__global__ __launch_bounds__(kDomBlock) void k_dom_final(DomArgs a) {
+	__shared__ unsigned long long s_top[kDomBlock / 64][2];
+	Two							  top = {kDomNone, kDomNone};
+	for (uint64_t g = threadIdx.x; g < a.n_tblk; g += kDomBlock) top.merge(a.top2[2u * g], a.top2[2u * g + 1u]);
+#pragma unroll
+	for (int o = 32; o >= 1; o >>= 1) top.merge(__shfl_xor(top.m1, o, 64), __shfl_xor(top.m2, o, 64));
+	if ((threadIdx.x & 63) == 0) {
+		s_top[threadIdx.x >> 6][0] = top.m1;
+		s_top[threadIdx.x >> 6][1] = top.m2;
+	}
+	__syncthreads();
 	if (threadIdx.x != 0)
 		return;
-	DomSummary*				 ds = a.ds;
+	for (int q = 1; q < kDomBlock / 64; ++q) top.merge(s_top[q][0], s_top[q][1]);
+	DomSummary* ds = a.ds;
+	ds->e1		   = top.m1;
+	ds->e2		   = top.m2;
 	unsigned long long		 fk = ds->fail_key;
-	const unsigned long long e2 = ds->e2;
+	const unsigned long long e2 = top.m2;
 	if (e2 != kDomNone && (e2 << 1) < fk)
 		fk = e2 << 1;  // a second top-level value is rejected before it is parsed
 	if (fk == kDomNone) {
@@ -790,7 +815,7 @@ __global__ void k_dom_final(DomArgs a) {
 	}
 }
 
-// 9. The tape, in document order, and the decoded string bytes.
+// 8. The tape, in document order, and the decoded string bytes.
 // Node records in document order; strings' decoded bytes afterwards, in work
 // order (bytes to copy, or gap bytes to decode again).  A block's strings own
 // one contiguous run of the arena: the lanes write their bytes into LDS and
@@ -923,7 +948,6 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 
 hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 	const uint64_t ntok = a.n_struct + 1;
-	const unsigned gt	= (unsigned)((ntok + kDomBlock - 1) / kDomBlock);	// one token per thread
 	const unsigned gs	= (unsigned)((ntok + kSortTok - 1) / kSortTok);		// kSortTok tokens per block
 	hipLaunchKernelGGL(k_dom_reset, dim3(1), dim3(64), 0, s, a.ds);
 	hipError_t e = launch_fill_i32(reinterpret_cast<int32_t*>(a.tblk), a.n_tblk * 3, 0, s);
@@ -937,8 +961,7 @@ hipError_t launch_json_tape(const DomArgs& a, hipStream_t s) {
 		hipLaunchKernelGGL(k_dom_parent, dim3((unsigned)a.n_hblk), dim3(kDomBlock), 0, s, a);
 	}
 	hipLaunchKernelGGL(k_dom_apply_check, dim3((unsigned)a.n_tblk), dim3(kDomBlock), 0, s, a);
-	hipLaunchKernelGGL(k_dom_top2, dim3(gt), dim3(kDomBlock), 0, s, a);
-	hipLaunchKernelGGL(k_dom_final, dim3(1), dim3(64), 0, s, a);
+	hipLaunchKernelGGL(k_dom_final, dim3(1), dim3(kDomBlock), 0, s, a);
 	hipLaunchKernelGGL(k_dom_tape, dim3(gs), dim3(kDomBlock), 0, s, a);
 	return hipGetLastError();
 }

@@ -254,7 +254,8 @@ struct DomArgs {
 	uint32_t* nodes_x;	// exclusive prefixes: nodes, decoded bytes, brackets
 	uint32_t* arena_x;
 	uint32_t* brk_x;
-	uint8_t*  topflag;
+	// per 4096-token group: its two smallest top-level value events
+	unsigned long long* top2;
 	uint32_t* tblk;	  // per token block: sums, then exclusive prefixes (3 each)
 	uint64_t  n_tblk;
 	// per sorted bracket position / per bracket

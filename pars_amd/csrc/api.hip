@@ -1176,9 +1176,11 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	const uint64_t ns = res.n_struct, nb = res.n_brackets, ntok = ns + 1;
 	const uint64_t n_tblk = (ntok + kDomTile - 1) / kDomTile, n_hblk = std::max<uint64_t>((nb + kDomTile - 1) / kDomTile, 1);
 	const uint64_t cap_nodes = std::min<uint64_t>(n, 2 * ns + 1);
-	for (DevBuf* b : {&sc->dom_tinfo, &sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec, &sc->dom_nx, &sc->dom_ax, &sc->dom_bx})
+	for (DevBuf* b : {&sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec, &sc->dom_nx, &sc->dom_ax, &sc->dom_bx})
 		if ((r = b->ensure(ntok * 4)))
 			return r;
+	if ((r = sc->dom_tinfo.ensure(ntok)))
+		return r;
 	if ((r = sc->dom_top.ensure(n_tblk * 2 * sizeof(unsigned long long))) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
 		(r = sc->dom_hblk.ensure(n_hblk * 4)) || (r = sc->dom_parent.ensure(std::max<uint64_t>(nb, 1) * 4)) ||
 		(r = sc->dom_sum.ensure(sizeof(DomSummary))) || (r = sc->tape_kind.ensure(cap_nodes)) ||
@@ -1199,7 +1201,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	a.bkey		= sc->bkey.as<int32_t>();
 	a.bsidx		= sc->bsidx.as<uint32_t>();
 	a.order		= ((h.sort_passes - 1) & 1) ? sc->order1.as<uint32_t>() : sc->order0.as<uint32_t>();
-	a.tinfo		= sc->dom_tinfo.as<uint32_t>();
+	a.tinfo		= sc->dom_tinfo.as<uint8_t>();
 	a.gbeg		= sc->dom_gbeg.as<uint32_t>();
 	a.gend		= sc->dom_gend.as<uint32_t>();
 	a.gdec		= sc->dom_gdec.as<uint32_t>();

@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 		if (t < ntok) {
 			const uint32_t sym = t < a.n_struct ? sym_class(a.data[a.positions[t]]) : SYM_END;
 			const uint32_t w   = sym | r_gap[j];
-			a.tinfo[t]		   = w;
+			a.tinfo[t]		   = (uint8_t)w;
 			a.gbeg[t]		   = r_beg[j];
 			a.gend[t]		   = r_end[j];
 			a.gdec[t]		   = r_dec[j];

@@ -247,7 +247,7 @@ struct DomArgs {
 	const uint32_t* bsidx;
 	const uint32_t* order;	   // R6: bracket at each sorted position
 	// per token
-	uint32_t* tinfo;
+	uint8_t*  tinfo;	// symbol class, gap kind, gap parsed whole (7 bits)
 	uint32_t* gbeg;
 	uint32_t* gend;
 	uint32_t* gdec;

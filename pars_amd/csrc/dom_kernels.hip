@@ -291,15 +291,21 @@ __host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, 
 		const bool lead = s == S_LEAD && t != S_LEAD;
 		g.beg			= lead ? (uint32_t)j : g.beg;
 		first			= lead ? k : first;
-		// \u units: four hex digits accumulate from S_U1 on; the fourth
-		// (S_U4 → S_STR) decodes to 1-3 UTF-8 bytes
-		const uint32_t hv = c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u);
-		const uint32_t vn = (v << 4) | hv;
-		const bool	   u4 = s == S_U4 && t == S_STR;
-		const uint32_t nu = vn < 0x80u ? 1u : vn < 0x800u ? 2u : 3u;
 		// decoded bytes: a plain string byte or an escape letter (one), a \u
-		// unit (nu)
-		const uint32_t nw = u4 ? nu : (t == S_STR && (s == S_STR || s == S_ESC)) ? 1u : 0u;
+		// unit (1-3 UTF-8 bytes)
+		uint32_t nw = (t == S_STR && (s == S_STR || s == S_ESC)) ? 1u : 0u;
+		// \u units: four hex digits accumulate from S_U1 on; the fourth
+		// (S_U4 → S_STR) decodes to 1-3 UTF-8 bytes.  Rare: a wave skips this
+		// when none of its lanes is inside one.
+		uint32_t vn = 0;
+		bool	 u4 = false;
+		if (s >= S_U1 && s <= S_U4) {
+			vn = (v << 4) | (c <= '9' ? c - '0' : (c | 0x20u) - ('a' - 10u));
+			u4 = s == S_U4 && t == S_STR;
+			if (u4)
+				nw = vn < 0x80u ? 1u : vn < 0x800u ? 2u : 3u;
+			v = vn;
+		}
 		if constexpr (EMIT) {
 			uint32_t w = s == S_ESC ? (uint32_t)L.esc[c] : c;
 			if (u4) {
@@ -312,7 +318,7 @@ __host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, 
 			}
 			for (uint32_t q = 0; q < nw; ++q) out[dec + q] = (uint8_t)(w >> (8 * q));
 		}
-		v = t == S_U1 ? 0u : (s >= S_U1 && s <= S_U3) ? vn : v;
+		v = t == S_U1 ? 0u : v;
 		dec += nw;
 		const bool fin = (x & kDone) != 0u;
 		done		   = done || fin;

@@ -82,18 +82,22 @@ struct Gap {
 // loads serve the buffer's edges, where a window would reach outside it.
 struct Bytes {
 	const uint8_t* d;
-	uint64_t	   lo, hi;	// windows [a, a + 32) with lo <= a and a + 32 <= hi lie in the buffer
-	uint64_t	   base;
+	uint32_t	   lo, lim;	 // windows [a, a + 32) with lo <= a <= lim lie in the buffer
+	uint32_t	   base;	 // the window's first index
 	uint4		   v0, v1;
 
-	__host__ __device__ Bytes(const uint8_t* d_, uint64_t n_)
-		: d(d_), lo((16u - ((uintptr_t)d_ & 15u)) & 15u), hi(n_), base(1ull << 63), v0(make_uint4(0, 0, 0, 0)),
-		  v1(make_uint4(0, 0, 0, 0)) {}  // no window yet: i - base >= 32 for every i < 2^63 - 32
+	// Indexes are 32-bit (documents are under 4 GiB).  `from` is the first
+	// index read: every read is at or after it, so base = from - 64 (mod 2^32)
+	// stands for "no window yet" (i - base >= 64 for every such i).
+	__host__ __device__ Bytes(const uint8_t* d_, uint64_t n_, uint32_t from)
+		: d(d_), lo(n_ >= 32u ? (16u - (uint32_t)((uintptr_t)d_ & 15u)) & 15u : ~0u),
+		  lim(n_ >= 32u ? (uint32_t)(n_ - 32u) : 0u), base(from - 64u), v0(make_uint4(0, 0, 0, 0)),
+		  v1(make_uint4(0, 0, 0, 0)) {}
 
-	__host__ __device__ __forceinline__ uint32_t operator[](uint64_t i) {
-		if (i - base >= 32u) {	// outside the window (i < base wraps)
-			const uint64_t a = i - (uint64_t)((uintptr_t)(d + i) & 15u);
-			if (i < lo || a + 32u > hi)
+	__host__ __device__ __forceinline__ uint32_t operator[](uint32_t i) {
+		if (i - base >= 32u) {	// outside the window
+			const uint32_t a = i - (uint32_t)(((uintptr_t)d + i) & 15u);
+			if (i < lo || a > lim)
 				return d[i];
 			base = a;
 			v0	 = *reinterpret_cast<const uint4*>(d + a);
@@ -101,7 +105,7 @@ struct Bytes {
 		}
 		// selects of values, not of fields (a select of fields can become a
 		// select of addresses, which keeps the window in scratch)
-		const uint32_t k  = (uint32_t)(i - base);
+		const uint32_t k  = i - base;
 		const uint64_t h0 = (k & 8u) ? ((uint64_t)v0.w << 32 | v0.z) : ((uint64_t)v0.y << 32 | v0.x);
 		const uint64_t h1 = (k & 8u) ? ((uint64_t)v1.w << 32 | v1.z) : ((uint64_t)v1.y << 32 | v1.x);
 		return (uint32_t)(((k & 16u) ? h1 : h0) >> (8u * (k & 7u))) & 0xFFu;
@@ -276,20 +280,20 @@ __device__ __forceinline__ void load_gap_tables(GapTables& L) {
 // past its last byte (0 for a string that did not close), and the decoded
 // bytes a string mixed into the hash.  EMIT writes those bytes to `out`.
 template<bool EMIT>
-__host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, uint64_t b, uint64_t e, uint8_t* out) {
+__host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, uint32_t b, uint32_t e, uint8_t* out) {
 	Gap		 g {G_EMPTY, 0u, 0u, 0u, 0u};
 	uint32_t s = S_LEAD, dec = 0, v = 0, first = K_OTHER;
 	bool	 done = false;
-	uint64_t j	  = b;
-	for (; j <= e && s < S_BADVAL; ++j) {
-		const uint32_t c = j < e ? d[j] : 0u;
-		const uint32_t k = j < e ? (uint32_t)L.cls[c] : (uint32_t)K_END;
+	for (uint32_t j = b;; ++j) {  // through j == e, the gap's end class
+		const bool	   end = j == e;
+		const uint32_t c   = end ? 0u : d[j];
+		const uint32_t k   = end ? (uint32_t)K_END : (uint32_t)L.cls[c];
 		const uint32_t x = L.tr[s * K_N + k];
 		const uint32_t t = x & 31u;
 		// every step computes every case and selects (no branches: the lanes
 		// of a wave are in different states)
 		const bool lead = s == S_LEAD && t != S_LEAD;
-		g.beg			= lead ? (uint32_t)j : g.beg;
+		g.beg			= lead ? j : g.beg;
 		first			= lead ? k : first;
 		// decoded bytes: a plain string byte or an escape letter (one), a \u
 		// unit (1-3 UTF-8 bytes)
@@ -322,8 +326,10 @@ __host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, 
 		dec += nw;
 		const bool fin = (x & kDone) != 0u;
 		done		   = done || fin;
-		g.end		   = fin ? (uint32_t)((x & kIncl) ? j + 1 : j) : g.end;
+		g.end		   = fin ? ((x & kIncl) ? j + 1u : j) : g.end;
 		s			   = t;
+		if (end || s >= S_BADVAL)
+			break;
 	}
 	if (first == K_OTHER && s == S_LEAD)
 		return g;  // whitespace only
@@ -513,7 +519,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 	for (int i = 0; i < kSortPer; ++i) {
 		const uint32_t j = perm[i * kDomBlock + threadIdx.x];
 		if (base + j < ntok) {
-			Bytes	  rd(a.data, a.n);
+			Bytes	  rd(a.data, a.n, r_beg[j]);
 			const Gap g = parse_gap<false>(L, rd, r_beg[j], r_end[j], nullptr);
 			r_beg[j]	= g.beg;
 			r_end[j]	= g.end;
@@ -919,15 +925,16 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 				continue;
 			const uint64_t t   = base + j;
 			uint8_t*	   out = dst(a.arena_x[t]);
-			Bytes		   rd(a.data, a.n);
 			if (how == H_COPY) {
-				const uint64_t src = (uint64_t)a.gbeg[t] + 1u;
+				const uint32_t src = a.gbeg[t] + 1u;
 				const uint32_t dec = a.gdec[t];
+				Bytes		   rd(a.data, a.n, src);
 				for (uint32_t q = 0; q < dec; ++q) out[q] = (uint8_t)rd[src + q];
 			} else {
 				uint64_t b, e;
 				gap_bounds(a, t, b, e);
-				(void)parse_gap<true>(L, rd, b, e, out);
+				Bytes rd(a.data, a.n, (uint32_t)b);
+				(void)parse_gap<true>(L, rd, (uint32_t)b, (uint32_t)e, out);
 			}
 		}
 	};

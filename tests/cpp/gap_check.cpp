@@ -42,16 +42,17 @@ int main(int argc, char** argv) {
 		const size_t			   shift = off % 16;
 		std::vector<unsigned char> buf(len + 64, 'Z');
 		std::memcpy(&buf[shift], &in[off], len);
-		Bytes	  rd(buf.data(), buf.size()), rd2(buf.data(), buf.size());
-		const Gap g	 = parse_gap<false>(T, rd, shift, shift + len, nullptr);
-		const Gap g2 = parse_gap<true>(T, rd2, shift, shift + len, out.data());
+		const uint32_t b = (uint32_t)shift, e = (uint32_t)(shift + len);
+		Bytes		   rd(buf.data(), buf.size(), b), rd2(buf.data(), buf.size(), b);
+		const Gap	   g  = parse_gap<false>(T, rd, b, e, nullptr);
+		const Gap	   g2 = parse_gap<true>(T, rd2, b, e, out.data());
 		if (g.dec != g2.dec || g.kind != g2.kind || g.end != g2.end || g.ok != g2.ok)
 			return 3;
 		// the gap alone in a buffer of its own size at an odd address: every
 		// read near the buffer's edges goes through the byte path
 		std::vector<unsigned char> tight(len + 1), out3(g2.dec + 1);
 		std::memcpy(&tight[1], &in[off], len);
-		Bytes	  rd3(tight.data() + 1, len);
+		Bytes	  rd3(tight.data() + 1, len, 0u);
 		const Gap g3 = parse_gap<true>(T, rd3, 0, len, out3.data());
 		if (g3.dec != g2.dec || g3.kind != g2.kind || g3.ok != g2.ok || (g3.end ? g3.end + shift : 0) != g2.end ||
 			(g3.kind && g3.beg + shift != g2.beg) || std::memcmp(out3.data(), out.data(), g2.dec) != 0)

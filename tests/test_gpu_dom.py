@@ -123,6 +123,25 @@ def test_corrupted_documents(torch_cuda):
         check(torch_cuda, d, f"corrupt at {i}")
 
 
+def test_string_runs_staged_and_direct(torch_cuda):
+    """Runs of short strings (a block's decoded bytes fit the LDS staging
+    buffer) alternate with runs of long ones (they do not: written directly),
+    so blocks of both kinds meet and share 16-byte chunks of the arena; escapes
+    make some strings decode again rather than copy."""
+    rng = random.Random(14)
+    pieces = ["a", "b", "z", " ", "0", "\\n", '\\"', "\\\\", "\\u00e9", "\\u4e2d", "\\/"]
+    items = []
+    for seg in range(8):
+        for _ in range(2500):
+            n = rng.randint(30, 300) if seg % 2 else rng.randint(0, 10)
+            items.append('"' + "".join(rng.choice(pieces) if rng.random() < 0.1 else "q" for _ in range(n)) + '"')
+    d = ("[" + ",".join(items) + "]").encode()
+    say(f"[gpu] tape string runs n={len(d)}")
+    got = check(torch_cuda, d)
+    assert got["ok"] and got["nodes"] == 1 + len(items)
+    check(torch_cuda, d[: len(d) * 2 // 3], "cut inside a run")  # the arena ends inside a block
+
+
 def test_repeated_calls_and_refusal(torch_cuda):
     a, b = W.json(31, 300_000, 8).tobytes(), b'{"x": [1, 2, {"y": "z"}]}'
     check(torch_cuda, a)

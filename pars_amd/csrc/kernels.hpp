@@ -187,7 +187,8 @@ struct PairArgs {
 	int32_t*	   pair;	  // CPU convention, per bracket index (optional)
 	int32_t*	   pair_pos;  // device convention, per structural index (optional)
 	uint32_t*	   close;	  // per opening bracket: the structural index of the next bracket at its
-							  // key in sorted order, ~0u if none (optional; one document)
+							  // key in sorted order, ~0u if none (optional; one document).  When set,
+							  // the pair pass writes only this: no pairs, no mismatch report
 	uint32_t	   out_base;  // structural index base of pair_pos entries
 	// batch mode: sort by (document, key); pair within a document; indices in
 	// the outputs are document-local

@@ -260,6 +260,21 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 			s0 = a.st_start[d];
 		}
 		const uint32_t p = a.rank[i] - b0, q = p ^ 1u;
+		if (a.close) {	// partners only (the tape reads no pairs and no mismatch report)
+			if (P->br_flags[a.bchar[i]] & 1u) {
+				// the bracket after this open in sorted order, when it has the same
+				// key: its close (after an open at key d only a close brings the depth
+				// back to d, so the next bracket at key d in document order closes)
+				uint32_t tc = ~0u;
+				if (p + 1u < n) {
+					const uint32_t y = order[b0 + p + 1u];
+					if (a.bkey[y] == a.bkey[i])
+						tc = a.bsidx[y];
+				}
+				a.close[i] = tc;
+			}
+			continue;
+		}
 		int32_t		   partner = -1;
 		if (q < n) {
 			const uint32_t j  = order[b0 + q];
@@ -278,18 +293,6 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 			a.pair[i] = partner >= 0 ? partner - (int32_t)b0 : -1;
 		if (a.pair_pos)
 			a.pair_pos[a.bsidx[i]] = partner >= 0 ? (int32_t)(a.bsidx[partner] - s0) : -1;
-		if (a.close && (P->br_flags[a.bchar[i]] & 1u)) {
-			// the bracket after this open in sorted order, when it has the same
-			// key: its close (after an open at key d only a close brings the depth
-			// back to d, so the next bracket at key d in document order closes)
-			uint32_t tc = ~0u;
-			if (p + 1u < n) {
-				const uint32_t y = (p & 1u) ? order[b0 + p + 1u] : order[b0 + q];
-				if (a.bkey[y] == a.bkey[i])
-					tc = a.bsidx[y];
-			}
-			a.close[i] = tc;
-		}
 	}
 	if (__ballot(bad) && lane_id() == 0)
 		atomicOr(&a.sum->mismatched, 1u);

@@ -32,9 +32,9 @@ def scanner() -> Scanner:
     return _sc["json"]
 
 
-def gpu_tape(torch, d: bytes) -> dict:
+def gpu_tape(torch, d: bytes, offset: int = 0) -> dict:
     arr = np.frombuffer(d, np.uint8)
-    t, ptr = to_device(torch, arr)
+    t, ptr = to_device(torch, arr, offset)
     r = scanner().json_tape(ptr, arr.size, torch.cuda.current_stream().cuda_stream)
     n = r.n_nodes
     out = {"ok": r.ok, "fail_event": r.fail_event, "nodes": n,
@@ -45,9 +45,9 @@ def gpu_tape(torch, d: bytes) -> dict:
     return out
 
 
-def check(torch, d: bytes, tag: str = ""):
+def check(torch, d: bytes, tag: str = "", offset: int = 0):
     exp = J.json_tape(d, O.scan(PLAN, np.frombuffer(d, np.uint8))["positions"])
-    got = gpu_tape(torch, d)
+    got = gpu_tape(torch, d, offset)
     tag = tag or repr(d[:60])
     assert (got["ok"], got["fail_event"], got["nodes"]) == (exp["ok"], exp["fail_event"], exp["nodes"]), tag
     assert np.array_equal(got["kind"], exp["kind"]), tag
@@ -140,6 +140,17 @@ def test_string_runs_staged_and_direct(torch_cuda):
     got = check(torch_cuda, d)
     assert got["ok"] and got["nodes"] == 1 + len(items)
     check(torch_cuda, d[: len(d) * 2 // 3], "cut inside a run")  # the arena ends inside a block
+
+
+def test_unaligned_inputs(torch_cuda):
+    """Input pointers off 16-byte alignment: strings at the buffer's first and
+    last bytes are read through the byte path, the rest in aligned chunks."""
+    docs = (b'"ab\\ncd efgh ijkl mnop"', b'"' + b"x" * 40 + b'"', b'["s", "tu", "' + b"v" * 30 + b'"]',
+            W.json(41, 50_000, 8).tobytes())
+    for off in (1, 3, 7, 13):
+        for d in docs:
+            say(f"[gpu] tape unaligned offset={off} n={len(d)}")
+            check(torch_cuda, d, f"off={off} {d[:20]!r}", offset=off)
 
 
 def test_repeated_calls_and_refusal(torch_cuda):

@@ -125,7 +125,7 @@ struct pars_scanner {
 	std::vector<uintptr_t>	gkey;  // shape and buffers the captured graph was made for
 	// the JSON tape: per-token and per-bracket scratch, the tape, its summary
 	int			force_radix = 0;  // R6 through the radix passes (their sorted order is kept)
-	DevBuf		dom_tinfo, dom_gbeg, dom_gend, dom_gdec, dom_nx, dom_ax, dom_bx, dom_top, dom_tblk, dom_hblk,
+	DevBuf		dom_tinfo, dom_gbeg, dom_gend, dom_gdec, dom_sub, dom_top, dom_tblk, dom_hblk,
 		dom_parent, dom_close, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
 	DomSummary* h_dom = nullptr;  // pinned
 	size_t		h_dom_cap = 0;
@@ -137,7 +137,7 @@ struct pars_scanner {
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
 						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
 						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &dom_tinfo, &dom_gbeg, &dom_gend,
-						  &dom_gdec, &dom_nx, &dom_ax, &dom_bx, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum,
+						  &dom_gdec, &dom_sub, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum,
 						  &tape_kind, &tape_beg, &tape_end, &tape_soff, &tape_arena})
 			b->release();
 		if (h_dom)
@@ -1176,9 +1176,11 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	const uint64_t ns = res.n_struct, nb = res.n_brackets, ntok = ns + 1;
 	const uint64_t n_tblk = (ntok + kDomTile - 1) / kDomTile, n_hblk = std::max<uint64_t>((nb + kDomTile - 1) / kDomTile, 1);
 	const uint64_t cap_nodes = std::min<uint64_t>(n, 2 * ns + 1);
-	for (DevBuf* b : {&sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec, &sc->dom_nx, &sc->dom_ax, &sc->dom_bx})
+	for (DevBuf* b : {&sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec})
 		if ((r = b->ensure(ntok * 4)))
 			return r;
+	if ((r = sc->dom_sub.ensure(((ntok + kDomSub - 1) / kDomSub) * 3 * 4)))
+		return r;
 	if ((r = sc->dom_tinfo.ensure(ntok)))
 		return r;
 	if ((r = sc->dom_top.ensure(n_tblk * 2 * sizeof(unsigned long long))) || (r = sc->dom_tblk.ensure(n_tblk * 3 * 4)) ||
@@ -1205,9 +1207,7 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	a.gbeg		= sc->dom_gbeg.as<uint32_t>();
 	a.gend		= sc->dom_gend.as<uint32_t>();
 	a.gdec		= sc->dom_gdec.as<uint32_t>();
-	a.nodes_x	= sc->dom_nx.as<uint32_t>();
-	a.arena_x	= sc->dom_ax.as<uint32_t>();
-	a.brk_x		= sc->dom_bx.as<uint32_t>();
+	a.sub		= sc->dom_sub.as<uint32_t>();
 	a.top2		= sc->dom_top.as<unsigned long long>();
 	a.tblk		= sc->dom_tblk.as<uint32_t>();
 	a.n_tblk	= n_tblk;

@@ -453,7 +453,7 @@ __global__ void k_dom_reset(DomSummary* ds) {
 // in order of their work (gap bytes to walk), so the 64 lanes of a wave walk
 // gaps of about one length: a wave lasts as long as its longest gap, and in
 // document order that is ~4.5× the mean on JSON.
-constexpr int kSortTok = 1024;
+constexpr int kSortTok = (int)kDomSub;
 constexpr int kSortPer = kSortTok / kDomBlock;
 constexpr int kBins	   = 64;
 static_assert(kDomTile % kSortTok == 0, "a block's tokens lie in one sum group");
@@ -744,7 +744,10 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 	for (int r = 0; r < kDomRows; ++r) {
 		const uint64_t t  = base + (uint64_t)r * kDomBlock + threadIdx.x;
 		const bool	   in = t < ntok;
-		uint32_t	   v[3] = {0u, 0u, 0u}, w = 0;
+		if (r % kSortPer == 0 && threadIdx.x == 0 && in)  // a kDomSub-token block starts here
+#pragma unroll
+			for (int f = 0; f < 3; ++f) a.sub[(t / kDomSub) * 3 + f] = carry[f];
+		uint32_t v[3] = {0u, 0u, 0u}, w = 0;
 		if (in) {
 			w	 = a.tinfo[t];
 			v[0] = t_nodes(w);
@@ -753,12 +756,8 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 		}
 		uint32_t tot[3];
 		block_excl3(v, tot, s);
-		if (in) {
-			a.nodes_x[t] = carry[0] + v[0];
-			a.arena_x[t] = carry[1] + v[1];
-			a.brk_x[t]	 = carry[2] + v[2];
+		if (in)
 			check_token(a, t, w, carry[2] + v[2], fk, top);
-		}
 #pragma unroll
 		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
 	}
@@ -791,39 +790,68 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_final(DomArgs a) {
 		s_top[threadIdx.x >> 6][1] = top.m2;
 	}
 	__syncthreads();
+	__shared__ unsigned long long s_fk;
+	__shared__ uint64_t			  s_pre[kDomBlock / 64][2];
+	DomSummary*					  ds = a.ds;
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < kDomBlock / 64; ++q) top.merge(s_top[q][0], s_top[q][1]);
+		ds->e1						= top.m1;
+		ds->e2						= top.m2;
+		unsigned long long		 fk = ds->fail_key;
+		const unsigned long long e2 = top.m2;
+		if (e2 != kDomNone && (e2 << 1) < fk)
+			fk = e2 << 1;  // a second top-level value is rejected before it is parsed
+		s_fk = fk;
+	}
+	__syncthreads();
+	// the token the hook stopped at (the last one when every event is
+	// accepted), and the node and decoded-byte prefixes before it: its
+	// block's entry in the table plus the tokens of that block before it
+	const unsigned long long fk = s_fk;
+	const uint64_t			 t	= fk == kDomNone ? a.n_struct : (fk >> 2);
+	const uint64_t			 t0 = (t / kDomSub) * kDomSub;
+	uint64_t				 pn = 0, pd = 0;
+	for (uint64_t u = t0 + threadIdx.x; u < t; u += kDomBlock) {
+		pn += t_nodes(a.tinfo[u]);
+		pd += a.gdec[u];
+	}
+#pragma unroll
+	for (int o = 32; o >= 1; o >>= 1) {
+		pn += __shfl_xor(pn, o, 64);
+		pd += __shfl_xor(pd, o, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		s_pre[threadIdx.x >> 6][0] = pn;
+		s_pre[threadIdx.x >> 6][1] = pd;
+	}
+	__syncthreads();
 	if (threadIdx.x != 0)
 		return;
-	for (int q = 1; q < kDomBlock / 64; ++q) top.merge(s_top[q][0], s_top[q][1]);
-	DomSummary* ds = a.ds;
-	ds->e1		   = top.m1;
-	ds->e2		   = top.m2;
-	unsigned long long		 fk = ds->fail_key;
-	const unsigned long long e2 = top.m2;
-	if (e2 != kDomNone && (e2 << 1) < fk)
-		fk = e2 << 1;  // a second top-level value is rejected before it is parsed
-	if (fk == kDomNone) {
-		const uint64_t t = a.n_struct;	// the last token: every event accepted
-		const uint32_t w = a.tinfo[t];
-		ds->fail_event	 = kDomNone;
-		ds->partial		 = 0;
-		ds->n_nodes		 = (uint64_t)a.nodes_x[t] + t_nodes(w);
-		ds->arena_bytes	 = (uint64_t)a.arena_x[t] + a.gdec[t];
-		ds->ok			 = a.sum->last_bal == 0 && ds->e1 != kDomNone;  // bench_parse.cu:196-201
+	uint64_t nodes_x = a.sub[(t / kDomSub) * 3], arena_x = a.sub[(t / kDomSub) * 3 + 1];
+	for (int q = 0; q < kDomBlock / 64; ++q) {
+		nodes_x += s_pre[q][0];
+		arena_x += s_pre[q][1];
+	}
+	const uint32_t w = a.tinfo[t];
+	if (fk == kDomNone) {  // the last token: every event accepted
+		ds->fail_event	= kDomNone;
+		ds->partial		= 0;
+		ds->n_nodes		= nodes_x + t_nodes(w);
+		ds->arena_bytes = arena_x + a.gdec[t];
+		ds->ok			= a.sum->last_bal == 0 && ds->e1 != kDomNone;  // bench_parse.cu:196-201
 		return;
 	}
-	const unsigned long long ev = fk >> 1;
+	const unsigned long long ev		 = fk >> 1;
 	const uint32_t			 partial = (uint32_t)(fk & 1u);
-	const uint64_t			 t		 = ev >> 1;
-	const uint32_t			 w		 = a.tinfo[t];
 	ds->fail_event					 = ev;
 	ds->partial						 = partial;
 	ds->ok							 = 0;
 	if ((ev & 1u) == 0) {  // the gap before structural t
-		ds->n_nodes		= (uint64_t)a.nodes_x[t] + partial;
-		ds->arena_bytes = (uint64_t)a.arena_x[t] + (partial ? a.gdec[t] : 0u);
+		ds->n_nodes		= nodes_x + partial;
+		ds->arena_bytes = arena_x + (partial ? a.gdec[t] : 0u);
 	} else {  // structural t, after its accepted gap
-		ds->n_nodes		= (uint64_t)a.nodes_x[t] + (t_gap(w) >= G_STR ? 1u : 0u);
-		ds->arena_bytes = (uint64_t)a.arena_x[t] + a.gdec[t];
+		ds->n_nodes		= nodes_x + (t_gap(w) >= G_STR ? 1u : 0u);
+		ds->arena_bytes = arena_x + a.gdec[t];
 	}
 }
 
@@ -843,35 +871,48 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	__shared__ uint32_t	 bins[kBins];
 	__shared__ uint8_t	 s_how[kSortTok];
 	__shared__ __align__(16) uint8_t stage[kStage];
-	__shared__ uint64_t	 s_rng[2];
+	__shared__ uint32_t	 s_arx[kSortTok];  // each token's decoded-byte prefix
+	__shared__ uint32_t	 s3[kDomBlock / 64][3];
 	load_gap_tables(L);
 	const uint64_t			 ntok = a.n_struct + 1;
 	const unsigned long long fe	  = a.ds->fail_event;  // kDomNone: none
 	const uint32_t			 part = a.ds->partial;
 	const uint64_t			 base = (uint64_t)blockIdx.x * kSortTok;
-	uint32_t				 work[kSortPer];
+	uint32_t				 carry[3];	// node, decoded-byte and bracket prefixes, row by row
+#pragma unroll
+	for (int f = 0; f < 3; ++f) carry[f] = a.sub[blockIdx.x * 3u + f];
+	const uint64_t a0 = carry[1];  // the block's run of the arena starts here
+	uint32_t	   work[kSortPer];
 #pragma unroll
 	for (int i = 0; i < kSortPer; ++i) {
-		const uint32_t j = (uint32_t)(i * kDomBlock) + threadIdx.x;
-		const uint64_t t = base + j;
-		work[i]			 = 0;
-		s_how[j]		 = H_NONE;
+		const uint32_t j  = (uint32_t)(i * kDomBlock) + threadIdx.x;
+		const uint64_t t  = base + j;
+		const bool	   in = t < ntok;
+		const uint32_t w  = in ? (uint32_t)a.tinfo[t] : 0u;
+		const uint32_t gd = in ? a.gdec[t] : 0u;
+		uint32_t	   v[3] = {in ? t_nodes(w) : 0u, gd, in && t_bracket(w) ? 1u : 0u}, tot[3];
+		block_excl3(v, tot, s3);  // every thread, before any skips its token
+		const uint32_t nodes_x = carry[0] + v[0], arena_x = carry[1] + v[1], brk_x = carry[2] + v[2];
+#pragma unroll
+		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
+		work[i]		 = 0;
+		s_how[j]	 = H_NONE;
+		s_arx[j]	 = arena_x;
 		const unsigned long long eg = 2ull * t;
-		if (t >= ntok || eg > fe)
+		if (!in || eg > fe)
 			continue;
-		const uint32_t w   = a.tinfo[t];
 		const uint32_t gk  = t_gap(w);
-		uint64_t	   idx = a.nodes_x[t];
+		uint64_t	   idx = nodes_x;
 		if (gk >= G_STR) {
 			if (eg < fe || part) {
 				if (idx < a.cap_nodes) {
 					a.kind[idx] = (uint8_t)gk;
 					a.nbeg[idx] = a.gbeg[t];
 					a.nend[idx] = a.gend[t];
-					a.soff[idx] = a.arena_x[t];
+					a.soff[idx] = arena_x;
 				}
-				const uint32_t dec = a.gdec[t];
-				if (gk == G_STR && (uint64_t)a.arena_x[t] + dec <= a.cap_arena) {
+				const uint32_t dec = gd;
+				if (gk == G_STR && (uint64_t)arena_x + dec <= a.cap_arena) {
 					const uint32_t gb = a.gbeg[t], ge = a.gend[t];
 					// closed, and as long decoded as between its quotes: no escape
 					// (every escape decodes shorter), the bytes as they are
@@ -892,26 +933,19 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 		if ((sym == SYM_OOPEN || sym == SYM_AOPEN) && eg + 1ull < fe && idx < a.cap_nodes) {
 			// the close that follows this open at the same key is its partner
 			// while the hook accepts it
-			const uint32_t tc  = a.close[a.brk_x[t]];
+			const uint32_t tc  = a.close[brk_x];
 			const uint32_t end = tc != ~0u && 2ull * tc + 1ull < fe ? a.positions[tc] : 0u;
 			a.kind[idx] = sym == SYM_OOPEN ? 0u : 1u;
 			a.nbeg[idx] = a.positions[t];
 			a.nend[idx] = end;
-			a.soff[idx] = a.arena_x[t] + a.gdec[t];
+			a.soff[idx] = arena_x + gd;
 		}
 	}
-	if (threadIdx.x == 0) {
-		// the block's run of the arena, cut at the tape's end (strings after
-		// the hook's first rejected event are not written)
-		const uint64_t last = (base + kSortTok < ntok ? base + kSortTok : ntok) - 1u;
-		const uint64_t a0	= a.arena_x[base];
-		const uint64_t a1	= (uint64_t)a.arena_x[last] + a.gdec[last];
-		const uint64_t lim	= a.ds->arena_bytes;
-		s_rng[0]			= a0;
-		s_rng[1]			= a1 < lim ? (a1 > a0 ? a1 : a0) : (lim > a0 ? lim : a0);
-	}
-	sort_by_work(work, perm, bins);	 // (its barriers publish s_rng)
-	const uint64_t a0 = s_rng[0], a1 = s_rng[1];
+	// the block's run of the arena, cut at the tape's end (strings after the
+	// hook's first rejected event are not written)
+	const uint64_t lim = a.ds->arena_bytes, ae = carry[1];
+	const uint64_t a1  = ae < lim ? (ae > a0 ? ae : a0) : (lim > a0 ? lim : a0);
+	sort_by_work(work, perm, bins);	 // (its barriers also publish s_how and s_arx)
 	const uint64_t c0 = a0 & ~15ull;  // stage[g - c0] holds arena byte g: LDS and HBM agree mod 16
 	const bool	   staged = a1 - c0 <= kStage;	// block-uniform
 	// one loop per destination, so the staged one stores to LDS directly
@@ -924,7 +958,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 			if (how == H_NONE)
 				continue;
 			const uint64_t t   = base + j;
-			uint8_t*	   out = dst(a.arena_x[t]);
+			uint8_t*	   out = dst(s_arx[j]);
 			if (how == H_COPY) {
 				const uint32_t src = a.gbeg[t] + 1u;
 				const uint32_t dec = a.gdec[t];

@@ -252,9 +252,9 @@ struct DomArgs {
 	uint32_t* gbeg;
 	uint32_t* gend;
 	uint32_t* gdec;
-	uint32_t* nodes_x;	// exclusive prefixes: nodes, decoded bytes, brackets
-	uint32_t* arena_x;
-	uint32_t* brk_x;
+	// per kDomSub-token block: exclusive prefixes of nodes, decoded bytes and
+	// brackets at its first token (the kernels scan within a block themselves)
+	uint32_t* sub;
 	// per 4096-token group: its two smallest top-level value events
 	unsigned long long* top2;
 	uint32_t* tblk;	  // per token block: sums, then exclusive prefixes (3 each)
@@ -275,6 +275,7 @@ struct DomArgs {
 };
 
 constexpr uint64_t kDomTile = 4096;	 // tokens or sorted brackets per block
+constexpr uint64_t kDomSub = 1024;	// tokens per block of the per-gap kernels
 
 hipError_t launch_json_tape(const DomArgs& a, hipStream_t s);
 

@@ -863,7 +863,7 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_final(DomArgs a) {
 // lanes into 64 strings cost one memory request per byte).  A run longer
 // than the staging buffer is written directly.
 enum : uint8_t { H_NONE = 0, H_COPY, H_DECODE };
-constexpr uint32_t kStage = 12288;	// staged decoded bytes per block
+constexpr uint32_t kStage = 8192;	// staged decoded bytes per block
 
 __global__ __launch_bounds__(kDomBlock) void k_dom_tape(DomArgs a) {
 	__shared__ GapTables L;

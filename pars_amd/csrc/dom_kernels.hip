@@ -548,11 +548,13 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_tok(DomArgs a) {
 	uint32_t tot[3];
 	block_excl3(sum, tot, s);
 	const uint64_t grp = base / kDomTile;
-	if (threadIdx.x == 0)
+	if (threadIdx.x == 0) {
 #pragma unroll
 		for (int f = 0; f < 3; ++f)
 			if (tot[f])
 				atomicAdd(&a.tblk[grp * 3 + f], tot[f]);
+		a.sub[blockIdx.x * 3u + 1u] = tot[1];  // this block's decoded bytes (the check pass makes it a prefix)
+	}
 }
 
 // 2. Exclusive scan of the block sums (one block; each thread a contiguous run).
@@ -739,6 +741,16 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 	uint32_t			carry[3];
 #pragma unroll
 	for (int f = 0; f < 3; ++f) carry[f] = a.tblk[blockIdx.x * 3 + f];
+	// the group's kDomSub-token blocks' decoded-byte totals (left by k_dom_tok
+	// in the table this pass turns into prefixes): the per-token counts are
+	// not read again
+	constexpr int		SUBS = (int)(kDomTile / kDomSub);
+	__shared__ uint32_t s_dsub[SUBS];
+	if (threadIdx.x < SUBS) {
+		const uint64_t sb	= (uint64_t)blockIdx.x * SUBS + threadIdx.x;
+		s_dsub[threadIdx.x] = sb * kDomSub < ntok ? a.sub[sb * 3u + 1u] : 0u;
+	}
+	__syncthreads();
 	unsigned long long fk  = kDomNone;
 	Two				   top = {kDomNone, kDomNone};	// a thread's tokens come in increasing order
 	for (int r = 0; r < kDomRows; ++r) {
@@ -751,15 +763,16 @@ __global__ __launch_bounds__(kDomBlock) void k_dom_apply_check(DomArgs a) {
 		if (in) {
 			w	 = a.tinfo[t];
 			v[0] = t_nodes(w);
-			v[1] = a.gdec[t];
 			v[2] = t_bracket(w) ? 1u : 0u;
 		}
 		uint32_t tot[3];
 		block_excl3(v, tot, s);
 		if (in)
 			check_token(a, t, w, carry[2] + v[2], fk, top);
-#pragma unroll
-		for (int f = 0; f < 3; ++f) carry[f] += tot[f];
+		carry[0] += tot[0];
+		carry[2] += tot[2];
+		if (r % kSortPer == kSortPer - 1)
+			carry[1] += s_dsub[r / kSortPer];
 	}
 	wave_atomic_min(&a.ds->fail_key, fk);
 	__shared__ unsigned long long s_top[kDomBlock / 64][2];

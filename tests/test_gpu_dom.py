@@ -113,6 +113,18 @@ def test_random_byte_soup(torch_cuda):
         check(torch_cuda, b"".join(rng.choice(parts) for _ in range(rng.randint(0, 40))))
 
 
+def test_random_token_soup(torch_cuda):
+    """Longer soups of JSON-ish tokens: escapes of every kind (surrogate pairs
+    included), long strings, nested fragments, control bytes."""
+    rng = random.Random(15)
+    parts = [b"{", b"}", b"[", b"]", b",", b":", b'"a"', b'"', b"1", b"-2.5e3", b"true", b"null", b" ", b"x",
+             b'\\"', b"01", b'"\\u00e9"', b"fals", b"\n", b'"k":', b'{"a":1}', b"[1,2]", b'"\\ud83d\\ude00"',
+             b'"' + b"y" * 40 + b'"', b'"\\\\"', b"-0.5e-7", b"\t", b'"\x01"', b"[]", b"{}"]
+    say("[gpu] tape random token soup x1000")
+    for _ in range(1000):
+        check(torch_cuda, b"".join(rng.choice(parts) for _ in range(rng.randint(0, 120))))
+
+
 def test_corrupted_documents(torch_cuda):
     rng = random.Random(12)
     base = W.json(21, 200_000, 10).tobytes()

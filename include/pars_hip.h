@@ -247,6 +247,7 @@ typedef struct {
 	uint64_t		t_done_ns;	   /* its batch's outputs were complete on the device */
 	uint32_t		batch_docs;	   /* documents in its batch */
 	uint32_t		reserved;
+	const uint8_t*	input;		   /* device: the request's bytes in the input ring, valid until release */
 } pars_request_result;
 
 typedef struct {
@@ -257,6 +258,8 @@ typedef struct {
 	uint64_t budget_waits;		 /* batch formations cut short by the HBM budget */
 	uint64_t device_bytes;		 /* held on the device (rings + scanner arenas) */
 	uint64_t inflight_reserved;	 /* budget held by admitted, unreleased requests */
+	uint64_t ring_wraps;		 /* device ring allocations that wrapped to offset 0 past live data */
+	uint64_t staging_wraps;		 /* the same for the pinned staging ring */
 } pars_server_stats;
 
 int pars_hip_server_create(const pars_server_config* cfg, pars_server** out);
@@ -268,7 +271,8 @@ int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, 
  * the first one. */
 int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, size_t* n_out, int64_t wait_us);
 /* The request's outputs are no longer needed (frees its budget; a scanner
- * slot is reused once all of its batch's requests are released). */
+ * slot, and its batch's region of the device input ring, are reused once all
+ * of the batch's requests are released). */
 int pars_hip_server_release(pars_server* sv, uint64_t id);
 int pars_hip_server_stats(pars_server* sv, pars_server_stats* out);
 /* Load driver: submits n requests (host bytes data[i], sizes[i], plan[i]) at

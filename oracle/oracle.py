@@ -188,3 +188,30 @@ def ref_full(grammar: str, data) -> dict:
 def ref_time(grammar: str, path: str, max_bytes: int) -> dict:
     r = subprocess.run([REF, "time", grammar, path, str(max_bytes)], check=True, stdout=subprocess.PIPE, text=True)
     return _json.loads(r.stdout)
+
+
+def batch_full(plan: bytes, buf: np.ndarray, spans) -> dict:
+    """model::scanner::scan + pair_brackets run on every document of a packed
+    batch alone (spans: (offset, size) pairs); per-document outputs concatenated
+    in document order with their start indices, as pars_hip_scan_batch lays
+    them out (positions document-local, pairs as document-local bracket indices)."""
+    nd = len(spans)
+    st, bst = np.zeros(nd + 1, np.uint64), np.zeros(nd + 1, np.uint64)
+    pos, bpos, bch, pr = [], [], [], []
+    flags = np.zeros(nd, np.uint8)
+    depth = np.zeros(nd, np.int32)
+    for d in range(nd):
+        o, n = int(spans[d].offset), int(spans[d].size)
+        r = full(plan, buf[o:o + n])
+        pos.append(r["positions"])
+        bpos.append(r["bracket_pos"])
+        bch.append(r["bracket_char"])
+        pr.append(r["pair"])
+        st[d + 1] = st[d] + r["positions"].size
+        bst[d + 1] = bst[d] + r["bracket_pos"].size
+        flags[d] = (1 if r["balanced"] else 0) | (2 if r["well_formed"] else 0) | (4 if r["unclosed_mode"] else 0)
+        depth[d] = r["max_depth"]
+    cat = lambda xs, t: np.concatenate(xs).astype(t) if xs else np.zeros(0, t)  # noqa: E731
+    return {"positions": cat(pos, np.uint32), "struct_start": st, "bracket_pos": cat(bpos, np.uint32),
+            "bracket_char": cat(bch, np.uint8), "bracket_start": bst, "pair": cat(pr, np.int32),
+            "flags": flags, "max_depth": depth}

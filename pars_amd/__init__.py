@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 import threading
 
 from . import _native as N
@@ -224,6 +225,7 @@ class Scanner:
         h = C.c_void_p()
         _check(self._lib.pars_hip_scanner_create(plan.blob, C.byref(h)))
         self._h = h
+        N.track(self)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -231,6 +233,9 @@ class Scanner:
             self._h = None
 
     def __del__(self):
+        # at interpreter shutdown the exit handler has closed every handle
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -350,6 +355,7 @@ class Server:
         h = C.c_void_p()
         _check(self._lib.pars_hip_server_create(C.byref(cfg), C.byref(h)))
         self._h = h
+        N.track(self)
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -357,6 +363,9 @@ class Server:
             self._h = None
 
     def __del__(self):
+        # at interpreter shutdown the exit handler has closed every handle
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

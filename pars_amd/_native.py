@@ -1,8 +1,10 @@
 """ctypes view of include/pars_hip.h (structures and prototypes)."""
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpars_hip.so")
@@ -150,6 +152,7 @@ class RequestResult(C.Structure):
         ("t_done_ns", C.c_uint64),
         ("batch_docs", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("input", C.c_void_p),
     ]
 
 
@@ -164,6 +167,8 @@ class ServerStats(C.Structure):
         ("budget_waits", C.c_uint64),
         ("device_bytes", C.c_uint64),
         ("inflight_reserved", C.c_uint64),
+        ("ring_wraps", C.c_uint64),
+        ("staging_wraps", C.c_uint64),
     ]
 
 
@@ -269,6 +274,35 @@ def _bind(lib, protos):
 
 _lib = None
 _gen = None
+# Every object holding a native handle (Scanner, Server): shutdown() closes
+# whatever is still open before the interpreter tears down.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def track(obj) -> None:
+    """Register an object with a close() that releases a native handle."""
+    _live.add(obj)
+
+
+def shutdown() -> None:
+    """Deterministic teardown, run at exit (before interpreter finalisation):
+    servers first (their worker threads and streams), then scanners, then the
+    library itself is unloaded while the HIP runtime is still fully alive, so
+    no HIP call is left for the process's static destructors."""
+    global _lib
+    objs = list(_live)
+    objs.sort(key=lambda o: 0 if hasattr(o, "submit") else 1)
+    for o in objs:
+        try:
+            o.close()
+        except Exception:
+            pass
+    _live.clear()
+    if _lib is not None:
+        handle, _lib = _lib._handle, None
+        import _ctypes
+
+        _ctypes.dlclose(handle)
 
 
 def lib():
@@ -281,6 +315,8 @@ def lib():
                 "(the scan path has no CPU fallback)"
             )
         _lib = _bind(C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL), PROTOTYPES)
+        atexit.unregister(shutdown)
+        atexit.register(shutdown)
     return _lib
 
 

@@ -22,7 +22,10 @@ OBJ = os.path.join(HERE, "_obj")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-HIP_SOURCES = ["scan_kernels.hip", "batch_kernels.hip", "pair_kernels.hip", "dom_kernels.hip", "api.hip", "server.hip"]
+HIP_SOURCES = [
+    "scan_kernels.hip", "batch_kernels.hip", "pair_kernels.hip", "pair_stack.hip", "dom_kernels.hip", "api.hip",
+    "server.hip",
+]
 CXX_SOURCES = ["plan_dev.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -100,12 +103,22 @@ def build_gap_check() -> str:
     return out
 
 
+def build_ring_check() -> str:
+    """tests/cpp/ring_check: the serving runtime's byte ring (csrc/ring.hpp),
+    host-only, under the server's retirement order."""
+    src = os.path.join(ROOT, "tests", "cpp", "ring_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "ring_check")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", src, "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_oracle()
     build_hip()
     build_facade_check()
     build_gap_check()
+    build_ring_check()
 
 
 if __name__ == "__main__":

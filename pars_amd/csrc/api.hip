@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <thread>
 #include <cstdarg>
 #include <cstdio>
@@ -86,6 +87,17 @@ struct DevBuf {
 
 }  // namespace
 
+namespace pars_hip {
+int set_error(int code, const std::string& msg) {
+	g_err = msg;
+	return code;
+}
+
+const std::string& last_error() {
+	return g_err;
+}
+}  // namespace pars_hip
+
 struct pars_scanner {
 	pars_plan plan;
 	DevPlan	  hplan;
@@ -93,6 +105,8 @@ struct pars_scanner {
 	DevBuf	  recs, ticket, summary;
 	DevBuf	  positions, chars, bitmap, pair_pos;
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
+	DevBuf	  stk_res, stk_rec, stk_st;	 // stack pairing: chunk residues, chunk depths, their sparse table
+	bool	  stack_pending = false;	 // the last enqueued R6 was the stack path (sort path if unbalanced)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
 	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0;  // tuning; 0 = default
@@ -110,7 +124,7 @@ struct pars_scanner {
 	// batches: the document table (pinned staging + device), per-document results
 	// the document table: n offsets (u64) then n lengths (u32), one copy up
 	DevBuf	  doc_tab, tile_doc, head_bits, st_start, br_start, doc_final, doc_min, doc_max, doc_last, bdoc, doc_bad,
-		doc_flags, doc_pack;
+		doc_flags, doc_pack, doc_kbase, kb_part;
 	uint8_t*  h_doctab	 = nullptr;
 	size_t	  h_tab_cap	 = 0;
 	int		  use_graph	 = 0;
@@ -133,10 +147,12 @@ struct pars_scanner {
 	~pars_scanner() {
 		if (gexec)
 			(void)hipGraphExecDestroy(gexec);
+		for (DevBuf* b : {&stk_res, &stk_rec, &stk_st})
+			b->release();
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
 						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
-						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &dom_tinfo, &dom_gbeg, &dom_gend,
+						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &doc_kbase, &kb_part, &dom_tinfo, &dom_gbeg, &dom_gend,
 						  &dom_gdec, &dom_sub, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum,
 						  &tape_kind, &tape_beg, &tape_end, &tape_soff, &tape_arena})
 			b->release();
@@ -211,6 +227,10 @@ int ensure_outputs(pars_scanner* sc, uint64_t n_tiles, uint64_t cap_struct, uint
 	const uint64_t chunks = pairing_max_chunks(cap_br);
 	if ((r = sc->hist.ensure(256 * std::max<uint64_t>(chunks, 1) * 4)))
 		return r;
+	const uint64_t schunks = (cap_br + kStkChunk - 1) / kStkChunk;
+	if ((r = sc->stk_res.ensure(cap_br * 4)) || (r = sc->stk_rec.ensure(std::max<uint64_t>(schunks, 1) * 12))
+		|| (r = sc->stk_st.ensure(std::max<uint64_t>(schunks, 1) * stack_levels(schunks) * 4)))
+		return r;
 	if ((r = sc->hist_part.ensure(256 * 4)))
 		return r;
 	sc->cap_struct = cap_struct;
@@ -266,6 +286,71 @@ PairArgs make_pair_args(pars_scanner* sc, bool want_pairs) {
 	p.close		 = sc->force_radix ? sc->dom_close.as<uint32_t>() : nullptr;
 	p.max_digit_bits = (uint32_t)sc->radix_bits;
 	return p;
+}
+
+StackArgs make_stack_args(pars_scanner* sc, bool want_pairs) {
+	StackArgs a {};
+	a.sum		 = sc->summary.as<DevSummary>();
+	a.plan		 = sc->dplan.as<DevPlan>();
+	a.bchar		 = sc->bchar.as<uint8_t>();
+	a.bkey		 = sc->bkey.as<int32_t>();
+	a.bsidx		 = sc->bsidx.as<uint32_t>();
+	a.cap_br	 = sc->cap_br;
+	a.pair		 = sc->pair.as<int32_t>();
+	a.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
+	a.res		 = sc->stk_res.as<uint32_t>();
+	a.crec		 = sc->stk_rec.as<int32_t>();
+	a.cst		 = sc->stk_st.as<int32_t>();
+	a.max_chunks = std::max<uint64_t>((sc->cap_br + kStkChunk - 1) / kStkChunk, 1);
+	a.st_levels	 = stack_levels(a.max_chunks);
+	return a;
+}
+
+// R6 of one document (or of the chunks of one): the stack path unless a
+// digit cap, a consumer of the sorted order or the one-block path asks for the
+// sort.  The stack path's kernels exit on an unbalanced document, which then
+// takes the sort once the summary shows it (finish_pairs).
+hipError_t enqueue_pairs(pars_scanner* sc, uint64_t n, bool want_pairs, hipStream_t s) {
+	const uint32_t small = small_pairing_mode(n, 1);
+	if (!sc->radix_bits && !sc->force_radix && small != 2) {
+		sc->stack_pending = true;
+		return launch_stack_pairing(make_stack_args(sc, want_pairs), s);
+	}
+	sc->stack_pending = false;
+	PairArgs p		  = make_pair_args(sc, want_pairs);
+	p.small			  = (sc->radix_bits || sc->force_radix) ? 0u : small;
+	return launch_pairing(p, s);
+}
+
+int read_summary(pars_scanner* sc, hipStream_t s);
+
+// After the summary of a stack-path R6 is read: an unbalanced document runs the
+// sort (its pairs differ from a stack's), then the summary is read again.
+int finish_pairs(pars_scanner* sc, bool want_pairs, hipStream_t s, double* ms) {
+	if (!sc->stack_pending)
+		return PARS_OK;
+	sc->stack_pending = false;
+	const DevSummary& h = *sc->h_sum;
+	if (h.overflow & kStackLost)
+		return fail(PARS_ERR_DEVICE, "stack pairing found an unmatched residue on a balanced document (internal error)");
+	if (h.min_bal >= 0 && h.last_bal == 0)
+		return PARS_OK;
+	hipError_t e;
+	if (ms && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	PairArgs p = make_pair_args(sc, want_pairs);
+	p.small	   = 0;
+	if ((e = launch_pairing(p, s)) != hipSuccess)
+		return hip_fail(e, "pairing launch");
+	if (ms && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	int r;
+	if ((r = read_summary(sc, s)))
+		return r;
+	float t = 0;
+	if (ms && hipEventElapsedTime(&t, sc->ev[1], sc->ev[2]) == hipSuccess)
+		*ms += t;
+	return PARS_OK;
 }
 
 int read_summary(pars_scanner* sc, hipStream_t s) {
@@ -326,13 +411,9 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 		return hip_fail(e, "scan kernel launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
-	if (run_pairs) {
-		PairArgs p = make_pair_args(sc, want_pairs);
-		// a set digit cap, or a consumer of the sorted order, asks for the radix path
-		p.small = (sc->radix_bits || sc->force_radix) ? 0u : small_pairing_mode(n, 1);
-		if ((e = launch_pairing(p, s)) != hipSuccess)
-			return hip_fail(e, "pairing launch");
-	}
+	sc->stack_pending = false;
+	if (run_pairs && (e = enqueue_pairs(sc, n, want_pairs, s)) != hipSuccess)
+		return hip_fail(e, "pairing launch");
 	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
 	return PARS_OK;
@@ -382,6 +463,7 @@ int ensure_batch(pars_scanner* sc, uint64_t n, uint64_t n_docs, uint64_t cap_br,
 		|| (r = sc->doc_final.ensure(n_docs * 4)) || (r = sc->doc_min.ensure(n_docs * 4))
 		|| (r = sc->doc_max.ensure(n_docs * 4)) || (r = sc->doc_last.ensure(n_docs * 4))
 		|| (r = sc->doc_bad.ensure(n_docs * 4)) || (r = sc->doc_flags.ensure(n_docs))
+		|| (r = sc->doc_kbase.ensure(n_docs * 4)) || (r = sc->kb_part.ensure(256 * 8))
 		|| (r = sc->bdoc.ensure(sc->cap_br * 4)))
 		return r;
 	if ((r = ensure_pinned((void**)&sc->h_doctab, sc->h_tab_cap, n_docs * 12)))
@@ -425,6 +507,10 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 		p.st_start = b.st_start;
 		p.n_docs   = b.n_docs;
 		p.doc_bad  = sc->doc_bad.as<uint32_t>();
+		p.doc_min	= b.doc_min;
+		p.doc_max	= b.doc_max;
+		p.doc_kbase = sc->doc_kbase.as<uint32_t>();
+		p.kb_part	= sc->kb_part.as<uint64_t>();
 		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, n_docs);
 		if ((e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
@@ -451,7 +537,7 @@ std::vector<uintptr_t> batch_graph_key(pars_scanner* sc, const uint8_t* d_in, ui
 	for (DevBuf* b : {&sc->recs, &sc->ticket, &sc->summary, &sc->positions, &sc->chars, &sc->bitmap, &sc->pair_pos,
 					  &sc->bpos, &sc->bsidx, &sc->bkey, &sc->bchar, &sc->ukeys0, &sc->ukeys1, &sc->order0,
 					  &sc->order1, &sc->rank, &sc->pair, &sc->hist, &sc->hist_part, &sc->doc_tab, &sc->tile_doc, &sc->head_bits, &sc->st_start, &sc->br_start, &sc->doc_final, &sc->doc_min,
-					  &sc->doc_max, &sc->doc_last, &sc->bdoc, &sc->doc_bad, &sc->doc_flags})
+					  &sc->doc_max, &sc->doc_last, &sc->bdoc, &sc->doc_bad, &sc->doc_flags, &sc->doc_kbase, &sc->kb_part})
 		k.push_back((uintptr_t)b->p);
 	return k;
 }
@@ -667,6 +753,8 @@ int pars_hip_scan_device(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_s
 		if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
 			st->p4 = ms;
 	}
+	if ((r = finish_pairs(sc, want_pairs != 0, s, st ? &st->p4 : nullptr)))
+		return r;
 	fill_result(sc, n, want_pairs != 0, run_pairs != 0, out);
 	return PARS_OK;
 }
@@ -776,8 +864,8 @@ int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const p
 		if ((r = ensure_batch(sc, n, n_docs, sc->h_sum->n_brackets, want_pairs != 0)))
 			return r;
 	}
-	if (sc->h_sum->overflow & 4u)
-		return fail(PARS_ERR_ARGUMENT, "batch pairing key (document, depth) exceeds 32 bits; split the batch");
+	if (sc->h_sum->overflow & 4u)  // more than 2^32 distinct (document, depth) keys
+		return fail(PARS_ERR_ARGUMENT, "batch pairing needs more than 2^32 (document, depth) keys; split the batch");
 	if (st) {
 		float ms = 0;
 		if (graph) {
@@ -887,10 +975,11 @@ int pars_hip_pair_chunks(pars_scanner* sc, const pars_chunk_carry* last, pars_st
 		if ((e = launch_fill_i32(sc->pair_pos.as<int32_t>(), last->out_base, -1, s)) != hipSuccess)
 			return hip_fail(e, "pair_pos fill");
 	}
-	PairArgs p = make_pair_args(sc, want_pairs != 0);
-	if ((e = launch_pairing(p, s)) != hipSuccess)
+	sc->stack_pending = false;
+	if ((e = enqueue_pairs(sc, std::max<uint64_t>(last->bracket_base, (uint64_t)kSmallBrackets + 1), want_pairs != 0, s))
+		!= hipSuccess)
 		return hip_fail(e, "pairing launch");
-	if ((r = read_summary(sc, s)))
+	if ((r = read_summary(sc, s)) || (r = finish_pairs(sc, want_pairs != 0, s, nullptr)))
 		return r;
 	fill_result(sc, 1, want_pairs != 0, true, out);
 	out->unclosed_mode = ((last->scan & 0xFFFFFFu) & ((1u << (3 * sc->hplan.n_descs)) - 1u))
@@ -1117,9 +1206,7 @@ int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t
 		}
 		if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 			return hip_fail(e, "hipEventRecord");
-		PairArgs p = make_pair_args(sc, want_pairs != 0);
-		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, 1);
-		if ((e = launch_pairing(p, s)) != hipSuccess)
+		if ((e = enqueue_pairs(sc, n, want_pairs != 0, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 		if (st && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
 			return hip_fail(e, "hipEventRecord");
@@ -1138,6 +1225,8 @@ int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t
 		if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
 			st->p4 = ms;
 	}
+	if ((r = finish_pairs(sc, want_pairs != 0, s, st ? &st->p4 : nullptr)))
+		return r;
 	fill_result(sc, n, want_pairs != 0, true, out);
 	return PARS_OK;
 }
@@ -1174,6 +1263,10 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	if (h.small_done || h.sort_passes < 1 || h.sort_passes > 4)
 		return fail(PARS_ERR_DEVICE, "the bracket sort did not run through the radix passes (internal error)");
 	const uint64_t ns = res.n_struct, nb = res.n_brackets, ntok = ns + 1;
+	// sorted bracket positions and parents are int32 in the tape kernels
+	if (nb > (uint64_t)INT32_MAX)
+		return fail(PARS_ERR_ARGUMENT, "the JSON tape handles at most 2^31-1 brackets (%llu here)",
+					(unsigned long long)nb);
 	const uint64_t n_tblk = (ntok + kDomTile - 1) / kDomTile, n_hblk = std::max<uint64_t>((nb + kDomTile - 1) / kDomTile, 1);
 	const uint64_t cap_nodes = std::min<uint64_t>(n, 2 * ns + 1);
 	for (DevBuf* b : {&sc->dom_gbeg, &sc->dom_gend, &sc->dom_gdec})

@@ -7,7 +7,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace pars_hip {
+
+// The calling thread's last error, which pars_hip_last_error() returns; every
+// part of the library (scanners and servers) records its failures here.
+int				   set_error(int code, const std::string& msg);
+const std::string& last_error();
 
 // Device pointers of one enqueued batch (valid until the scanner's next batch).
 struct BatchPtrs {

@@ -39,13 +39,15 @@ struct DevSummary {
 	uint32_t n_chunks;	   // pairing sort chunks
 	uint32_t overflow;	   // an output exceeded its capacity
 	uint32_t digit_bits;   // radix digit width of the pairing sort
-	uint32_t key_span;	   // batch pairing: keys per document (kmax - kmin + 1)
 	// Look-back epoch of the current launch: advanced by every summary reset
 	// (never reset itself), so records of earlier launches - also of earlier
 	// replays of one captured graph - read as unpublished.
 	uint32_t epoch;
 	uint32_t small_done;  // R6 was completed by the one-block path (k_pair_small)
 	uint32_t pad;
+	// batch pairing: distinct (document, key) values, the sum over documents
+	// of (max depth - min depth + 2)
+	uint64_t key_total;
 };
 
 // Batch mode: the input holds n_docs documents in ascending order, each
@@ -191,12 +193,18 @@ struct PairArgs {
 							  // the pair pass writes only this: no pairs, no mismatch report
 	uint32_t	   out_base;  // structural index base of pair_pos entries
 	// batch mode: sort by (document, key); pair within a document; indices in
-	// the outputs are document-local
+	// the outputs are document-local.  Document d's keys k map to
+	// doc_kbase[d] + k - (doc_min[d] - 1), doc_kbase being the exclusive prefix
+	// of the per-document key spans doc_max[d] - doc_min[d] + 2
 	const uint32_t* bdoc;
 	const uint32_t* br_start;
 	const uint32_t* st_start;
 	uint32_t		n_docs;
 	uint32_t*		doc_bad;  // [n_docs] a sorted pair of the document does not match
+	const int32_t*	doc_min;
+	const int32_t*	doc_max;
+	uint32_t*		doc_kbase;	// [n_docs]
+	uint64_t*		kb_part;	// [256] partial sums of the key-base scan
 	uint32_t		max_digit_bits;	 // radix digit width cap per pass (1..8; 0 = 8)
 	// One-block R6 for at most kSmallBrackets brackets: 0 = not tried, 1 = tried
 	// first (the radix path then runs only if it did not apply), 2 = the only
@@ -207,15 +215,39 @@ struct PairArgs {
 constexpr uint32_t kSmallBrackets = 8192;
 // The R6 path for n input bytes in n_docs documents (1 for a single document):
 // the one-block path is certain when n bounds the bracket count to
-// kSmallBrackets and the (document, key) range to 19 bits, worth trying up to
-// 1 MiB, and skipped above.
+// kSmallBrackets (the (document, key) range is then at most n + 2 n_docs + 1,
+// within its 19 bits), worth trying up to 1 MiB, and skipped above.
 inline uint32_t small_pairing_mode(uint64_t n, uint64_t n_docs) {
-	if (n <= kSmallBrackets && n_docs * (n + 2) < (1ull << 19))
+	if (n <= kSmallBrackets && n + 2 * n_docs + 2 < (1ull << 19))
 		return 2;
 	return n <= (1ull << 20) ? 1u : 0u;
 }
 
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
+
+// R6 for balanced documents by stack matching (pair_stack.hip): the same pairs
+// the sort gives when the depth never drops below 0 and ends at 0; the kernels
+// exit at once otherwise and the sorting path runs instead.
+constexpr uint32_t kStkChunk  = 16384;		  // brackets per chunk (1024 threads x 16)
+constexpr uint32_t kStackLost = 0x10u;		  // summary overflow bit: a residue close found no open (internal)
+struct StackArgs {
+	DevSummary*		sum;
+	const DevPlan*	plan;
+	const uint8_t*	bchar;
+	const int32_t*	bkey;
+	const uint32_t* bsidx;
+	uint64_t		cap_br;
+	int32_t*		pair;	   // CPU convention, per bracket index
+	int32_t*		pair_pos;  // optional: structural-index convention
+	uint32_t*		res;	   // [cap_br] per chunk: unmatched closes, then unmatched opens (bracket indices)
+	int32_t*		crec;	   // [3 max_chunks] per chunk: entry, minimum and exit depth
+	int32_t*		cst;	   // [st_levels max_chunks] sparse table of chunk minima
+	uint64_t		max_chunks;
+	uint32_t		st_levels;
+};
+hipError_t launch_stack_pairing(const StackArgs& a, hipStream_t s);
+// Sparse-table levels for up to max_chunks chunks.
+uint32_t stack_levels(uint64_t max_chunks);
 
 // Fill n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t s);

@@ -48,9 +48,9 @@ __global__ void k_sort_prep(PairArgs a) {
 	const uint32_t nb	= n_brackets(a);
 	const int32_t  kmin = s->min_bal - 1;  // open keys are depth-1, depths >= min_bal
 	const int32_t  kmax = s->max_bal;
-	// batch: the sort key is (document, key) as document * span + (key - kmin)
-	const uint64_t span	 = (uint64_t)(kmax - kmin) + 1u;
-	const uint64_t range = a.bdoc ? (uint64_t)a.n_docs * span - 1u : (uint64_t)(kmax - kmin);
+	// batch: the key is a document's own key range, placed after the ranges of
+	// the documents before it (k_kbase_*)
+	const uint64_t range = a.bdoc ? (s->key_total ? s->key_total - 1u : 0u) : (uint64_t)(kmax - kmin);
 	uint32_t	   bits	 = range ? 64u - (uint32_t)__clzll((long long)range) : 1u;
 	if (bits > 32u) {  // more than one 32-bit sort key can hold
 		s->overflow |= 4u;
@@ -60,7 +60,6 @@ __global__ void k_sort_prep(PairArgs a) {
 	const uint32_t p   = (bits + cap - 1u) / cap;
 	if (p > 4u)	 // four passes are launched: wider digits
 		s->overflow |= 8u;
-	s->key_span		 = (uint32_t)span;
 	s->kmin			 = kmin;
 	s->sort_passes		 = p;
 	s->digit_bits		 = (bits + p - 1) / p;
@@ -69,11 +68,117 @@ __global__ void k_sort_prep(PairArgs a) {
 		s->overflow |= 2u;
 }
 
+// ---- batch key bases -------------------------------------------------------
+// Document d's keys lie in [doc_min[d] - 1, doc_max[d]] (doc_min <= 0 <= doc_max:
+// depths after its brackets, and 0): span_d = doc_max[d] - doc_min[d] + 2 values.
+// doc_kbase[d] = the sum of the spans of the documents before d, so the batch's
+// (document, key) order is the order of doc_kbase[d] + key - (doc_min[d] - 1),
+// within sum(span_d) <= n_brackets + 2 n_docs values.
+__device__ __forceinline__ uint64_t doc_span(const PairArgs& a, uint64_t d) {
+	return (uint64_t)((int64_t)a.doc_max[d] - (int64_t)a.doc_min[d]) + 2u;
+}
+
+__device__ __forceinline__ uint32_t doc_key(const PairArgs& a, uint32_t i) {
+	const uint32_t d = a.bdoc[i];
+	return a.doc_kbase[d] + (uint32_t)(a.bkey[i] - (a.doc_min[d] - 1));
+}
+
+// Exclusive block scan of 64-bit values (blockDim.x a multiple of 64, <= 1024).
+__device__ uint64_t block_excl_sum64(uint64_t v, uint64_t* s_w, uint64_t& total) {
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	uint64_t  inc  = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint64_t t = __shfl_up(inc, o, 64);
+		if (lane >= o)
+			inc += t;
+	}
+	if (lane == 63)
+		s_w[wid] = inc;
+	__syncthreads();
+	uint64_t p = 0, t = 0;
+	for (int w = 0; w < nw; ++w) {
+		if (w < wid)
+			p += s_w[w];
+		t += s_w[w];
+	}
+	__syncthreads();
+	total = t;
+	return p + inc - v;
+}
+
+__device__ __forceinline__ void store_kbase(const PairArgs& a, uint64_t d, uint64_t base) {
+	a.doc_kbase[d] = (uint32_t)base;  // exact whenever key_total fits 32 bits (else the sort refuses)
+}
+
+// Up to kKbaseOneBlock documents: one block.
+constexpr uint32_t kKbaseOneBlock = 16384;
+
+__global__ __launch_bounds__(1024) void k_kbase_one(PairArgs a) {
+	__shared__ uint64_t s_w[16];
+	const uint64_t		nd	= a.n_docs;
+	uint64_t			run = 0;
+	for (uint64_t d0 = 0; d0 < nd; d0 += blockDim.x) {
+		const uint64_t d = d0 + threadIdx.x;
+		const uint64_t v = d < nd ? doc_span(a, d) : 0u;
+		uint64_t	   tot;
+		const uint64_t ex = block_excl_sum64(v, s_w, tot);
+		if (d < nd)
+			store_kbase(a, d, run + ex);
+		run += tot;
+	}
+	if (threadIdx.x == 0)
+		a.sum->key_total = run;
+}
+
+__device__ __forceinline__ void kb_range(uint64_t len, uint32_t b, uint64_t& lo, uint64_t& hi) {
+	const uint64_t per = (len + kScanGrid - 1) / kScanGrid;
+	lo				   = min<uint64_t>((uint64_t)b * per, len);
+	hi				   = min<uint64_t>(lo + per, len);
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_kbase_partial(PairArgs a) {
+	__shared__ uint64_t s_w[4];
+	uint64_t			lo, hi;
+	kb_range(a.n_docs, blockIdx.x, lo, hi);
+	uint64_t acc = 0;
+	for (uint64_t d = lo + threadIdx.x; d < hi; d += kSortBlock) acc += doc_span(a, d);
+	uint64_t tot;
+	(void)block_excl_sum64(acc, s_w, tot);
+	if (threadIdx.x == 0)
+		a.kb_part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_kbase_top(PairArgs a) {
+	__shared__ uint64_t s_w[4];
+	uint64_t			tot;
+	const uint64_t		v  = a.kb_part[threadIdx.x];  // kScanGrid == kSortBlock
+	const uint64_t		ex = block_excl_sum64(v, s_w, tot);
+	a.kb_part[threadIdx.x] = ex;
+	if (threadIdx.x == 0)
+		a.sum->key_total = tot;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_kbase_apply(PairArgs a) {
+	__shared__ uint64_t s_w[4];
+	uint64_t			lo, hi;
+	kb_range(a.n_docs, blockIdx.x, lo, hi);
+	uint64_t run = a.kb_part[blockIdx.x];
+	for (uint64_t d0 = lo; d0 < hi; d0 += kSortBlock) {
+		const uint64_t d = d0 + threadIdx.x;
+		const uint64_t v = d < hi ? doc_span(a, d) : 0u;
+		uint64_t	   tot;
+		const uint64_t ex = block_excl_sum64(v, s_w, tot);
+		if (d < hi)
+			store_kbase(a, d, run + ex);
+		run += tot;
+	}
+}
+
 __device__ __forceinline__ uint32_t key_at(const PairArgs& a, int pass, uint32_t i, int32_t kmin) {
 	if (pass != 0)
 		return a.ukeys[(pass - 1) & 1][i];
-	const uint32_t k = (uint32_t)(a.bkey[i] - kmin);
-	return a.bdoc ? a.bdoc[i] * a.sum->key_span + k : k;
+	return a.bdoc ? doc_key(a, i) : (uint32_t)(a.bkey[i] - kmin);
 }
 
 template<int PASS>
@@ -299,7 +404,8 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs a) {
 }
 
 // R6 in one block for at most kSmallBrackets brackets whose (document, key)
-// range fits 19 bits: every bracket becomes (key << 13) | index, so sorting
+// range fits 19 bits (batch keys as in k_kbase_*): every bracket becomes
+// (key << 13) | index, so sorting
 // these distinct values is the stable sort by key; a bitonic sort in LDS, then
 // sorted neighbours pair exactly as in k_pair.  Small batches (the serving
 // case) and small documents then cost one launch instead of the radix passes.
@@ -313,18 +419,15 @@ __global__ __launch_bounds__(kSmallBlock) void k_pair_small(PairArgs a) {
 	const uint32_t		nb	= n_brackets(a);
 	const int			tid = threadIdx.x;
 	const int32_t		kmin = sum->min_bal - 1, kmax = sum->max_bal;
-	const uint64_t		span = (uint64_t)(kmax - kmin) + 1u;
-	const uint64_t		range = a.bdoc ? (uint64_t)a.n_docs * span - 1u : (uint64_t)(kmax - kmin);
+	const uint64_t		range = a.bdoc ? (sum->key_total ? sum->key_total - 1u : 0u) : (uint64_t)(kmax - kmin);
 	if (nb > kSmallBrackets || nb == 0 || range >= (1ull << 19))
 		return;	 // not this path (nb == 0: nothing to pair, flags need no sort)
 	uint32_t N = 1;
 	while (N < nb) N <<= 1;
 	for (uint32_t i = tid; i < N; i += kSmallBlock) {
 		uint32_t v = 0xFFFFFFFFu;  // padding sorts last
-		if (i < nb) {
-			const uint32_t k = (uint32_t)(a.bkey[i] - kmin);
-			v				 = ((a.bdoc ? a.bdoc[i] * (uint32_t)span + k : k) << 13) | i;
-		}
+		if (i < nb)
+			v = ((a.bdoc ? doc_key(a, i) : (uint32_t)(a.bkey[i] - kmin)) << 13) | i;
 		s_v[i] = v;
 	}
 	__syncthreads();
@@ -401,6 +504,15 @@ hipError_t launch_pass(const PairArgs& a, unsigned grid, hipStream_t s) {
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
 	if (a.cap_br == 0)
 		return hipSuccess;
+	if (a.bdoc) {  // per-document key bases first
+		if (a.n_docs <= kKbaseOneBlock) {
+			hipLaunchKernelGGL(k_kbase_one, dim3(1), dim3(1024), 0, s, a);
+		} else {
+			hipLaunchKernelGGL(k_kbase_partial, dim3(kScanGrid), dim3(kSortBlock), 0, s, a);
+			hipLaunchKernelGGL(k_kbase_top, dim3(1), dim3(kSortBlock), 0, s, a);
+			hipLaunchKernelGGL(k_kbase_apply, dim3(kScanGrid), dim3(kSortBlock), 0, s, a);
+		}
+	}
 	if (a.small) {
 		hipLaunchKernelGGL(k_pair_small, dim3(1), dim3(kSmallBlock), 0, s, a);
 		if (a.small == 2)

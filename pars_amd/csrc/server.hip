@@ -17,6 +17,7 @@
 // overlaps the previous batch's kernels.  No counterpart in the reference.
 #include "../../include/pars_hip.h"
 #include "internal.hpp"
+#include "ring.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -49,47 +50,6 @@ uint64_t align_up(uint64_t x) {
 	return (x + kAlign - 1) / kAlign * kAlign;
 }
 
-// A byte ring with FIFO release: allocations are contiguous and released in
-// the order they were made; live data spans [tail, head), possibly wrapped.
-struct Ring {
-	uint64_t			 cap = 0, head = 0, tail = 0;
-	std::deque<uint64_t> live;	// start offsets in allocation order
-
-	bool alloc(uint64_t bytes, uint64_t& off) {
-		if (bytes == 0 || bytes > cap)
-			return false;
-		if (live.empty())
-			head = tail = 0;
-		uint64_t at;
-		if (live.empty() || head > tail) {	// free: [head, cap) and [0, tail)
-			if (head + bytes <= cap)
-				at = head;
-			else if (bytes <= tail)
-				at = 0;	 // wrap; [head, cap) idles until the tail passes it
-			else
-				return false;
-		} else if (head < tail && head + bytes <= tail) {  // free: [head, tail)
-			at = head;
-		} else {
-			return false;
-		}
-		off	 = at;
-		head = at + bytes;
-		live.push_back(at);
-		return true;
-	}
-
-	void release_oldest() {
-		if (live.empty())
-			return;
-		live.pop_front();
-		if (live.empty())
-			head = tail = 0;
-		else
-			tail = live.front();
-	}
-};
-
 enum ReqState : int { kPending = 0, kInBatch = 1, kDone = 2 };
 
 struct Request {
@@ -110,6 +70,9 @@ struct Slot {
 	pars_scanner*		  sc = nullptr;
 	hipEvent_t			  h2d = nullptr, done = nullptr;
 	int					  state = kFree;
+	int					  fail_status = PARS_OK;  // the batch could not be enqueued: retire it as failed
+	uint64_t			  ring_off	  = 0;		  // its inputs in the device ring, held until the slot frees
+	bool				  ring_live	  = false;
 	std::vector<uint64_t> reqs;
 	uint32_t			  outstanding = 0;
 	uint64_t			  launch_ns	  = 0;
@@ -159,16 +122,12 @@ struct pars_server {
 
 namespace {
 
-thread_local std::string g_err;
-
 int sfail(int code, const char* msg) {
-	g_err = msg;
-	return code;
+	return set_error(code, msg);
 }
 
 int shipfail(hipError_t e, const char* what) {
-	g_err = std::string(what) + ": " + hipGetErrorString(e);
-	return PARS_ERR_DEVICE;
+	return set_error(PARS_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
 }  // namespace
@@ -211,31 +170,39 @@ bool pars_server::retire() {
 	for (uint32_t p = 0; p < plans.size(); ++p) {
 		PlanState& ps = plans[p];
 		while (!ps.running.empty()) {
-			Slot&	   s = ps.slots[ps.running.front()];
-			hipError_t e = hipEventQuery(s.done);
+			Slot& s = ps.slots[ps.running.front()];
+			// a batch that failed to enqueue never recorded its event; it is
+			// retired in launch order like the others, so the ring regions freed
+			// below are always the oldest batch's own
+			const hipError_t e = s.fail_status ? hipSuccess : hipEventQuery(s.done);
 			if (e == hipErrorNotReady)
 				break;
 			const uint64_t	t	   = now_ns();
 			const size_t	nd	   = s.reqs.size();
 			const uint32_t* pk	   = s.h_pack;	// packed: starts, bracket starts, depths, flags, overflow
-			const bool		failed = e != hipSuccess || batch_failed_word(pk[4 * nd + 2]);
+			const bool		failed = s.fail_status || e != hipSuccess || batch_failed_word(pk[4 * nd + 2]);
+			const int		status = s.fail_status ? s.fail_status : PARS_ERR_DEVICE;
 			std::lock_guard<std::mutex> lk(mu);
 			for (size_t d = 0; d < nd; ++d) {
 				Request&			 q = reqs[s.reqs[d]];
 				pars_request_result& r = q.res;
+				r					   = pars_request_result {};
 				r.id				   = q.id;
-				r.status			   = failed ? PARS_ERR_DEVICE : PARS_OK;
+				r.status			   = failed ? status : PARS_OK;
 				r.plan				   = p;
 				r.n_bytes			   = q.n;
-				const uint32_t		 st0 = pk[d], br0 = pk[nd + 1 + d];
-				r.n_struct				 = pk[d + 1] - st0;
-				r.n_brackets			 = pk[nd + 2 + d] - br0;
-				r.structural			 = s.ptrs.structural + st0;
-				r.bracket_pos			 = s.ptrs.bracket_pos + br0;
-				r.bracket_char			 = s.ptrs.bracket_char + br0;
-				r.bracket_pair			 = s.ptrs.bracket_pair ? s.ptrs.bracket_pair + br0 : nullptr;
-				r.flags					 = pk[3 * nd + 2 + d];
-				r.max_depth				 = (int32_t)pk[2 * nd + 2 + d];
+				if (!failed) {
+					const uint32_t st0 = pk[d], br0 = pk[nd + 1 + d];
+					r.n_struct		   = pk[d + 1] - st0;
+					r.n_brackets	   = pk[nd + 2 + d] - br0;
+					r.structural	   = s.ptrs.structural + st0;
+					r.bracket_pos	   = s.ptrs.bracket_pos + br0;
+					r.bracket_char	   = s.ptrs.bracket_char + br0;
+					r.bracket_pair	   = s.ptrs.bracket_pair ? s.ptrs.bracket_pair + br0 : nullptr;
+					r.input			   = ps.ring + s.ring_off + s.offs[d];
+					r.flags			   = pk[3 * nd + 2 + d];
+					r.max_depth		   = (int32_t)pk[2 * nd + 2 + d];
+				}
 				r.t_submit_ns		   = q.t_submit;
 				r.t_launch_ns		   = q.t_launch;
 				r.t_done_ns			   = t;
@@ -244,11 +211,15 @@ bool pars_server::retire() {
 				q.state				   = kDone;
 				completed.push_back(q.id);
 				st.completed++;
-				st.bytes_completed += q.n;
+				if (!failed)
+					st.bytes_completed += q.n;
 			}
-			ps.dev_ring.release_oldest();
+			// staging is free once the inputs are on the device; the device ring
+			// region stays until the slot frees (the request's input is readable
+			// beside its results until release)
 			for (size_t d = 0; d < s.reqs.size(); ++d) ps.stage_ring.release_oldest();
-			s.state = kHeld;
+			s.state		  = kHeld;
+			s.fail_status = PARS_OK;
 			ps.running.pop_front();
 			any = true;
 		}
@@ -271,6 +242,10 @@ bool pars_server::launch(uint32_t p) {
 				slot = (int)i;
 				break;
 			}
+		if (slot >= 0 && ps.slots[slot].ring_live) {
+			ps.dev_ring.release(ps.slots[slot].ring_off);
+			ps.slots[slot].ring_live = false;
+		}
 		if (slot < 0 || ps.pending.empty())
 			return false;
 		const uint64_t avail = cfg.hbm_budget > fixed ? cfg.hbm_budget - fixed : 0;
@@ -297,13 +272,17 @@ bool pars_server::launch(uint32_t p) {
 			return false;
 	}
 	uint64_t ring_off = 0;
-	if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: give the budget back and retry later
+	{
 		std::lock_guard<std::mutex> lk(mu);
-		for (uint64_t id : take) reserved -= reqs[id].reserve;
-		return false;
+		if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: give the budget back and retry later
+			for (uint64_t id : take) reserved -= reqs[id].reserve;
+			return false;
+		}
 	}
-	Slot& s = ps.slots[slot];
-	s.reqs	= take;
+	Slot& s		= ps.slots[slot];
+	s.reqs		= take;
+	s.ring_off	= ring_off;
+	s.ring_live = true;
 	s.offs.resize(take.size());
 	s.lens.resize(take.size());
 	const uint64_t t = now_ns();
@@ -332,7 +311,6 @@ bool pars_server::launch(uint32_t p) {
 	if (r == PARS_OK)
 		r = batch_enqueue(s.sc, d_in, bytes, s.offs.data(), s.lens.data(), take.size(), ps.compute,
 						  cfg.run_pairs != 0, &s.ptrs);
-	const size_t nd = take.size();
 	if (r == PARS_OK) {	 // one copy brings every per-request result back
 		e = hipMemcpyAsync(s.h_pack, s.ptrs.pack, s.ptrs.pack_bytes, hipMemcpyDeviceToHost, ps.compute);
 		if (e != hipSuccess)
@@ -340,23 +318,17 @@ bool pars_server::launch(uint32_t p) {
 	}
 	if (r == PARS_OK && (e = hipEventRecord(s.done, ps.compute)) != hipSuccess)
 		r = PARS_ERR_DEVICE;
-	if (r != PARS_OK) {	 // the device is in trouble: fail the batch's requests now
+	if (r != PARS_OK) {
+		// The device is in trouble.  Whatever was enqueued for this batch is
+		// drained, and the batch joins the running queue as failed: retire()
+		// fails its requests and frees its ring and staging regions in launch
+		// order, after the batches still running ahead of it (their regions are
+		// older and stay untouched).
+		(void)hipStreamSynchronize(ps.copy);
+		(void)hipStreamSynchronize(ps.compute);
 		std::lock_guard<std::mutex> lk(mu);
-		error = std::string("batch launch failed: ") + g_err;
-		for (size_t d = 0; d < nd; ++d) {
-			Request& q	  = reqs[take[d]];
-			q.res.id	  = q.id;
-			q.res.status  = r;
-			q.res.plan	  = p;
-			q.state		  = kDone;
-			q.t_done	  = now_ns();
-			completed.push_back(q.id);
-		}
-		ps.dev_ring.release_oldest();
-		for (size_t d = 0; d < nd; ++d) ps.stage_ring.release_oldest();
-		s.state = kHeld;
-		cv_done.notify_all();
-		return false;
+		error		  = std::string("batch launch failed: ") + last_error();
+		s.fail_status = r;
 	}
 	s.state		= kRunning;
 	s.launch_ns = t;
@@ -532,8 +504,13 @@ int pars_hip_server_release(pars_server* sv, uint64_t id) {
 		Slot& s = sv->plans[q.plan].slots[q.slot];
 		if (s.outstanding)
 			s.outstanding--;
-		if (s.outstanding == 0 && s.state == kHeld)
+		if (s.outstanding == 0 && s.state == kHeld) {
 			s.state = kFree;
+			if (s.ring_live) {	// its inputs are no longer needed
+				sv->plans[q.plan].dev_ring.release(s.ring_off);
+				s.ring_live = false;
+			}
+		}
 	}
 	sv->reserved -= std::min(sv->reserved, q.reserve);
 	sv->st.released++;
@@ -549,6 +526,12 @@ int pars_hip_server_stats(pars_server* sv, pars_server_stats* out) {
 	std::lock_guard<std::mutex> lk(sv->mu);
 	*out				   = sv->st;
 	out->inflight_reserved = sv->reserved;
+	out->ring_wraps		   = 0;
+	out->staging_wraps	   = 0;
+	for (const PlanState& p : sv->plans) {
+		out->ring_wraps += p.dev_ring.wraps;
+		out->staging_wraps += p.stage_ring.wraps;
+	}
 	return PARS_OK;
 }
 

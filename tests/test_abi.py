@@ -107,3 +107,15 @@ def test_batch_workload_is_deterministic():
     for k in ("json", "csv"):
         assert np.array_equal(a[k][0], b[k][0])
         assert len(a[k][1]) == len(a[k][2])
+
+
+def test_server_errors_reach_last_error():
+    """A server call that fails before touching the device records its message
+    where pars_hip_last_error() reads it (one error state for the library)."""
+    lib = N.lib()
+    cfg = N.ServerConfig(None, 0, 0, 0, 0, 0, 0, 0, 0)
+    h = C.c_void_p()
+    assert lib.pars_hip_server_create(C.byref(cfg), C.byref(h)) == 1  # PARS_ERR_ARGUMENT
+    assert b"at least one plan" in lib.pars_hip_last_error()
+    assert lib.pars_hip_server_submit(None, 0, None, 0, C.byref(C.c_uint64())) == 1
+    assert b"submit" in lib.pars_hip_last_error()

@@ -1,9 +1,11 @@
-"""Benchmark: the plain JSON pipeline on one seeded 2 GiB document, one MI355X.
+"""Benchmark: the plain JSON pipeline on one seeded 2 GiB document, one MI355X,
+and beside it the 1 GiB six-mode C-like document, the 100k-document batch and
+the serving loop.
 
 A step is one device-resident scan of the whole document through the C ABI
 (pars_hip_scan_device: R1 classify .. R5 compaction in one kernel, then R6
-pairing; the reference's N1b tier, bench/bench_boundary.cu:252-260), with
-the input already in HBM.  `value` = input bytes / step time / 1e9 (the
+pairing; the reference's N1b tier, bench/bench_boundary.cu:252-260), with the
+input already in HBM.  `value` = input bytes / step time / 1e9 (the
 reference's GB/s, bench/bench_boundary.cu:31-33).
 
 roofline: the dominant kernel is the fused R1-R5 scan; its duration comes from
@@ -16,19 +18,21 @@ with k, k_b the document's structural and bracket counts.  peak = 8 TB/s.
 counters (FETCH_SIZE + WRITE_SIZE, calibrated on a plain device copy); it is
 read from the committed profile summary of this workload when present.
 
-cpu_baseline: the reference's own CPU backend (model::scanner + pair_brackets,
-compiled from its sources into oracle/_ref/) on a bounded prefix of the same
-document, one core; the C restatement under oracle/ stands in ("port") where
-that binary was not built.
+cpu_baseline: the C restatement of the reference's CPU path
+(model::scanner::scan + model::pair_brackets, oracle/pars_oracle.c) built
+-O3 -march=native for this host, pinned to one core, timed over the whole
+document (C2, C3), every document of the batch (C4) and every request of the
+serving stream (C5).  The GPU's outputs are compared with what it produced:
+counts, flags and FNV-1a hashes of the positions and pair tables.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -38,14 +42,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "input GB/s + % HBM roofline (2 GiB JSON); batched/served vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
-SCAN_KERNEL = "k_scan<EscapeEval, 4> (R1-R5)"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_scan_json_2gib.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-def measured_traffic(n: int) -> int | None:
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(name: str, n: int) -> int | None:
     """HBM bytes per scan launch from the committed counter summary (same workload size)."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(PROFILES, name)) as f:
             fields = dict(
                 line.strip().lstrip("- ").split(": ", 1) for line in f if line.strip().lstrip("- ").startswith("traffic_")
             )
@@ -56,125 +63,150 @@ def measured_traffic(n: int) -> int | None:
         return None
 
 
-def log(msg: str) -> None:
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+# ---------------------------------------------------------------------------
+# CPU baseline
+# ---------------------------------------------------------------------------
+class Baseline:
+    """The restatement built for this host (-O3 -march=native), timed pinned to one core."""
 
+    def __init__(self):
+        from oracle import oracle as O
 
-def cpu_baseline(doc, sample_bytes: int) -> dict:
-    """Reference CPU path on the first `sample_bytes` of the document, one core."""
-    from oracle import oracle as O
+        self.O = O
+        self.dir = tempfile.mkdtemp(prefix="pars_cpu_")
+        self.lib, self.flags = O.native_lib(self.dir)
+        self.host = O.host_info()
 
-    sample = doc[:sample_bytes]
-    cores = 1
-    if O.ref_available():
-        with tempfile.NamedTemporaryFile(suffix=".json", delete=False) as f:
-            path = f.name
-        try:
-            sample.tofile(path)
-            cmd = [O.REF, "time", "json", path, str(sample.size)]
-            r = json.loads(subprocess.run(cmd, check=True, stdout=subprocess.PIPE, text=True).stdout)
-        finally:
-            os.unlink(path)
-        secs = r["scan_s"] + r["pair_s"]
-        kind = "reference"
-    else:
+    def run(self, plan: str, buf, offs, sizes, what: str, per_doc: bool = False) -> tuple[dict, dict]:
         from pars_amd import Plan
 
-        plan = Plan.load("json").blob
-        t0 = time.perf_counter()
-        s = O.scan(plan, sample)
-        O.pair(plan, s["bracket_char"])
-        secs = time.perf_counter() - t0
-        kind = "port"
-    return {
-        "value": round(sample.size / secs / 1e9, 4),
-        "unit": "GB/s",
-        "cores": cores,
-        "kind": kind,
-        "sample": f"first {sample.size >> 20} MiB of the 2 GiB document, scan + pair_brackets, {secs:.2f} s",
-    }
+        r = self.O.time_docs(self.lib, Plan.load(plan).blob, buf, offs, sizes, per_doc=per_doc)
+        line = {
+            "value": round(r["bytes"] / r["seconds"] / 1e9, 4),
+            "unit": "GB/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{what}: {r['bytes']} bytes, scan + pair_brackets, {r['seconds']:.2f} s",
+            "build": self.flags,
+            "pinned_core": r["core"],
+            **self.host,
+        }
+        return line, r
 
 
-def bench_batch(torch, n_docs: int, iters: int, plain_sample: int, tune: bool = True) -> dict:
-    """The 100k-document batch: 70% JSON / 30% CSV, log-normal sizes (median
-    4 KiB, sigma 1.85, clipped to [200 B, 4 MiB]), one batched launch chain per
-    plan (graph-replayed), against one plain launch chain per document."""
+def fnv(arr) -> int:
+    from oracle import oracle as O
+
+    return O.fnv(arr)
+
+
+_hip = None
+
+
+def d2h(ptr: int, count: int, dtype):
+    """Device-to-host copy through the HIP runtime torch loaded (for the parity check)."""
     import numpy as np
 
-    from pars_amd import Scanner
-    from pars_amd import _native as N
-    from pars_amd import workloads as W
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so.7")
+        _hip.hipMemcpy.restype = C.c_int
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    out = np.empty(count, dtype)
+    if count:
+        assert _hip.hipMemcpy(out.ctypes.data, C.c_void_p(ptr), out.nbytes, 2) == 0, "hipMemcpy"
+    return out
 
-    t0 = time.perf_counter()
-    b = W.batch(4, n_docs, {"json": 70, "csv": 30}, median=4096.0, sigma=1.85)
-    total = int(b["sizes"].sum())
-    log(f"batch: {n_docs} documents, {total} bytes, generated in {time.perf_counter() - t0:.1f} s")
+
+def parity_single(r, cpu: dict) -> dict:
+    """The GPU's outputs of one document against the baseline's."""
+    import numpy as np
+
+    pos = d2h(r.structural, r.n_struct, np.uint32)
+    pair = d2h(r.bracket_pair, r.n_brackets, np.int32) if r.bracket_pair else np.zeros(0, np.int32)
+    checks = {
+        "counts": (int(r.n_struct), int(r.n_brackets)) == (cpu["n_struct"], cpu["n_brackets"]),
+        "flags": (bool(r.balanced), bool(r.well_formed), int(r.max_depth), bool(r.unclosed_mode))
+        == (cpu["balanced"], cpu["well_formed"], cpu["max_depth"], cpu["unclosed"]),
+        "positions_fnv": fnv(pos) == cpu["hash_pos"],
+        "pairs_fnv": fnv(pair) == cpu["hash_pair"],
+    }
+    checks["equal"] = all(checks.values())
+    return checks
+
+
+# ---------------------------------------------------------------------------
+# one document (C2, C3)
+# ---------------------------------------------------------------------------
+def bench_document(torch, plan: str, d_ptr: int, n: int, steps: int, warmup: int, knobs: str, tune: bool,
+                   traffic_file: str, kernel: str):
+    """Timed steps of pars_hip_scan_device on a device-resident document."""
+    import numpy as np
+
+    from pars_amd import Plan, Scanner
+    from pars_amd import _native as N
+
+    sc = Scanner(Plan.load(plan))
+    sc.set_option(N.PARS_OPT_WANT_BITMAP, 1)
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
-    groups = []
-    for kind in ("json", "csv"):
-        buf, spans, idx = b[kind]
-        d = torch.from_numpy(buf).to("cuda")
-        sc = Scanner(kind)
-        sc.set_option(N.PARS_OPT_GRAPH, 1)
-        groups.append((kind, d, spans, sc, len(idx)))
-    torch.cuda.synchronize()
-    tuned = {}
-    if tune:
-        for kind, d, spans, sc, nd in groups:
-            t0 = time.perf_counter()
-            tuned[kind] = sc.autotune(d.data_ptr(), d.numel(), spans, stream=sp, reps=2)
-            log(f"batch autotune {kind} ({time.perf_counter() - t0:.1f} s): {tuned[kind]}")
-
-    def one_pass():
-        out = []
-        for kind, d, spans, sc, nd in groups:
-            r = sc.scan_batch(d.data_ptr(), d.numel(), spans, sp, want_pairs=False, run_pairs=True)
-            out.append((r.n_struct, r.n_brackets))
-        return out
-
-    first = one_pass()  # eager
-    for _ in range(2):  # capture, then replay
-        assert one_pass() == first
-    t0 = time.perf_counter()
-    for _ in range(iters):
-        one_pass()
-    t_batch = (time.perf_counter() - t0) / iters
-
-    # the same documents through one plain launch chain each (bounded sample, in document order)
-    plain_t, plain_docs, plain_bytes = 0.0, 0, 0
-    for kind, d, spans, sc, nd in groups:
-        k = max(1, plain_sample * nd // n_docs)
-        sc1 = Scanner(kind)
-        base = d.data_ptr()
-        sc1.scan_device(base + spans[0].offset, spans[0].size, sp)  # warm
+    st = N.StageTimes()
+    tuned = None
+    if knobs:
+        opts = {"rows": N.PARS_OPT_ROWS_PER_TILE, "lb_window": N.PARS_OPT_LOOKBACK_WINDOW,
+                "blocks_per_cu": N.PARS_OPT_BLOCKS_PER_CU, "radix_bits": N.PARS_OPT_RADIX_BITS}
+        tuned = {}
+        for kv in knobs.split(","):
+            k, v = kv.split("=")
+            sc.set_option(opts[k.strip()], int(v))
+            tuned[k.strip()] = int(v)
+        log(f"{plan}: fixed knobs {tuned}")
+    elif tune:
         t0 = time.perf_counter()
-        for i in range(min(k, nd)):
-            sc1.scan_device(base + spans[i].offset, spans[i].size, sp, want_pairs=False, run_pairs=True)
-        plain_t += time.perf_counter() - t0
-        plain_docs += min(k, nd)
-        plain_bytes += int(sum(spans[i].size for i in range(min(k, nd))))
-        sc1.close()
-    for g in groups:
-        g[3].close()
-    docs_s = n_docs / t_batch
-    plain_docs_s = plain_docs / plain_t
-    return {
-        "workload": "100k uneven documents, 70% JSON / 30% CSV, log-normal median 4 KiB (seed 4)",
-        "docs": n_docs,
-        "bytes": total,
-        "ms_per_batch": round(t_batch * 1e3, 3),
-        "docs_per_s": round(docs_s, 1),
-        "gb_per_s": round(total / t_batch / 1e9, 3),
-        "plain_docs_per_s": round(plain_docs_s, 1),
-        "plain_gb_per_s": round(plain_bytes / plain_t / 1e9, 4),
-        "plain_sample_docs": plain_docs,
-        "speedup_docs_per_s": round(docs_s / plain_docs_s, 1),
-        "n_struct": int(sum(x[0] for x in first)),
-        "n_brackets": int(sum(x[1] for x in first)),
-        "graph": True,
-        "tuned": tuned or None,
+        tuned = sc.autotune(d_ptr, n, stream=sp, reps=3)
+        log(f"{plan}: autotune ({time.perf_counter() - t0:.1f} s): {tuned}")
+    for i in range(warmup):
+        r = sc.scan_device(d_ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
+        log(f"{plan} warmup {i}: {r}")
+    scan_ms, pair_ms, counts = [], [], set()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = sc.scan_device(d_ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
+        scan_ms.append(st.p1)
+        pair_ms.append(st.p4)
+        counts.add((r.n_struct, r.n_brackets, r.balanced, r.well_formed))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert len(counts) == 1, f"steps disagree: {counts}"
+    ms_step = wall / steps * 1e3
+    k, kb = r.n_struct, r.n_brackets
+    b_alg = n + n / 8 + 4 * k + 4 * kb
+    t_scan = float(np.mean(scan_ms)) / 1e3
+    achieved = b_alg / t_scan / 1e9
+    out = {
+        "ms_per_step": ms_step,
+        "value": n / (ms_step / 1e3) / 1e9,
+        "n_struct": int(k),
+        "n_brackets": int(kb),
+        "max_depth": int(r.max_depth),
+        "tuned": tuned,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(traffic_file, n),
+            "b_alg_bytes": int(b_alg),
+            "kernel_ms": round(t_scan * 1e3, 4),
+        },
+        "e2e_roofline_frac": round(b_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stage_ms": {"r1_r5": round(float(np.mean(scan_ms)), 4), "r6": round(float(np.mean(pair_ms)), 4)},
     }
+    log(f"{plan}: {ms_step:.3f} ms/step; scan {out['stage_ms']['r1_r5']} ms, R6 {out['stage_ms']['r6']} ms")
+    return sc, r, out, sp
 
 
 def bench_stream(sc, doc, stream: int, reps: int = 3) -> dict:
@@ -214,7 +246,7 @@ def bench_tape(d_ptr: int, doc, stream: int, reps: int, cpu_sample: int, no_cpu:
     import numpy as np
     import torch
 
-    from pars_amd import Scanner
+    from pars_amd import Plan, Scanner
     from pars_amd import _native as N
 
     def run(sc, ptr, n):
@@ -248,23 +280,162 @@ def bench_tape(d_ptr: int, doc, stream: int, reps: int, cpu_sample: int, no_cpu:
         from oracle import oracle as O
 
         sample = np.ascontiguousarray(doc[:cpu_sample])
-        pos = O.scan(Plan_json(), sample)["positions"]
-        t0 = time.perf_counter()
-        r = O.json_tape(sample, pos)
-        dt = time.perf_counter() - t0
+        pos = O.scan(Plan.load("json").blob, sample)["positions"]
+        prev = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {sorted(prev)[0]})
+        try:
+            t0 = time.perf_counter()
+            r = O.json_tape(sample, pos)
+            dt = time.perf_counter() - t0
+        finally:
+            os.sched_setaffinity(0, prev)
         out["cpu_baseline"] = {"value": round(sample.size / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
                                "stage": "replay + JSON hook over a given structural index (oracle/json_dom.c)",
                                "sample": f"first {sample.size} bytes of the 2 GiB document, {r['nodes']} nodes"}
     return out
 
 
-def Plan_json() -> bytes:
-    from pars_amd import Plan
+# ---------------------------------------------------------------------------
+# C4: the batch
+# ---------------------------------------------------------------------------
+def bench_batch(torch, n_docs: int, iters: int, plain_sample: int, base: Baseline | None, tune: bool = True) -> dict:
+    """The 100k-document batch: 70% JSON / 30% CSV, log-normal sizes (median
+    4 KiB, sigma 1.85, clipped to [200 B, 4 MiB]), one batched launch chain per
+    plan (graph-replayed), against one plain launch chain per document."""
+    import numpy as np
 
-    return Plan.load("json").blob
+    from pars_amd import Scanner
+    from pars_amd import _native as N
+    from pars_amd import workloads as W
+
+    t0 = time.perf_counter()
+    b = W.batch(4, n_docs, {"json": 70, "csv": 30}, median=4096.0, sigma=1.85)
+    total = int(b["sizes"].sum())
+    log(f"batch: {n_docs} documents, {total} bytes, generated in {time.perf_counter() - t0:.1f} s")
+    cpu, cpu_runs = None, {}
+    if base is not None:
+        secs, nbytes = 0.0, 0
+        for kind in ("json", "csv"):
+            buf, spans, idx = b[kind]
+            offs = np.array([s.offset for s in spans], np.uint64)
+            sz = np.array([s.size for s in spans], np.uint64)
+            line, r = base.run(kind, buf, offs, sz, kind, per_doc=True)
+            cpu_runs[kind] = r
+            secs += r["seconds"]
+            nbytes += r["bytes"]
+        cpu = dict(line)
+        cpu["value"] = round(nbytes / secs / 1e9, 4)
+        cpu["docs_per_s"] = round(n_docs / secs, 1)
+        cpu["sample"] = (f"every document of the batch ({n_docs} documents, {nbytes} bytes) scanned and paired "
+                         f"alone, one after the other: {secs:.2f} s")
+        log(f"batch cpu baseline: {cpu}")
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    groups = []
+    for kind in ("json", "csv"):
+        buf, spans, idx = b[kind]
+        d = torch.from_numpy(buf).to("cuda")
+        sc = Scanner(kind)
+        sc.set_option(N.PARS_OPT_GRAPH, 1)
+        groups.append((kind, d, spans, sc, len(idx)))
+    torch.cuda.synchronize()
+    tuned = {}
+    if tune:
+        for kind, d, spans, sc, nd in groups:
+            t0 = time.perf_counter()
+            tuned[kind] = sc.autotune(d.data_ptr(), d.numel(), spans, stream=sp, reps=2)
+            log(f"batch autotune {kind} ({time.perf_counter() - t0:.1f} s): {tuned[kind]}")
+
+    def one_pass():
+        out = []
+        for kind, d, spans, sc, nd in groups:
+            r = sc.scan_batch(d.data_ptr(), d.numel(), spans, sp, want_pairs=False, run_pairs=True)
+            out.append((r.n_struct, r.n_brackets))
+        return out
+
+    first = one_pass()  # eager
+    for _ in range(2):  # capture, then replay
+        assert one_pass() == first
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        one_pass()
+    t_batch = (time.perf_counter() - t0) / iters
+    # parity of the last replay with the baseline's per-document outputs
+    parity = None
+    if cpu_runs:
+        ok = True
+        for kind, d, spans, sc, nd in groups:
+            r = sc.scan_batch(d.data_ptr(), d.numel(), spans, sp, want_pairs=False, run_pairs=True)
+            c = cpu_runs[kind]
+            ss = d2h(r.struct_start, nd + 1, np.uint32).astype(np.int64)
+            bs = d2h(r.bracket_start, nd + 1, np.uint32).astype(np.int64)
+            ok = ok and np.array_equal(np.diff(ss), c["doc_struct"].astype(np.int64))
+            ok = ok and np.array_equal(np.diff(bs), c["doc_brackets"].astype(np.int64))
+            ok = ok and np.array_equal(d2h(r.doc_flags, nd, np.uint8), c["doc_flags"])
+            ok = ok and np.array_equal(d2h(r.max_depth, nd, np.int32), c["doc_depth"])
+            ok = ok and fnv(d2h(r.structural, r.n_struct, np.uint32)) == c["hash_pos"]
+            ok = ok and fnv(d2h(r.bracket_pair, r.n_brackets, np.int32)) == c["hash_pair"]
+        parity = {"checked": "per-document counts, flags and max depth; hashes of every position and pair",
+                  "equal": bool(ok)}
+        log(f"batch parity: {parity}")
+
+    # the same documents through one plain launch chain each (bounded sample, in document order)
+    plain_t, plain_docs, plain_bytes = 0.0, 0, 0
+    for kind, d, spans, sc, nd in groups:
+        k = max(1, plain_sample * nd // n_docs)
+        sc1 = Scanner(kind)
+        base_ptr = d.data_ptr()
+        sc1.scan_device(base_ptr + spans[0].offset, spans[0].size, sp)  # warm
+        t0 = time.perf_counter()
+        for i in range(min(k, nd)):
+            sc1.scan_device(base_ptr + spans[i].offset, spans[i].size, sp, want_pairs=False, run_pairs=True)
+        plain_t += time.perf_counter() - t0
+        plain_docs += min(k, nd)
+        plain_bytes += int(sum(spans[i].size for i in range(min(k, nd))))
+        sc1.close()
+    for g in groups:
+        g[3].close()
+    docs_s = n_docs / t_batch
+    plain_docs_s = plain_docs / plain_t
+    n_struct = int(sum(x[0] for x in first))
+    n_br = int(sum(x[1] for x in first))
+    b_alg = total + total / 8 + 4 * n_struct + 4 * n_br
+    out = {
+        "workload": "100k uneven documents, 70% JSON / 30% CSV, log-normal median 4 KiB (seed 4)",
+        "docs": n_docs,
+        "bytes": total,
+        "ms_per_batch": round(t_batch * 1e3, 3),
+        "docs_per_s": round(docs_s, 1),
+        "gb_per_s": round(total / t_batch / 1e9, 3),
+        "plain_docs_per_s": round(plain_docs_s, 1),
+        "plain_gb_per_s": round(plain_bytes / plain_t / 1e9, 4),
+        "plain_sample_docs": plain_docs,
+        "speedup_docs_per_s": round(docs_s / plain_docs_s, 1),
+        "n_struct": n_struct,
+        "n_brackets": n_br,
+        "graph": True,
+        "tuned": tuned or None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "the whole batch chain per plan (prep, scan, R6, flags; graph replay), wall clock",
+            "achieved": round(b_alg / t_batch / 1e9, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(b_alg / t_batch / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "b_alg_bytes": int(b_alg),
+        },
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["parity"] = parity
+    return out
 
 
-def bench_serve(n_req: int, seed: int = 5) -> dict:
+# ---------------------------------------------------------------------------
+# C5: serving
+# ---------------------------------------------------------------------------
+def bench_serve(n_req: int, base: Baseline | None, seed: int = 5) -> dict:
     """Serving: Poisson arrivals of JSON / CSV / C-like requests (60/25/15,
     log-normal sizes, median 4 KiB, sigma 1.85, clipped to [200 B, 4 MiB])
     through a 4 GiB device ring under a 64 GiB HBM budget.  Saturation is
@@ -284,6 +455,26 @@ def bench_serve(n_req: int, seed: int = 5) -> dict:
     pidx = [plans.index(k) for k in kinds]
     total = int(sz.sum())
     log(f"serve: {n_req} requests, {total} bytes, generated in {time.perf_counter() - t0:.1f} s")
+    cpu = None
+    if base is not None:
+        secs, nbytes = 0.0, 0
+        for p in plans:
+            sel = [i for i, k in enumerate(kinds) if k == p]
+            if not sel:
+                continue
+            buf = np.concatenate([docs[i] for i in sel])
+            sizes = np.array([docs[i].size for i in sel], np.uint64)
+            offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.uint64)
+            line, r = base.run(p, buf, offs, sizes, p)
+            secs += r["seconds"]
+            nbytes += r["bytes"]
+        cpu = dict(line)
+        cpu["value"] = round(nbytes / secs / 1e9, 4)
+        cpu["us_per_request"] = round(secs / n_req * 1e6, 2)
+        cpu["requests_per_s"] = round(n_req / secs, 1)
+        cpu["sample"] = (f"every request of the stream ({n_req} requests, {nbytes} bytes) scanned and paired alone, "
+                         f"one after the other: {secs:.2f} s")
+        log(f"serve cpu baseline: {cpu}")
     sv = Server(plans)  # 4 GiB ring, 1 GiB staging, 64 GiB budget, 64 MiB batches
     sv.replay(pidx[:200], docs[:200], np.zeros(200, np.uint64))  # warm every plan's kernels
     lat, el = sv.replay(pidx, docs, np.zeros(n_req, np.uint64))
@@ -309,7 +500,7 @@ def bench_serve(n_req: int, seed: int = 5) -> dict:
     st = sv.stats()
     sv.close()
     at80 = next(p for p in points if p["load_pct"] == 80)
-    return {
+    out = {
         "workload": "Poisson requests, 60% JSON / 25% CSV / 15% C-like, log-normal median 4 KiB (seed 5)",
         "requests_per_point": n_req,
         "bytes_per_point": total,
@@ -320,8 +511,13 @@ def bench_serve(n_req: int, seed: int = 5) -> dict:
         "gb_per_s_at_80": at80["gb_per_s"],
         "points": points,
         "batches": int(st.batches),
+        "ring_wraps": int(st.ring_wraps),
+        "staging_wraps": int(st.staging_wraps),
         "device_bytes": int(st.device_bytes),
     }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    return out
 
 
 def main() -> int:
@@ -330,10 +526,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size-mib", type=int, default=2048)
-    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip every CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain device copy of the document (counter calibration under rocprofv3)")
+    ap.add_argument("--no-clike", action="store_true", help="skip the 1 GiB six-mode C-like document")
+    ap.add_argument("--only-clike", action="store_true", help="run only the 1 GiB six-mode C-like document")
+    ap.add_argument("--clike-mib", type=int, default=1024)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-documents measurement")
     ap.add_argument("--only-batch", action="store_true", help="run only the batched-documents measurement")
     ap.add_argument("--batch-docs", type=int, default=100_000)
@@ -347,7 +545,7 @@ def main() -> int:
     ap.add_argument("--no-tape", action="store_true", help="skip the JSON tape (stage-2 hook) measurement")
     ap.add_argument("--tape-cpu-mib", type=int, default=256, help="CPU baseline sample of the JSON tape")
     ap.add_argument("--knobs", default="",
-                    help="fixed launch knobs for the document scan instead of tuning, e.g. lb_window=16,rows=4 "
+                    help="fixed launch knobs for the document scans instead of tuning, e.g. lb_window=16,rows=4 "
                          "(rows, lb_window, blocks_per_cu, radix_bits); profiles use this")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -356,22 +554,66 @@ def main() -> int:
 
     import numpy as np
 
-    from pars_amd import Plan, Scanner
-    from pars_amd import _native as N
     from pars_amd import workloads as W
 
-    if args.only_serve:
+    base = None if args.no_cpu else Baseline()
+    if base is not None:
+        log(f"cpu baseline build: {base.flags}; host {base.host}")
+
+    if args.only_serve or args.only_batch:
         import torch
 
         torch.cuda.init()
-        print(json.dumps({"served": bench_serve(args.serve_requests)}), flush=True)
+        if args.only_serve:
+            print(json.dumps({"served": bench_serve(args.serve_requests, base)}), flush=True)
+        else:
+            r = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, base, tune=not args.no_tune)
+            print(json.dumps({"batched": r}), flush=True)
         return 0
-    if args.only_batch:
+
+    def clike_line(torch) -> dict:
+        n3 = args.clike_mib << 20
+        log(f"generating {n3} bytes of six-mode C-like source (seed 3, depth 24)")
+        src = W.clike(3, n3, 24)
+        cpu3 = raw3 = None
+        if base is not None:
+            cpu3, raw3 = base.run("clike6", src, [0], [src.size], f"the whole {n3 >> 20} MiB C-like document")
+            log(f"clike6 cpu baseline: {cpu3}")
+        d3 = torch.from_numpy(src).to("cuda")
+        torch.cuda.synchronize()
+        sc3, r3, m3, _ = bench_document(torch, "clike6", d3.data_ptr(), n3, max(args.steps // 2, 3), args.warmup,
+                                        args.knobs, not args.no_tune, "k_scan_clike6_1gib.md",
+                                        "k_scan<GeneralEval<5, true>, 1> (R1-R5)")
+        line3 = {
+            "workload": f"clike6-{n3 >> 20}MiB plain pipeline (R1-R6, device-resident, N1b; six modes, 5 descriptors)",
+            "bytes": n3,
+            "seed": 3,
+            "metric": "input GB/s",
+            "value": round(m3["value"], 3),
+            "unit": "GB/s",
+            "ms_per_step": round(m3["ms_per_step"], 4),
+            "max_depth": m3["max_depth"],
+            "n_struct": m3["n_struct"],
+            "n_brackets": m3["n_brackets"],
+            "tuned": m3["tuned"],
+            "roofline": m3["roofline"],
+            "e2e_roofline_frac": m3["e2e_roofline_frac"],
+            "stage_ms": m3["stage_ms"],
+        }
+        if cpu3 is not None:
+            line3["cpu_baseline"] = cpu3
+            line3["parity"] = parity_single(r3, raw3)
+            log(f"clike6 parity: {line3['parity']}")
+        sc3.close()
+        del d3
+        torch.cuda.empty_cache()
+        return line3
+
+    if args.only_clike:
         import torch
 
         torch.cuda.init()
-        r = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, tune=not args.no_tune)
-        print(json.dumps({"batched": r}), flush=True)
+        print(json.dumps({"clike6": clike_line(torch)}), flush=True)
         return 0
 
     n = args.size_mib << 20
@@ -379,11 +621,10 @@ def main() -> int:
     t0 = time.perf_counter()
     doc = W.json(2, n, 32, 0.30)
     str_frac = W.json.last_string_bytes / n
-    log(f"generated in {time.perf_counter() - t0:.1f} s, string fraction {str_frac:.3f}")
-    cpu = None
-    if not args.no_cpu:  # before the GPU is initialised: the baseline runs as a child process
-        log("cpu baseline")
-        cpu = cpu_baseline(doc, min(n, args.cpu_sample_mib << 20))
+    log(f"generated in {time.perf_counter() - t0:.1f} s, string fraction {str_frac:.4f}")
+    cpu = raw = None
+    if base is not None:
+        cpu, raw = base.run("json", doc, [0], [n], f"the whole {n >> 20} MiB document")
         log(f"cpu baseline: {cpu}")
 
     import torch
@@ -391,7 +632,6 @@ def main() -> int:
     torch.cuda.init()
     d_doc = torch.from_numpy(doc).to("cuda")
     torch.cuda.synchronize()
-
     if args.copy_calibration:
         for _ in range(2):
             d_copy = d_doc.clone()
@@ -399,58 +639,16 @@ def main() -> int:
             del d_copy
         log(f"copy calibration: 2 device copies of {n} bytes")
 
-    sc = Scanner(Plan.load("json"))
-    sc.set_option(N.PARS_OPT_WANT_BITMAP, 1)
-    stream = torch.cuda.Stream()
-    sp = stream.cuda_stream
-    st = N.StageTimes()
-    ptr = d_doc.data_ptr()
-    tuned = None
-    if args.knobs:
-        opts = {"rows": N.PARS_OPT_ROWS_PER_TILE, "lb_window": N.PARS_OPT_LOOKBACK_WINDOW,
-                "blocks_per_cu": N.PARS_OPT_BLOCKS_PER_CU, "radix_bits": N.PARS_OPT_RADIX_BITS}
-        tuned = {}
-        for kv in args.knobs.split(","):
-            k, v = kv.split("=")
-            sc.set_option(opts[k.strip()], int(v))
-            tuned[k.strip()] = int(v)
-        log(f"fixed knobs: {tuned}")
-    elif not args.no_tune:
-        t0 = time.perf_counter()
-        tuned = sc.autotune(ptr, n, stream=sp, reps=3)
-        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}")
-
-    for i in range(args.warmup):
-        r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
-        log(f"warmup {i}: {r}")
-
-    scan_ms, pair_ms, counts = [], [], set()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        r = sc.scan_device(ptr, n, sp, want_pairs=False, run_pairs=True, stage_times=st)
-        scan_ms.append(st.p1)
-        pair_ms.append(st.p4)
-        counts.add((r.n_struct, r.n_brackets, r.balanced, r.well_formed))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = wall / args.steps * 1e3
-    assert len(counts) == 1, f"steps disagree: {counts}"
-    k, kb = r.n_struct, r.n_brackets
-    log(f"steps done: {ms_step:.3f} ms/step; scan kernel {np.mean(scan_ms):.3f} ms, pairing {np.mean(pair_ms):.3f} ms")
-
-    b_alg = n + n / 8 + 4 * k + 4 * kb
-    t_scan = float(np.mean(scan_ms)) / 1e3
-    achieved = b_alg / t_scan / 1e9
-    value = n / (ms_step / 1e3) / 1e9
+    sc, r, m, sp = bench_document(torch, "json", d_doc.data_ptr(), n, args.steps, args.warmup, args.knobs,
+                                  not args.no_tune, "k_scan_json_2gib.md", "k_scan<EscapeEval, 4> (R1-R5)")
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(m["value"], 3),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(m["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -460,32 +658,24 @@ def main() -> int:
             "workload": "json-2GiB plain pipeline (R1-R6, device-resident, N1b)",
             "bytes": n,
             "seed": 2,
-            "max_depth": int(r.max_depth),
+            "max_depth": m["max_depth"],
             "string_fraction": round(str_frac, 4),
-            "n_struct": int(k),
-            "n_brackets": int(kb),
+            "n_struct": m["n_struct"],
+            "n_brackets": m["n_brackets"],
             "parallelism": "single GPU",
-            "tuned": tuned,
+            "tuned": m["tuned"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": SCAN_KERNEL,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(n),
-            "b_alg_bytes": int(b_alg),
-            "kernel_ms": round(t_scan * 1e3, 4),
-        },
-        "e2e_roofline_frac": round(b_alg / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-        "stage_ms": {"r1_r5": round(float(np.mean(scan_ms)), 4), "r6": round(float(np.mean(pair_ms)), 4)},
+        "roofline": m["roofline"],
+        "e2e_roofline_frac": m["e2e_roofline_frac"],
+        "stage_ms": m["stage_ms"],
     }
     if cpu is not None:
         line["cpu_baseline"] = cpu
+        line["parity"] = parity_single(r, raw)
+        log(f"parity: {line['parity']}")
     if not args.no_stream:
         line["host_stream"] = bench_stream(sc, doc, sp)
-        assert line["host_stream"]["n_struct"] == k, "streamed scan disagrees with the device-resident one"
+        assert line["host_stream"]["n_struct"] == m["n_struct"], "streamed scan disagrees with the device-resident one"
         log(f"host stream: {line['host_stream']}")
     if not args.no_tape:
         line["json_tape"] = bench_tape(d_doc.data_ptr(), doc, sp, 5, args.tape_cpu_mib << 20, args.no_cpu)
@@ -493,13 +683,15 @@ def main() -> int:
     sc.close()
     del d_doc
     torch.cuda.empty_cache()
+    if not args.no_clike:
+        line["clike6"] = clike_line(torch)
     if not args.no_batch:
-        line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample,
+        line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, base,
                                       tune=not args.no_tune)
-        log(f"batched: {line['batched']}")
+        log(f"batched: { {k: v for k, v in line['batched'].items() if k != 'tuned'} }")
     if not args.no_serve:
         torch.cuda.empty_cache()
-        line["served"] = bench_serve(args.serve_requests)
+        line["served"] = bench_serve(args.serve_requests, base)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -48,8 +48,9 @@ typedef enum {
 } pars_status;
 
 /* pipeline.cuh:745-748: per-stage milliseconds measured with events on the
- * call's own stream.  p1 = R1-R5 (one fused kernel), p2 = 0, p3 = 0 unless the
- * staged mode splits them, p4 = R6 pairing. */
+ * call's own stream.  p1 = R1-R5 (one fused kernel: the stages have no
+ * separate launches), p2 = the JSON tape's kernels (pars_hip_json_tape; else
+ * 0), p3 = 0, p4 = R6 pairing (all of its kernels; a kernel trace splits them). */
 typedef struct {
 	double p1, p2, p3, p4;
 } pars_stage_times;

@@ -215,3 +215,78 @@ def batch_full(plan: bytes, buf: np.ndarray, spans) -> dict:
     return {"positions": cat(pos, np.uint32), "struct_start": st, "bracket_pos": cat(bpos, np.uint32),
             "bracket_char": cat(bch, np.uint8), "bracket_start": bst, "pair": cat(pr, np.int32),
             "flags": flags, "max_depth": depth}
+
+
+# ---------------------------------------------------------------------------
+# the timed CPU baseline (baseline.c)
+# ---------------------------------------------------------------------------
+class Run(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("bytes", C.c_uint64), ("n_struct", C.c_uint64),
+                ("n_brackets", C.c_uint64), ("hash_pos", C.c_uint64), ("hash_pair", C.c_uint64),
+                ("balanced", C.c_int32), ("well_formed", C.c_int32), ("max_depth", C.c_int32),
+                ("unclosed", C.c_int32)]
+
+
+SOURCES = ["pars_oracle.c", "json_dom.c", "baseline.c"]
+NATIVE_FLAGS = ["-O3", "-march=native", "-std=gnu99", "-fPIC", "-shared"]
+
+
+def native_lib(out_dir: str):
+    """The restatement built -O3 -march=native for the host that runs it
+    (gcc on that host, into out_dir).  Returns (CDLL, flags string), or the
+    prebuilt portable liboracle.so and its flags when gcc is unavailable."""
+    dst = os.path.join(out_dir, "liboracle_native.so")
+    try:
+        subprocess.run(["gcc", *NATIVE_FLAGS, "-o", dst] + [os.path.join(HERE, s) for s in SOURCES], check=True,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+        L, flags = C.CDLL(dst), " ".join(NATIVE_FLAGS[:2])
+    except (OSError, subprocess.SubprocessError):
+        L, flags = C.CDLL(LIB), "-O2 (prebuilt; gcc unavailable on this host)"
+    L.oracle_time_docs.restype = C.c_int
+    L.oracle_time_docs.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Run)]
+    return L, flags
+
+
+def host_info() -> dict:
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def time_docs(L, plan: bytes, buf: np.ndarray, offs, sizes, per_doc: bool = False, core: int | None = None) -> dict:
+    """Run the baseline over documents buf[offs[d], +sizes[d]) pinned to one
+    core (the first the process may use, unless given); returns the timing,
+    totals, hashes, and per-document counts / flags / depths when asked."""
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    offs = np.ascontiguousarray(offs, dtype=np.uint64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+    nd = offs.size
+    arrs = None
+    if per_doc:
+        arrs = (np.zeros(nd, np.uint32), np.zeros(nd, np.uint32), np.zeros(nd, np.uint8), np.zeros(nd, np.int32))
+    prev = os.sched_getaffinity(0)
+    core = sorted(prev)[0] if core is None else core
+    os.sched_setaffinity(0, {core})
+    try:
+        r = Run()
+        rc = L.oracle_time_docs(plan, buf.ctypes.data, offs.ctypes.data, sizes.ctypes.data, nd,
+                                *(a.ctypes.data for a in arrs) if arrs else (None, None, None, None), C.byref(r))
+    finally:
+        os.sched_setaffinity(0, prev)
+    if rc:
+        raise MemoryError("oracle_time_docs")
+    out = {"seconds": r.seconds, "bytes": int(r.bytes), "n_struct": int(r.n_struct), "n_brackets": int(r.n_brackets),
+           "hash_pos": int(r.hash_pos), "hash_pair": int(r.hash_pair), "balanced": bool(r.balanced),
+           "well_formed": bool(r.well_formed), "max_depth": int(r.max_depth), "unclosed": bool(r.unclosed),
+           "core": core}
+    if arrs:
+        out.update({"doc_struct": arrs[0], "doc_brackets": arrs[1], "doc_flags": arrs[2], "doc_depth": arrs[3]})
+    return out

@@ -58,6 +58,22 @@ void oracle_final_states(
  * 0xcbf29ce484222325). */
 uint64_t oracle_fnv1a(uint64_t h, const void* p, size_t n);
 
+/* baseline.c: the restatement timed over documents buf[offs[d], +sizes[d]).
+ * Per-document outputs (any array may be NULL): structural and bracket
+ * counts, flags (bit0 balanced, bit1 well_formed, bit2 unclosed_mode) and max
+ * depth.  Returns 0, or -1 when out of memory. */
+typedef struct {
+	double	 seconds;	  /* scan + pair only */
+	uint64_t bytes, n_struct, n_brackets;
+	uint64_t hash_pos;	  /* FNV-1a 64 chain over every document's positions, in order */
+	uint64_t hash_pair;	  /* the same over the pair tables (document-local bracket indices) */
+	int32_t	 balanced, well_formed, max_depth, unclosed; /* of a single document */
+} oracle_run;
+
+int oracle_time_docs(const pars_plan* S, const uint8_t* buf, const uint64_t* offs, const uint64_t* sizes, size_t nd,
+					 uint32_t* doc_struct, uint32_t* doc_brackets, uint8_t* doc_flags, int32_t* doc_depth,
+					 oracle_run* out);
+
 /* The JSON hook's results (json_dom.c). */
 typedef struct {
 	uint64_t n_nodes;	 /* nodes pushed (the arrays hold the first cap_nodes) */

@@ -105,7 +105,7 @@ struct pars_scanner {
 	DevBuf	  recs, ticket, summary;
 	DevBuf	  positions, chars, bitmap, pair_pos;
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
-	DevBuf	  stk_res, stk_rec, stk_st;	 // stack pairing: chunk residues, chunk depths, their sparse table
+	DevBuf	  stk_res, stk_rec, stk_g1, stk_g2;	 // stack pairing: chunk residues, chunk depths, group minima
 	bool	  stack_pending = false;	 // the last enqueued R6 was the stack path (sort path if unbalanced)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
@@ -147,7 +147,7 @@ struct pars_scanner {
 	~pars_scanner() {
 		if (gexec)
 			(void)hipGraphExecDestroy(gexec);
-		for (DevBuf* b : {&stk_res, &stk_rec, &stk_st})
+		for (DevBuf* b : {&stk_res, &stk_rec, &stk_g1, &stk_g2})
 			b->release();
 		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
 						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
@@ -228,8 +228,9 @@ int ensure_outputs(pars_scanner* sc, uint64_t n_tiles, uint64_t cap_struct, uint
 	if ((r = sc->hist.ensure(256 * std::max<uint64_t>(chunks, 1) * 4)))
 		return r;
 	const uint64_t schunks = (cap_br + kStkChunk - 1) / kStkChunk;
+	// group minima start zeroed: epoch 0 is never a launch's, so they read as unset
 	if ((r = sc->stk_res.ensure(cap_br * 4)) || (r = sc->stk_rec.ensure(std::max<uint64_t>(schunks, 1) * 12))
-		|| (r = sc->stk_st.ensure(std::max<uint64_t>(schunks, 1) * stack_levels(schunks) * 4)))
+		|| (r = sc->stk_g1.ensure((schunks / 64 + 1) * 8, true)) || (r = sc->stk_g2.ensure((schunks / 4096 + 1) * 8, true)))
 		return r;
 	if ((r = sc->hist_part.ensure(256 * 4)))
 		return r;
@@ -300,9 +301,9 @@ StackArgs make_stack_args(pars_scanner* sc, bool want_pairs) {
 	a.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
 	a.res		 = sc->stk_res.as<uint32_t>();
 	a.crec		 = sc->stk_rec.as<int32_t>();
-	a.cst		 = sc->stk_st.as<int32_t>();
+	a.gmin1		 = sc->stk_g1.as<unsigned long long>();
+	a.gmin2		 = sc->stk_g2.as<unsigned long long>();
 	a.max_chunks = std::max<uint64_t>((sc->cap_br + kStkChunk - 1) / kStkChunk, 1);
-	a.st_levels	 = stack_levels(a.max_chunks);
 	return a;
 }
 

@@ -174,8 +174,8 @@ struct JsonGen {
 			w[4] = w[5] = wn;
 		} else if (deficit == 1) {
 			w[0] = 40, w[1] = 6, w[2] = 6, w[3] = 4, w[4] = can ? 2 : 0, w[5] = can ? 2 : 0;
-		} else {
-			w[0] = 0, w[1] = 12, w[2] = 12, w[3] = 4, w[4] = can ? 16 / (uint32_t)depth : 0, w[5] = can ? 1 : 0;
+		} else {  // above the target: numbers and key-free number arrays dilute the keys' strings
+			w[0] = 0, w[1] = 12, w[2] = 12, w[3] = 4, w[4] = can ? 32 / (uint32_t)depth : 0, w[5] = 0;
 		}
 		uint32_t tot = 0;
 		for (uint32_t v : w) tot += v;

@@ -228,7 +228,7 @@ hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
 // R6 for balanced documents by stack matching (pair_stack.hip): the same pairs
 // the sort gives when the depth never drops below 0 and ends at 0; the kernels
 // exit at once otherwise and the sorting path runs instead.
-constexpr uint32_t kStkChunk  = 16384;		  // brackets per chunk (1024 threads x 16)
+constexpr uint32_t kStkChunk  = 8192;		  // brackets per chunk (256 threads x 32)
 constexpr uint32_t kStackLost = 0x10u;		  // summary overflow bit: a residue close found no open (internal)
 struct StackArgs {
 	DevSummary*		sum;
@@ -241,13 +241,12 @@ struct StackArgs {
 	int32_t*		pair_pos;  // optional: structural-index convention
 	uint32_t*		res;	   // [cap_br] per chunk: unmatched closes, then unmatched opens (bracket indices)
 	int32_t*		crec;	   // [3 max_chunks] per chunk: entry, minimum and exit depth
-	int32_t*		cst;	   // [st_levels max_chunks] sparse table of chunk minima
+	// minima of groups of 64 and of 4096 chunks, tagged with the launch epoch
+	unsigned long long* gmin1;	// [max_chunks / 64 + 1]
+	unsigned long long* gmin2;	// [max_chunks / 4096 + 1]
 	uint64_t		max_chunks;
-	uint32_t		st_levels;
 };
 hipError_t launch_stack_pairing(const StackArgs& a, hipStream_t s);
-// Sparse-table levels for up to max_chunks chunks.
-uint32_t stack_levels(uint64_t max_chunks);
 
 // Fill n int32 values (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t s);

@@ -10,19 +10,21 @@
 // bracket at its key: exactly the partner a stack matcher gives.  The sort is
 // then unnecessary, and the stack runs in three levels:
 //
-//   thread   16 consecutive brackets, the stack a bit mask of unmatched opens
-//            (the partner of a close is the highest set bit) and a shift
-//            register of their expected closing bytes;
-//   chunk    1024 threads = 16384 brackets: a thread's unmatched closes sit at
-//            the levels [m, e) below its entry depth e, its unmatched opens at
-//            [m, x) below its exit depth x (m = its minimum depth).  The close
-//            at level k pairs with the open at level k of the last earlier
-//            thread whose minimum is <= k, found by binary lifting over a
-//            sparse table of the threads' minima in LDS;
+//   thread   32 consecutive brackets, the stack a bit mask of unmatched opens
+//            (the partner of a close is the highest set bit);
+//   chunk    256 threads = 8192 brackets, staged in LDS: a thread's unmatched
+//            closes sit at the levels [m, e) below its entry depth e, one per
+//            level, its unmatched opens at [m, x) below its exit depth x
+//            (m = its minimum depth).  The close at level k pairs with the
+//            open at level k of the last earlier thread whose minimum is <= k,
+//            found by binary lifting over a sparse table of the threads' minima
+//            in LDS; partners are collected in LDS, checked and stored
+//            coalesced;
 //   document what no thread of a chunk matches is the chunk's residue - one
 //            close per level below its entry depth, one open per level below
-//            its exit depth - matched across chunks the same way, through a
-//            sparse table of the chunks' minima in global memory.
+//            its exit depth - matched across chunks by one wave per chunk that
+//            walks back to the nearest chunk whose minimum is below the levels
+//            still open, skipping groups of 64 and 4096 chunks by their minima.
 //
 // An unbalanced document (min depth < 0 or final depth != 0) takes the sorting
 // path (pair_kernels.hip) instead: these kernels exit at once, and the host
@@ -40,11 +42,13 @@ namespace pars_hip {
 
 namespace {
 
-constexpr int kStkThreads = 1024;
-constexpr int kStkItems	  = 16;
-constexpr int kStkLevels  = 10;						 // 2^9 = 512: lifting and range minima over 1024 threads
-constexpr int16_t kNoMin  = INT16_MAX;				 // a thread without brackets
-constexpr int kResolveBlock = 256;					 // four chunks per block in the resolve pass
+constexpr int	  kStkThreads = 256;
+constexpr int	  kStkItems	  = 32;
+constexpr int	  kStkLevels  = 8;	// 2^7 = 128: lifting and range minima over 256 threads
+constexpr int16_t kNoMin	  = INT16_MAX;	// a thread without brackets
+constexpr uint16_t kPartLater = 0xFFFEu;	// a chunk residue: the resolve pass writes it
+constexpr int	  kResolveBlock = 256;		// four chunks per block in the resolve pass
+static_assert(kStkThreads * kStkItems == (int)kStkChunk, "chunk = threads x items");
 
 __device__ __forceinline__ uint32_t stack_nb(const StackArgs& a) {
 	const uint32_t n = a.sum->n_brackets;
@@ -55,8 +59,28 @@ __device__ __forceinline__ bool stack_applies(const DevSummary* s) {
 	return s->min_bal >= 0 && s->last_bal == 0;
 }
 
-__device__ __forceinline__ uint32_t nib(uint64_t w, uint32_t j) {
-	return (uint32_t)(w >> (4 * j)) & 15u;
+// The j-th set bit (from 0) of m, which has more than j set bits.
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t j) {
+	uint32_t pos = 0, c;
+	c = (uint32_t)__popc(m & 0xFFFFu);
+	if (j >= c) j -= c, m >>= 16, pos += 16;
+	c = (uint32_t)__popc(m & 0xFFu);
+	if (j >= c) j -= c, m >>= 8, pos += 8;
+	c = (uint32_t)__popc(m & 0xFu);
+	if (j >= c) j -= c, m >>= 4, pos += 4;
+	c = (uint32_t)__popc(m & 0x3u);
+	if (j >= c) j -= c, m >>= 2, pos += 2;
+	return pos + ((j >= (m & 1u)) ? 1u : 0u);
+}
+
+// Group minima tagged with the launch epoch, kept by atomicMax: the newest
+// epoch wins, and within it the smallest minimum (stored inverted).
+__device__ __forceinline__ unsigned long long tag_min(uint32_t epoch, int32_t m) {
+	return ((unsigned long long)epoch << 32) | (uint32_t)(0x7FFFFFFF - m);
+}
+
+__device__ __forceinline__ int32_t read_min(unsigned long long v, uint32_t epoch) {
+	return (uint32_t)(v >> 32) == epoch ? 0x7FFFFFFF - (int32_t)(uint32_t)v : INT_MAX;
 }
 
 // The last index u < t with m[u] <= k, or -1 (st[j][i] = min m[i - 2^j + 1 .. i]).
@@ -70,26 +94,17 @@ __device__ __forceinline__ int last_at_most(const int16_t (*st)[kStkThreads], in
 }
 
 // min m[l .. r] for 0 <= l <= r < kStkThreads: two blocks of 2^j that cover
-// the range (the top level, 512, covers any range of up to 1024 twice over).
+// the range (the top level, 128, covers any range of up to 256 twice over).
 __device__ __forceinline__ int range_min(const int16_t (*st)[kStkThreads], int l, int r) {
 	const int j = min(31 - __clz(r - l + 1), kStkLevels - 1);
 	return min((int)st[j][l + (1 << j) - 1], (int)st[j][r]);
 }
 
-__device__ __forceinline__ uint32_t byte_of(const uint32_t (&w)[4], uint32_t q) {
-	const uint32_t x = (q & 8u) ? ((q & 4u) ? w[3] : w[2]) : ((q & 4u) ? w[1] : w[0]);
-	return (x >> (8 * (q & 3u))) & 0xFFu;
-}
-
-__device__ __forceinline__ void write_pair(const StackArgs& a, uint32_t i, int32_t v) {
-	a.pair[i] = v;
-	if (a.pair_pos)
-		a.pair_pos[a.bsidx[i]] = v >= 0 ? (int32_t)a.bsidx[v] : -1;
-}
-
 __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 	__shared__ int16_t	s_st[kStkLevels][kStkThreads];	// thread minima relative to the chunk's entry depth
-	__shared__ uint64_t s_ro[kStkThreads];	// a thread's unmatched opens: local indices, ascending level
+	__shared__ uint32_t s_ro[kStkThreads];				// a thread's unmatched opens (bit per item)
+	__shared__ uint8_t	s_ch[kStkChunk];				// the chunk's bracket bytes
+	__shared__ uint16_t s_part[kStkChunk];				// the chunk's partners (chunk-local) or kPartLater
 	__shared__ uint8_t	s_flags[256], s_cof[256];
 	__shared__ int32_t	s_wsum[kStkThreads / 64];
 	__shared__ int32_t	s_entry;
@@ -98,65 +113,63 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 	const int	   tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	const uint32_t nb  = stack_nb(a);
 	const uint32_t nch = (nb + kStkChunk - 1) / kStkChunk;
-	if (tid < 256) {
-		s_flags[tid] = a.plan->br_flags[tid];
-		s_cof[tid]	 = a.plan->br_close_of[tid];
-	}
+	const uint32_t epoch = a.sum->epoch;
+	s_flags[tid] = a.plan->br_flags[tid];
+	s_cof[tid]	 = a.plan->br_close_of[tid];
 	bool bad = false, lost = false;	 // lost: a residue outside its chunk's range (cannot happen)
 	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
-		__syncthreads();  // LDS of the previous chunk is free; tables loaded
-		const uint32_t base = c * kStkChunk, i0 = base + (uint32_t)tid * kStkItems;
-		const uint32_t cnt	= i0 < nb ? min(nb - i0, (uint32_t)kStkItems) : 0u;
-		uint32_t	   w[4] = {0u, 0u, 0u, 0u};
-		if (cnt == (uint32_t)kStkItems) {
-			const uint4 v = *reinterpret_cast<const uint4*>(a.bchar + i0);
-			w[0] = v.x, w[1] = v.y, w[2] = v.z, w[3] = v.w;
-		} else {
-			for (uint32_t q = 0; q < cnt; ++q) w[q >> 2] |= (uint32_t)a.bchar[i0 + q] << (8 * (q & 3));
+		const uint32_t base = c * kStkChunk, lim = min(nb - base, (uint32_t)kStkChunk);
+		int32_t		   key0 = 0;
+		if (tid == 0)
+			key0 = a.bkey[base];  // in flight while the bytes load
+		__syncthreads();		  // LDS of the previous chunk is free; tables loaded
+		// ---- the chunk's bytes into LDS, coalesced ----
+		for (uint32_t o = (uint32_t)tid * 16u; o < lim; o += kStkThreads * 16u) {
+			if (o + 16u <= lim) {
+				*reinterpret_cast<uint4*>(s_ch + o) = *reinterpret_cast<const uint4*>(a.bchar + base + o);
+			} else {
+				for (uint32_t q = o; q < lim; ++q) s_ch[q] = a.bchar[base + q];
+			}
 		}
-		// ---- the thread's stack ----
-		uint32_t mask = 0, hp = 0, okm = 0, rc = 0;
-		uint64_t pk = 0, rcl = 0, tlo = 0, thi = 0;	 // partners; unmatched closes; expected closes (LIFO)
+		__syncthreads();
+		const uint32_t l0  = (uint32_t)tid * kStkItems;	 // chunk-local index of the thread's first bracket
+		const uint32_t cnt = l0 < lim ? min(lim - l0, (uint32_t)kStkItems) : 0u;
+		uint32_t	   w[8];
+#pragma unroll
+		for (int q = 0; q < 8; ++q) w[q] = reinterpret_cast<const uint32_t*>(s_ch + l0)[q];
+		// ---- the thread's stack: the opens' bits first (independent lookups), then bit
+		// operations only; each pair's bytes are checked when the chunk's partners are stored ----
+		uint32_t opn = 0;
+#pragma unroll
+		for (uint32_t q = 0; q < (uint32_t)kStkItems; ++q)
+			opn |= (uint32_t)(s_flags[(w[q >> 2] >> (8 * (q & 3))) & 0xFFu] & 1u) << q;
+		const uint32_t valid = cnt >= 32u ? ~0u : ((1u << cnt) - 1u);
+		opn &= valid;
+		const uint32_t cls	= valid & ~opn;
+		uint32_t	   mask = 0, rcm = 0;  // unmatched opens; unmatched closes
 #pragma unroll
 		for (uint32_t q = 0; q < (uint32_t)kStkItems; ++q) {
-			if (q < cnt) {
-				const uint32_t ch = (w[q >> 2] >> (8 * (q & 3))) & 0xFFu;
-				if (s_flags[ch] & 1u) {
-					mask |= 1u << q;
-					thi = (thi << 8) | (tlo >> 56);
-					tlo = (tlo << 8) | s_cof[ch];
-				} else if (mask) {
+			if ((opn >> q) & 1u) {
+				mask |= 1u << q;
+			} else if ((cls >> q) & 1u) {
+				if (mask) {
 					const uint32_t p = 31u - (uint32_t)__clz(mask);
 					mask ^= 1u << p;
-					const uint32_t e = (uint32_t)(tlo & 0xFFu);
-					tlo				 = (tlo >> 8) | (thi << 56);
-					thi >>= 8;
-					pk |= ((uint64_t)q << (4 * p)) | ((uint64_t)p << (4 * q));
-					hp |= (1u << p) | (1u << q);
-					if (e == ch)
-						okm |= (1u << p) | (1u << q);
+					s_part[l0 + p] = (uint16_t)(l0 + q);
+					s_part[l0 + q] = (uint16_t)(l0 + p);
 				} else {
-					rcl |= (uint64_t)q << (4 * rc);
-					++rc;
+					rcm |= 1u << q;
 				}
 			}
 		}
-		const uint32_t ro = (uint32_t)__popc(mask);
-		uint64_t	   rol = 0;	 // unmatched opens, ascending
-		{
-			uint32_t mm = mask;
-			for (uint32_t j = 0; mm; ++j) {
-				rol |= (uint64_t)(__ffs(mm) - 1) << (4 * j);
-				mm &= mm - 1;
-			}
-		}
+		const uint32_t ro = (uint32_t)__popc(mask), rc = (uint32_t)__popc(rcm);
 		// ---- depths relative to the chunk's entry: e = exclusive prefix of deltas ----
 		const int32_t delta = (int32_t)ro - (int32_t)rc;
 		const int32_t winc	= (int32_t)wave_incl_sum((uint32_t)delta);
 		if (lane == 63)
 			s_wsum[wid] = winc;
 		if (tid == 0)
-			s_entry = nb ? a.bkey[base] + ((s_flags[w[0] & 0xFFu] & 1u) ? 0 : 1) : 0;
+			s_entry = key0 + ((s_flags[w[0] & 0xFFu] & 1u) ? 0 : 1);	// depth before the first bracket
 		__syncthreads();
 		int32_t e = winc - delta, xc = 0;
 #pragma unroll
@@ -167,7 +180,7 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		}
 		const int32_t m = e - (int32_t)rc;
 		s_st[0][tid]	= cnt ? (int16_t)m : kNoMin;
-		s_ro[tid]		= rol;
+		s_ro[tid]		= mask;
 		__syncthreads();
 #pragma unroll
 		for (int j = 1; j < kStkLevels; ++j) {
@@ -177,54 +190,73 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		}
 		const int32_t mc	= range_min(s_st, 0, kStkThreads - 1);	// <= 0: thread 0 enters at 0
 		const int32_t entry = s_entry;
-		// ---- pairs inside the thread ----
-#pragma unroll
-		for (uint32_t q = 0; q < (uint32_t)kStkItems; ++q) {
-			if (q < cnt && ((hp >> q) & 1u)) {
-				const bool ok = (okm >> q) & 1u;
-				bad			  = bad || !ok;
-				write_pair(a, i0 + q, ok ? (int32_t)(i0 + nib(pk, q)) : -1);
-			}
-		}
 		// ---- the thread's unmatched closes: an earlier thread's open, or the chunk's residue ----
-		for (uint32_t j = 0; j < rc; ++j) {
-			const int32_t  k  = e - 1 - (int32_t)j;
-			const uint32_t gc = i0 + nib(rcl, j);
-			const int	   u  = last_at_most(s_st, tid, k);
-			if (u >= 0) {
-				const uint32_t go = base + (uint32_t)u * kStkItems + nib(s_ro[u], (uint32_t)(k - s_st[0][u]));
-				if (go >= nb || go >= gc) {	 // cannot happen: an earlier thread's open
-					lost = true;
-					continue;
+		{
+			uint32_t r = rcm;
+			for (uint32_t j = 0; r; ++j, r &= r - 1) {
+				const int32_t  k  = e - 1 - (int32_t)j;
+				const uint32_t lc = l0 + (uint32_t)(__ffs(r) - 1);
+				const int	   u  = last_at_most(s_st, tid, k);
+				if (u >= 0) {
+					const int32_t jo = k - (int32_t)s_st[0][u];
+					if (jo < 0 || jo >= __popc(s_ro[u])) {	// cannot happen: the open at level k
+						lost = true;
+						continue;
+					}
+					const uint32_t lo = (uint32_t)u * kStkItems + select_bit(s_ro[u], (uint32_t)jo);
+					s_part[lc]		  = (uint16_t)lo;
+					s_part[lo]		  = (uint16_t)lc;
+				} else {
+					const uint32_t idx = (uint32_t)(-1 - k);  // closes: level entry-1, entry-2, ...
+					if (k < 0 && idx < lim)
+						a.res[base + idx] = base + lc;
+					else
+						lost = true;
+					s_part[lc] = kPartLater;
 				}
-				const bool ok = s_cof[a.bchar[go]] == byte_of(w, nib(rcl, j));
-				bad				  = bad || !ok;
-				write_pair(a, gc, ok ? (int32_t)go : -1);
-				write_pair(a, go, ok ? (int32_t)gc : -1);
-			} else {
-				const uint32_t idx = (uint32_t)(-1 - k);  // closes: level entry-1, entry-2, ...
-				if (k < 0 && base + idx < nb)
-					a.res[base + idx] = gc;
-				else
-					lost = true;
 			}
 		}
 		// ---- the thread's unmatched opens no later thread closes: the chunk's residue ----
-		const int32_t suf = tid + 1 < kStkThreads ? range_min(s_st, tid + 1, kStkThreads - 1) : INT_MAX;
-		for (uint32_t j = 0; j < ro; ++j) {
-			const int32_t k = m + (int32_t)j;
-			if (k < suf) {
-				const uint32_t idx = (uint32_t)(-mc) + (uint32_t)(k - mc);	// opens: level min, min+1, ...
-				if (base + idx < nb)
-					a.res[base + idx] = i0 + nib(rol, j);
-				else
-					lost = true;
+		{
+			const int32_t suf = tid + 1 < kStkThreads ? range_min(s_st, tid + 1, kStkThreads - 1) : INT_MAX;
+			uint32_t	  r	  = mask;
+			for (uint32_t j = 0; r; ++j, r &= r - 1) {
+				const int32_t k = m + (int32_t)j;
+				if (k < suf) {
+					const uint32_t lo  = l0 + (uint32_t)(__ffs(r) - 1);
+					const uint32_t idx = (uint32_t)(-mc) + (uint32_t)(k - mc);	// opens: level min, min+1, ...
+					if (idx < lim)
+						a.res[base + idx] = base + lo;
+					else
+						lost = true;
+					s_part[lo] = kPartLater;
+				}
 			}
 		}
 		if (tid == 0) {
 			a.crec[3 * (uint64_t)c + 0] = entry;
 			a.crec[3 * (uint64_t)c + 1] = entry + mc;
 			a.crec[3 * (uint64_t)c + 2] = entry + xc;
+			atomicMax(a.gmin1 + (c >> 6), tag_min(epoch, entry + mc));
+			atomicMax(a.gmin2 + (c >> 12), tag_min(epoch, entry + mc));
+		}
+		__syncthreads();
+		// ---- the chunk's partners, checked and stored coalesced: a pair whose bytes do
+		// not match is -1 on both sides (nest.hpp:80-88); residues get -1 until the
+		// resolve pass ----
+#pragma unroll 4
+		for (uint32_t l = (uint32_t)tid; l < lim; l += kStkThreads) {
+			const uint16_t v  = s_part[l];
+			int32_t		   pv = -1;
+			if (v < (uint16_t)kStkChunk) {
+				const uint32_t cl = s_ch[l], cv = s_ch[v];
+				const bool	   ok = (s_flags[cl] & 1u) ? s_cof[cl] == cv : s_cof[cv] == cl;
+				bad				  = bad || !ok;
+				pv				  = ok ? (int32_t)(base + v) : -1;
+			}
+			a.pair[base + l] = pv;
+			if (a.pair_pos && v != kPartLater)
+				a.pair_pos[a.bsidx[base + l]] = pv >= 0 ? (int32_t)a.bsidx[pv] : -1;
 		}
 	}
 	if (__ballot(bad) && lane == 0)
@@ -233,76 +265,100 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		atomicOr(&a.sum->overflow, kStackLost);
 }
 
-// The chunks' minimum depths as a sparse table: cst[j * max_chunks + c] =
-// min over chunks c - 2^j + 1 .. c.  One block; levels in turn.
-__global__ __launch_bounds__(1024) void k_stack_table(StackArgs a) {
-	if (!stack_applies(a.sum))
-		return;
-	const uint32_t nch = (stack_nb(a) + kStkChunk - 1) / kStkChunk;
-	for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) a.cst[c] = a.crec[3 * (uint64_t)c + 1];
-	__syncthreads();
-	for (uint32_t j = 1; j < a.st_levels && (1u << (j - 1)) < nch; ++j) {
-		const uint32_t h	= 1u << (j - 1);
-		const int32_t* prev = a.cst + (uint64_t)(j - 1) * a.max_chunks;
-		int32_t*	   cur	= a.cst + (uint64_t)j * a.max_chunks;
-		for (uint32_t c = threadIdx.x; c < nch; c += blockDim.x) cur[c] = c >= h ? min(prev[c], prev[c - h]) : prev[c];
-		__syncthreads();
-	}
+__device__ __forceinline__ void write_pair(const StackArgs& a, uint32_t i, int32_t v) {
+	a.pair[i] = v;
+	if (a.pair_pos)
+		a.pair_pos[a.bsidx[i]] = v >= 0 ? (int32_t)a.bsidx[v] : -1;
 }
 
-// Every chunk's residue closes, one wave per chunk, one lane per level: the
-// open at level k of the last earlier chunk whose minimum is <= k.
+// The nearest chunk p <= cur whose minimum is below thr, or -1: one wave,
+// 64 chunks per step, groups of 64 chunks (gmin1) and of 4096 (gmin2) skipped
+// by their minima and entered where one is below thr.
+__device__ int64_t prev_below(const StackArgs& a, uint32_t epoch, int64_t cur, int32_t thr) {
+	const int lane = lane_id();
+	auto	  m0   = [&](int64_t c) -> int32_t { return a.crec[3 * (uint64_t)c + 1]; };
+	auto	  m1   = [&](int64_t g) -> int32_t { return read_min(a.gmin1[g], epoch); };
+	auto	  m2   = [&](int64_t h) -> int32_t { return read_min(a.gmin2[h], epoch); };
+	// the highest index <= hi of a 64-aligned window where f(index) < thr, or -1
+	auto scan64 = [&](int64_t hi, auto f) -> int64_t {
+		const int64_t lo = hi & ~63ll, i = lo + lane;
+		const uint64_t b = __ballot(i <= hi && f(i) < thr);
+		return b ? lo + 63 - __clzll((long long)b) : -1;
+	};
+	while (cur >= 0) {
+		int64_t p = scan64(cur, m0);  // the rest of cur's group of 64
+		if (p >= 0)
+			return p;
+		int64_t g = (cur >> 6) - 1;	// groups before it
+		if (g < 0)
+			return -1;
+		int64_t G = scan64(g, m1);	// the rest of its group of 4096
+		if (G < 0) {
+			int64_t h = (g >> 6) - 1;  // groups of 4096 before it, 64 at a time
+			int64_t H = -1;
+			while (h >= 0 && (H = scan64(h, m2)) < 0) h = (h & ~63ll) - 1;
+			if (H < 0)
+				return -1;
+			G = scan64(H * 64 + 63, m1);
+			if (G < 0)	// a stale minimum (cannot happen): continue below that group
+			{
+				cur = H * 4096 - 1;
+				continue;
+			}
+		}
+		p = scan64(G * 64 + 63, m0);
+		if (p >= 0)
+			return p;
+		cur = G * 64 - 1;  // a stale minimum (cannot happen)
+	}
+	return -1;
+}
+
+// Every chunk's residue closes, one wave per chunk: walk back to the nearest
+// chunk whose minimum is below the highest level still open; its opens close
+// the levels from max(its minimum, ours) up; repeat below it.
 __global__ __launch_bounds__(kResolveBlock) void k_stack_resolve(StackArgs a) {
 	if (!stack_applies(a.sum))
 		return;
 	const uint32_t nb	 = stack_nb(a);
 	const uint32_t nch	 = (nb + kStkChunk - 1) / kStkChunk;
+	const uint32_t epoch = a.sum->epoch;
 	const int	   lane	 = threadIdx.x & 63;
 	const uint32_t waves = gridDim.x * (kResolveBlock / 64);
-	uint32_t	   L	 = 1;  // table levels in use
-	while (L < a.st_levels && (1u << (L - 1)) < nch) ++L;
-	const DevPlan* P   = a.plan;
+	const DevPlan* P	 = a.plan;
 	bool		   bad = false, lost = false;
 	for (uint32_t c = blockIdx.x * (kResolveBlock / 64) + (threadIdx.x >> 6); c < nch; c += waves) {
 		const int32_t E = a.crec[3 * (uint64_t)c], M = a.crec[3 * (uint64_t)c + 1];
-		for (int32_t j = lane; j < E - M; j += 64) {
-			const int32_t k = E - 1 - j;
-			if ((uint64_t)c * kStkChunk + (uint32_t)j >= nb) {	// cannot happen: more residues than brackets
+		int32_t		  rm  = E;	// levels [M, rm) still need their opens
+		int64_t		  cur = (int64_t)c - 1;
+		while (rm > M) {
+			const int64_t p = prev_below(a, epoch, cur, rm);
+			if (p < 0) {  // cannot happen on a balanced document
 				lost = true;
-				continue;
+				break;
 			}
-			const uint32_t gc = a.res[(uint64_t)c * kStkChunk + (uint32_t)j];
-			if (gc >= nb) {
-				lost = true;
-				continue;
+			const int32_t Ep = a.crec[3 * (uint64_t)p], Mp = a.crec[3 * (uint64_t)p + 1],
+						  Xp = a.crec[3 * (uint64_t)p + 2];
+			const int32_t lo = max(Mp, M);
+			for (int32_t k = lo + lane; k < rm; k += 64) {
+				const uint64_t ic = (uint64_t)c * kStkChunk + (uint32_t)(E - 1 - k);
+				const uint64_t io = (uint64_t)p * kStkChunk + (uint32_t)(Ep - Mp) + (uint32_t)(k - Mp);
+				if (k >= Xp || ic >= nb || io >= nb) {	// cannot happen on a balanced document
+					lost = true;
+					continue;
+				}
+				const uint32_t gc = a.res[ic], go = a.res[io];
+				if (gc >= nb || go >= nb) {
+					lost = true;
+					continue;
+				}
+				const bool ok = P->br_close_of[a.bchar[go]] == a.bchar[gc];
+				bad			  = bad || !ok;
+				write_pair(a, gc, ok ? (int32_t)go : -1);
+				write_pair(a, go, ok ? (int32_t)gc : -1);
 			}
-			int64_t		   pos = (int64_t)c - 1;
-			for (int jj = (int)L - 1; jj >= 0; --jj)
-				if (pos - (1ll << jj) + 1 >= 0 && a.cst[(uint64_t)jj * a.max_chunks + (uint64_t)pos] > k)
-					pos -= 1ll << jj;
-			if (pos < 0) {	// cannot happen on a balanced document
-				lost = true;
-				write_pair(a, gc, -1);
-				continue;
-			}
-			const int32_t Eo = a.crec[3 * (uint64_t)pos], Mo = a.crec[3 * (uint64_t)pos + 1],
-						  Xo = a.crec[3 * (uint64_t)pos + 2];
-			const uint64_t ro = (uint64_t)pos * kStkChunk + (uint32_t)(Eo - Mo) + (uint32_t)(k - Mo);
-			if (k < Mo || k >= Xo || ro >= nb) {  // cannot happen on a balanced document
-				lost = true;
-				write_pair(a, gc, -1);
-				continue;
-			}
-			const uint32_t go = a.res[ro];
-			if (go >= nb) {
-				lost = true;
-				write_pair(a, gc, -1);
-				continue;
-			}
-			const bool	   ok = P->br_close_of[a.bchar[go]] == a.bchar[gc];
-			bad				  = bad || !ok;
-			write_pair(a, gc, ok ? (int32_t)go : -1);
-			write_pair(a, go, ok ? (int32_t)gc : -1);
+			rm	= lo;
+			cur = p - 1;
 		}
 	}
 	if (__ballot(bad) && lane == 0)
@@ -313,24 +369,17 @@ __global__ __launch_bounds__(kResolveBlock) void k_stack_resolve(StackArgs a) {
 
 }  // namespace
 
-uint32_t stack_levels(uint64_t max_chunks) {
-	uint32_t L = 1;
-	while ((1ull << (L - 1)) < max_chunks) ++L;
-	return L;
-}
-
 hipError_t launch_stack_pairing(const StackArgs& a, hipStream_t s) {
 	if (a.cap_br == 0)
 		return hipSuccess;
 	int dev = 0, cus = 0;
 	(void)hipGetDevice(&dev);
 	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-	cus						= std::max(cus, 1);
-	const unsigned lgrid	= (unsigned)std::min<uint64_t>(std::max<uint64_t>(a.max_chunks, 1), 2ull * cus);
-	const unsigned rgrid	= (unsigned)std::min<uint64_t>(
-		   std::max<uint64_t>((a.max_chunks + kResolveBlock / 64 - 1) / (kResolveBlock / 64), 1), 8ull * cus);
+	cus					 = std::max(cus, 1);
+	const unsigned lgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(a.max_chunks, 1), 8ull * cus);
+	const unsigned rgrid = (unsigned)std::min<uint64_t>(
+		std::max<uint64_t>((a.max_chunks + kResolveBlock / 64 - 1) / (kResolveBlock / 64), 1), 8ull * cus);
 	hipLaunchKernelGGL(k_stack_local, dim3(lgrid), dim3(kStkThreads), 0, s, a);
-	hipLaunchKernelGGL(k_stack_table, dim3(1), dim3(1024), 0, s, a);
 	hipLaunchKernelGGL(k_stack_resolve, dim3(rgrid), dim3(kResolveBlock), 0, s, a);
 	return hipGetLastError();
 }

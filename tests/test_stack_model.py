@@ -3,8 +3,10 @@ for index: threads of T brackets with a bit-mask stack, chunks of W threads
 whose unmatched closes find the last earlier thread with a small enough
 minimum depth by binary lifting over a sparse table, chunk residues laid out
 as the kernels lay them out (closes from the entry level down, then opens from
-the minimum up) and resolved across chunks through a sparse table of chunk
-minima.  Checked against the oracle's pair_brackets (nest.hpp:45-92) on
+the minimum up) and resolved across chunks by walking back to the nearest
+chunk whose minimum is below the highest open level, skipping groups of G and
+G*G chunks by their minima (G = 64 in the kernels; smaller here, so that
+every level of the walk runs).  Checked against the oracle's pair_brackets (nest.hpp:45-92) on
 balanced documents, mismatched bracket bytes included, with small T and W so
 that every level - thread, chunk, document - carries work; every residue index
 is asserted in range as the kernels' writes require."""
@@ -52,7 +54,7 @@ def range_min(st, lo: int, hi: int) -> int:
     return min(st[j][lo + (1 << j) - 1], st[j][hi])
 
 
-def stack_pairs(chars: np.ndarray, T: int, W: int):
+def stack_pairs(chars: np.ndarray, T: int, W: int, G: int = 64):
     nb = chars.size
     pair = np.full(nb, -2, np.int64)  # -2: never written
     mismatched = False
@@ -121,21 +123,61 @@ def stack_pairs(chars: np.ndarray, T: int, W: int):
                     res[base + idx] = i0 + q
         crec.append((entry, entry + mc, entry + xc))
         del first
-    cst = sparse([r[1] for r in crec], stack_levels(nch)) if crec else []
+    M0 = [r[1] for r in crec]
+    M1 = [min(M0[i:i + G]) for i in range(0, nch, G)]
+    M2 = [min(M0[i:i + G * G]) for i in range(0, nch, G * G)]
+
+    def scan(arr, hi, thr):  # highest index <= hi in hi's G-aligned window with arr < thr
+        lo = hi - hi % G
+        for i in range(hi, lo - 1, -1):
+            if arr[i] < thr:
+                return i
+        return -1
+
+    def prev_below(cur, thr):
+        steps = 0
+        while cur >= 0:
+            steps += 1
+            p = scan(M0, cur, thr)
+            if p >= 0:
+                return p
+            g = cur // G - 1
+            if g < 0:
+                return -1
+            Gi = scan(M1, g, thr)
+            if Gi < 0:
+                h, H = g // G - 1, -1
+                while h >= 0:
+                    H = scan(M2, h, thr)
+                    if H >= 0:
+                        break
+                    h = h - h % G - 1
+                if H < 0:
+                    return -1
+                Gi = scan(M1, H * G + G - 1, thr)
+                assert Gi >= 0
+            p = scan(M0, Gi * G + G - 1, thr)
+            assert p >= 0
+            return p
+        return -1
+
     for c in range(nch):
         E, M, X = crec[c]
-        for j in range(E - M):
-            k = E - 1 - j
-            gc = int(res[c * CH + j])
-            pos = last_at_most(cst, c, k)
-            assert pos >= 0, "a residue close without an open on a balanced document"
-            Eo, Mo, Xo = crec[pos]
-            assert Mo <= k < Xo
-            go = int(res[pos * CH + (Eo - Mo) + (k - Mo)])
-            ok = OPEN[int(chars[go])] == int(chars[gc])
-            mismatched |= not ok
-            pair[gc] = go if ok else -1
-            pair[go] = gc if ok else -1
+        rm, cur = E, c - 1
+        while rm > M:
+            p = prev_below(cur, rm)
+            assert p >= 0, "a residue close without an open on a balanced document"
+            Ep, Mp, Xp = crec[p]
+            lo = max(Mp, M)
+            for k in range(lo, rm):
+                assert k < Xp
+                gc = int(res[c * CH + (E - 1 - k)])
+                go = int(res[p * CH + (Ep - Mp) + (k - Mp)])
+                ok = OPEN[int(chars[go])] == int(chars[gc])
+                mismatched |= not ok
+                pair[gc] = go if ok else -1
+                pair[go] = gc if ok else -1
+            rm, cur = lo, p - 1
     return pair, mismatched
 
 
@@ -160,9 +202,9 @@ def balanced_brackets(rng, n_pairs: int, max_depth: int, p_swap: float) -> np.nd
 depth_before: np.ndarray = np.zeros(0, np.int64)
 
 
-@pytest.mark.parametrize("T,W", [(16, 1024), (4, 8), (2, 4), (16, 2), (1, 1024)])
+@pytest.mark.parametrize("T,W,G", [(32, 64, 64), (32, 256, 64), (4, 8, 2), (2, 4, 3), (16, 2, 2), (1, 256, 4), (32, 4, 64)])
 @pytest.mark.parametrize("seed", range(6))
-def test_stack_model_equals_depth_sort(T, W, seed):
+def test_stack_model_equals_depth_sort(T, W, G, seed):
     global depth_before
     rng = np.random.default_rng(seed)
     for trial in range(4):
@@ -173,7 +215,7 @@ def test_stack_model_equals_depth_sort(T, W, seed):
         depth_before = np.concatenate([[0], np.cumsum(step)[:-1]])
         exp = O.pair(JSON, chars)
         assert exp["balanced"]
-        got, mism = stack_pairs(chars, T, W)
+        got, mism = stack_pairs(chars, T, W, G)
         assert (got != -2).all(), "some bracket was never written"
         assert np.array_equal(got, exp["pair"]), (T, W, seed, trial)
         assert (not mism) == exp["well_formed"]

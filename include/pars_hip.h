@@ -156,6 +156,27 @@ int pars_hip_scan_device(
 	pars_stage_times* st, pars_device_result* out
 );
 
+/* No counterpart in the reference: pars_hip_scan_device for callers that
+ * cannot allocate device memory - code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The caller passes the input and
+ * every output buffer (positions[cap_struct]; optionally struct_bitmap,
+ * ceil(n/64) words; bracket_pos / bracket_char [cap_brackets]; bracket_pair
+ * [cap_brackets] when run_pairs); all scratch (look-back records, summary, the
+ * bracket hand-off, R6's residues, keys and histograms) lives in a static
+ * device array of about 34 MiB that each GPU holds until the process ends, so
+ * the call performs no hipMalloc / hipFree or other allocation.  Limits:
+ * n <= 32 MiB, cap_brackets <= 2^20; a document with more structural or
+ * bracket bytes than the capacities returns PARS_ERR_ARGUMENT.  The scanner
+ * supplies its plan and launch options; chars and pair_pos are not emitted.
+ * Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two streams, two threads) race, so
+ * serialise them.  Blocks until the results are ready. */
+int pars_hip_scan_device_noalloc(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, int run_pairs, uint32_t* positions,
+	size_t cap_struct, uint64_t* struct_bitmap, uint32_t* bracket_pos, uint8_t* bracket_char, int32_t* bracket_pair,
+	size_t cap_brackets, pars_device_result* out
+);
+
 /* pipeline.cuh:1176-1212 scanner::scan_host: pinned staging, H2D, scan,
  * D2H of the structural index (and pair_pos when want_pairs). */
 int pars_hip_scan_host(

@@ -200,6 +200,11 @@ PROTOTYPES = {
         [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_int, C.POINTER(StageTimes),
          C.POINTER(DeviceResult)],
     ),
+    "pars_hip_scan_device_noalloc": (
+        C.c_int,
+        [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+         C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(DeviceResult)],
+    ),
     "pars_hip_scan_host": (
         C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.c_void_p, C.c_int, C.POINTER(HostResult)]
     ),

@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 #include <new>
 #include <string>
@@ -629,6 +630,121 @@ bool batch_failed(const void* h_summary) {
 
 }  // namespace pars_hip
 
+
+// ---------------------------------------------------------------------------
+// Scanning without allocating (pars_hip_scan_device_noalloc).  Every scratch
+// buffer of the single-document chain - the tile look-back records, the
+// ticket, the summary, the bracket hand-off R6 reads (structural index, key),
+// the stack path's residues, chunk depths and group minima, and the sorting
+// path's keys, order, ranks and histograms - lives in one file-scope device
+// array, `scratch`, sized for documents of up to kNaMaxBytes bytes and
+// kNaMaxBrackets brackets.  Each GPU holds its own copy until the process
+// ends; its address is looked up once per device.  Nothing needs clearing
+// between calls: look-back records and group minima are tagged with the
+// summary's epoch (the array starts at zero, epoch 0 is never a launch's), the
+// summary is reset by a kernel, the scan kernel leaves its ticket at zero.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kNaMaxBytes	  = 32ull << 20;
+constexpr uint64_t kNaMaxBrackets = 1ull << 20;
+constexpr uint64_t na_up(uint64_t x) {
+	return (x + 255) / 256 * 256;
+}
+constexpr uint64_t kNaTiles		 = (kNaMaxBytes + kTile - 1) / kTile;  // one-row tiles need the most records
+constexpr uint64_t kNaStkChunks	 = (kNaMaxBrackets + kStkChunk - 1) / kStkChunk;
+constexpr uint64_t kNaSortChunks = kNaMaxBrackets / 1024;  // at least the sort's chunks (4096 brackets each)
+// byte offsets into the array
+constexpr uint64_t kNaRecs	 = 0;
+constexpr uint64_t kNaTicket = kNaRecs + na_up(kNaTiles * kRecWords * 8);
+constexpr uint64_t kNaSum	 = kNaTicket + 256;
+constexpr uint64_t kNaBr	 = kNaSum + na_up(sizeof(DevSummary));	// 8 arrays of 4 B per bracket
+constexpr uint64_t kNaBrArr	 = na_up(kNaMaxBrackets * 4);
+enum : int { kNaBsidx, kNaBkey, kNaUk0, kNaUk1, kNaOrd0, kNaOrd1, kNaRank, kNaRes, kNaBrCount };
+constexpr uint64_t kNaHist	   = kNaBr + kNaBrCount * kNaBrArr;
+constexpr uint64_t kNaHistPart = kNaHist + na_up(256 * 4 * kNaSortChunks);
+constexpr uint64_t kNaCrec	   = kNaHistPart + na_up(256 * 4);
+constexpr uint64_t kNaG1	   = kNaCrec + na_up(kNaStkChunks * 12);
+constexpr uint64_t kNaG2	   = kNaG1 + na_up((kNaStkChunks / 64 + 1) * 8);
+constexpr uint64_t kNaBytes	   = kNaG2 + na_up((kNaStkChunks / 4096 + 1) * 8);
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex	  g_na_mu;
+uint8_t*	  g_na_ptr[kNaMaxDevices] = {};
+
+// The calling thread's device's copy of `scratch`: its address, looked up the
+// first time a call runs on that device (with it current).  The array's name
+// in host code is not a device address; only this pointer is passed on.
+int na_scratch(uint8_t** out) {
+	int		   dev = 0;
+	hipError_t e   = hipGetDevice(&dev);
+	if (e != hipSuccess)
+		return hip_fail(e, "hipGetDevice");
+	if (dev < 0 || dev >= kNaMaxDevices)
+		return fail(PARS_ERR_DEVICE, "device %d is beyond the %d this entry point serves", dev, kNaMaxDevices);
+	std::lock_guard<std::mutex> lk(g_na_mu);
+	if (!g_na_ptr[dev]) {
+		void* p = nullptr;
+		if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess)
+			return hip_fail(e, "hipGetSymbolAddress(scratch)");
+		if (!p)
+			return fail(PARS_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned no address");
+		g_na_ptr[dev] = static_cast<uint8_t*>(p);
+	}
+	*out = g_na_ptr[dev];
+	return PARS_OK;
+}
+
+template<class T>
+T* na_at(uint8_t* base, uint64_t off) {
+	return reinterpret_cast<T*>(base + off);
+}
+
+PairArgs na_pair_args(pars_scanner* sc, uint8_t* S, uint64_t cap_br, const uint8_t* bchar, int32_t* pair) {
+	PairArgs p {};
+	p.sum			 = na_at<DevSummary>(S, kNaSum);
+	p.plan			 = sc->dplan.as<DevPlan>();
+	p.bkey			 = na_at<int32_t>(S, kNaBr + kNaBkey * kNaBrArr);
+	p.bchar			 = bchar;
+	p.bsidx			 = na_at<uint32_t>(S, kNaBr + kNaBsidx * kNaBrArr);
+	p.cap_br		 = cap_br;
+	p.ukeys[0]		 = na_at<uint32_t>(S, kNaBr + kNaUk0 * kNaBrArr);
+	p.ukeys[1]		 = na_at<uint32_t>(S, kNaBr + kNaUk1 * kNaBrArr);
+	p.order[0]		 = na_at<uint32_t>(S, kNaBr + kNaOrd0 * kNaBrArr);
+	p.order[1]		 = na_at<uint32_t>(S, kNaBr + kNaOrd1 * kNaBrArr);
+	p.rank			 = na_at<uint32_t>(S, kNaBr + kNaRank * kNaBrArr);
+	p.hist			 = na_at<uint32_t>(S, kNaHist);
+	p.hist_part		 = na_at<uint32_t>(S, kNaHistPart);
+	p.max_chunks	 = pairing_max_chunks(cap_br);
+	p.pair			 = pair;
+	p.max_digit_bits = (uint32_t)sc->radix_bits;
+	return p;
+}
+
+int na_read_summary(pars_scanner* sc, DevSummary* d_sum, hipStream_t s) {
+	hipError_t e = hipMemcpyAsync(sc->h_sum, d_sum, sizeof(DevSummary), hipMemcpyDeviceToHost, s);
+	if (e != hipSuccess)
+		return hip_fail(e, "summary readback");
+	if ((e = hipStreamSynchronize(s)) != hipSuccess)
+		return hip_fail(e, "hipStreamSynchronize");
+	const uint32_t o = sc->h_sum->overflow;
+	if (o & 0x80000000u)
+		return fail(PARS_ERR_DEVICE, "scan look-back exceeded its spin bound (internal error)");
+	if (o & 8u)
+		return fail(PARS_ERR_ARGUMENT, "the radix digit cap needs more than four pairing passes for this key range");
+	if (o & kStackLost)
+		return fail(PARS_ERR_DEVICE, "stack pairing found an unmatched residue on a balanced document (internal error)");
+	return PARS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* pars_hip_version(void) {
@@ -893,6 +1009,117 @@ int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const p
 	out->doc_flags	   = sc->doc_flags.as<uint8_t>();
 	out->max_depth	   = sc->doc_max.as<int32_t>();
 	out->struct_bitmap = sc->want_bitmap ? sc->bitmap.as<uint64_t>() : nullptr;
+	return PARS_OK;
+}
+
+
+int pars_hip_scan_device_noalloc(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, int run_pairs,
+								 uint32_t* positions, size_t cap_struct, uint64_t* struct_bitmap,
+								 uint32_t* bracket_pos, uint8_t* bracket_char, int32_t* bracket_pair,
+								 size_t cap_brackets, pars_device_result* out) {
+	if (!sc || !out || (n && !d_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if ((uint64_t)n > kNaMaxBytes)
+		return fail(PARS_ERR_ARGUMENT, "pars_hip_scan_device_noalloc takes documents of at most %llu bytes (%zu given)",
+					(unsigned long long)kNaMaxBytes, n);
+	if ((uint64_t)cap_brackets > kNaMaxBrackets)
+		return fail(PARS_ERR_ARGUMENT, "pars_hip_scan_device_noalloc takes at most %llu brackets (cap_brackets %zu)",
+					(unsigned long long)kNaMaxBrackets, cap_brackets);
+	if (n && (!positions || !bracket_pos || !bracket_char || (run_pairs && !bracket_pair)))
+		return fail(PARS_ERR_ARGUMENT, "positions, bracket_pos, bracket_char (and bracket_pair with run_pairs) are required");
+	std::memset(out, 0, sizeof *out);
+	if (n == 0) {
+		out->balanced = out->well_formed = 1;
+		return PARS_OK;
+	}
+	uint8_t* S = nullptr;
+	int		 r;
+	if ((r = na_scratch(&S)))
+		return r;
+	hipStream_t s	  = (hipStream_t)stream;
+	DevSummary* d_sum = na_at<DevSummary>(S, kNaSum);
+	ScanArgs	a {};
+	a.data			= d_in;
+	a.n				= n;
+	a.n_tiles		= (uint32_t)((n + kTile - 1) / kTile);
+	a.plan			= sc->dplan.as<DevPlan>();
+	a.recs			= na_at<uint64_t>(S, kNaRecs);
+	a.ticket		= na_at<uint32_t>(S, kNaTicket);
+	a.done			= a.ticket + 1;
+	a.sum			= d_sum;
+	a.positions		= positions;
+	a.bitmap		= struct_bitmap;
+	a.bpos			= bracket_pos;
+	a.bsidx			= na_at<uint32_t>(S, kNaBr + kNaBsidx * kNaBrArr);
+	a.bkey			= na_at<int32_t>(S, kNaBr + kNaBkey * kNaBrArr);
+	a.bchar			= bracket_char;
+	a.cap_struct	= cap_struct;
+	a.cap_br		= cap_brackets;
+	a.in_states		= sc->hplan.start_packed;
+	a.lb_window		= (uint32_t)sc->lb_window;
+	a.blocks_per_cu = (uint32_t)sc->blocks_per_cu;
+	a.chase			= (uint32_t)sc->chase;
+	hipError_t e;
+	if ((e = launch_summary_reset(d_sum, SummaryInit {}, s)) != hipSuccess)
+		return hip_fail(e, "summary reset");
+	if ((e = launch_scan(sc, a, s)) != hipSuccess)
+		return hip_fail(e, "scan kernel launch");
+	bool stack = false;
+	if (run_pairs && cap_brackets) {
+		const uint32_t small = small_pairing_mode(n, 1);
+		if (!sc->radix_bits && small != 2) {
+			StackArgs k {};
+			k.sum		 = d_sum;
+			k.plan		 = sc->dplan.as<DevPlan>();
+			k.bchar		 = bracket_char;
+			k.bkey		 = a.bkey;
+			k.bsidx		 = a.bsidx;
+			k.cap_br	 = cap_brackets;
+			k.pair		 = bracket_pair;
+			k.res		 = na_at<uint32_t>(S, kNaBr + kNaRes * kNaBrArr);
+			k.crec		 = na_at<int32_t>(S, kNaCrec);
+			k.gmin1		 = na_at<unsigned long long>(S, kNaG1);
+			k.gmin2		 = na_at<unsigned long long>(S, kNaG2);
+			k.max_chunks = std::max<uint64_t>((cap_brackets + kStkChunk - 1) / kStkChunk, 1);
+			e			 = launch_stack_pairing(k, s);
+			stack		 = true;
+		} else {
+			PairArgs p = na_pair_args(sc, S, cap_brackets, bracket_char, bracket_pair);
+			p.small	   = sc->radix_bits ? 0u : small;
+			e		   = launch_pairing(p, s);
+		}
+		if (e != hipSuccess)
+			return hip_fail(e, "pairing launch");
+	}
+	if ((r = na_read_summary(sc, d_sum, s)))
+		return r;
+	const DevSummary& h = *sc->h_sum;
+	if (h.n_struct > cap_struct || h.n_brackets > cap_brackets)
+		return fail(PARS_ERR_ARGUMENT,
+					"outputs too small: the document has %u structural and %u bracket bytes (capacities %zu, %zu)",
+					h.n_struct, h.n_brackets, cap_struct, cap_brackets);
+	if (stack && !(h.min_bal >= 0 && h.last_bal == 0)) {  // unbalanced: the sort's pairs
+		PairArgs p = na_pair_args(sc, S, cap_brackets, bracket_char, bracket_pair);
+		if ((e = launch_pairing(p, s)) != hipSuccess)
+			return hip_fail(e, "pairing launch");
+		if ((r = na_read_summary(sc, d_sum, s)))
+			return r;
+	}
+	out->structural	   = positions;
+	out->pair_pos	   = nullptr;
+	out->n_struct	   = h.n_struct;
+	out->balanced	   = (h.min_bal >= 0) && (h.last_bal == 0);
+	out->well_formed   = out->balanced && (h.n_brackets % 2 == 0) && !(run_pairs && h.mismatched);
+	out->max_depth	   = h.max_bal;
+	out->chars		   = nullptr;
+	out->struct_bitmap = struct_bitmap;
+	out->bracket_pos   = bracket_pos;
+	out->bracket_char  = bracket_char;
+	out->bracket_pair  = run_pairs ? bracket_pair : nullptr;
+	out->n_brackets	   = h.n_brackets;
+	const uint32_t nd  = sc->hplan.n_descs;
+	const uint32_t m   = nd ? ((1u << (3 * nd)) - 1u) : 0u;
+	out->unclosed_mode = (h.final_states & m) != (sc->hplan.start_packed & m);
 	return PARS_OK;
 }
 

@@ -119,3 +119,17 @@ def test_server_errors_reach_last_error():
     assert b"at least one plan" in lib.pars_hip_last_error()
     assert lib.pars_hip_server_submit(None, 0, None, 0, C.byref(C.c_uint64())) == 1
     assert b"submit" in lib.pars_hip_last_error()
+
+
+def test_no_alloc_limits_refused_before_the_device():
+    """pars_hip_scan_device_noalloc refuses sizes beyond its scratch with a
+    clear error, before any device call (so this runs without a GPU)."""
+    lib = N.lib()
+    r = N.DeviceResult()
+    dummy = C.c_void_p(1)
+    rc = lib.pars_hip_scan_device_noalloc(dummy, dummy, (32 << 20) + 1, None, 1, dummy, 1, None, dummy, dummy,
+                                          dummy, 1, C.byref(r))
+    assert rc == 1 and b"at most" in lib.pars_hip_last_error()
+    rc = lib.pars_hip_scan_device_noalloc(dummy, dummy, 1024, None, 1, dummy, 1024, None, dummy, dummy, dummy,
+                                          (1 << 20) + 1, C.byref(r))
+    assert rc == 1 and b"brackets" in lib.pars_hip_last_error()

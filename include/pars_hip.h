@@ -249,6 +249,8 @@ typedef struct {
 	uint64_t		 max_batch_bytes; /* per launch (default 64 MiB; also the largest request) */
 	uint32_t		 max_batch_docs;  /* per launch (default 16384) */
 	uint32_t		 run_pairs;		  /* run R6 (default 1) */
+	uint32_t		 submit_threads;  /* threads pars_hip_server_replay submits from (default 8) */
+	uint32_t		 reserved;
 } pars_server_config;
 
 typedef struct {
@@ -287,7 +289,8 @@ typedef struct {
 int pars_hip_server_create(const pars_server_config* cfg, pars_server** out);
 int pars_hip_server_destroy(pars_server* sv);
 /* Copies n bytes into the pinned staging ring; PARS_ERR_BUSY when it is full
- * (retry after polling and releasing). */
+ * (retry after polling and releasing).  Thread-safe: concurrent submitters
+ * copy in parallel (only the ring reservation is serialised). */
 int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, size_t n, uint64_t* id);
 /* Up to cap completed requests not returned before; waits up to wait_us for
  * the first one. */
@@ -298,9 +301,11 @@ int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, 
 int pars_hip_server_release(pars_server* sv, uint64_t id);
 int pars_hip_server_stats(pars_server* sv, pars_server_stats* out);
 /* Load driver: submits n requests (host bytes data[i], sizes[i], plan[i]) at
- * arrival_ns[i] after the start (all 0: as fast as admission allows), polls
- * and releases as they complete; latency_ns[i] = completion - scheduled
- * arrival.  *elapsed_ns = first arrival to last completion. */
+ * arrival_ns[i] after the start (all 0: as fast as admission allows) from
+ * submit_threads threads (request i from thread i mod submit_threads, each in
+ * arrival order), while the calling thread polls and releases as they
+ * complete; latency_ns[i] = completion - scheduled arrival.  *elapsed_ns =
+ * first arrival to last completion. */
 int pars_hip_server_replay(
 	pars_server* sv, size_t n, const uint32_t* plan, const uint8_t* const* data, const uint64_t* sizes,
 	const uint64_t* arrival_ns, uint64_t* latency_ns, uint64_t* elapsed_ns

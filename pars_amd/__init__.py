@@ -343,14 +343,15 @@ class Server:
 
     def __init__(self, plans, slots_per_plan: int = 2, ring_bytes: int = 4 << 30, staging_bytes: int = 1 << 30,
                  hbm_budget: int = 64 << 30, max_batch_bytes: int = 64 << 20, max_batch_docs: int = 16384,
-                 run_pairs: bool = True):
+                 run_pairs: bool = True, submit_threads: int = 8):
         import numpy as np
 
         self.plans = [Plan.load(p) if isinstance(p, str) else p for p in plans]
         blob = b"".join(p.blob for p in self.plans)
         self._plan_buf = np.frombuffer(blob, dtype=np.uint8).copy()
         cfg = N.ServerConfig(self._plan_buf.ctypes.data, len(self.plans), slots_per_plan, ring_bytes,
-                             staging_bytes, hbm_budget, max_batch_bytes, max_batch_docs, int(run_pairs))
+                             staging_bytes, hbm_budget, max_batch_bytes, max_batch_docs, int(run_pairs),
+                             submit_threads, 0)
         self._lib = N.lib()
         h = C.c_void_p()
         _check(self._lib.pars_hip_server_create(C.byref(cfg), C.byref(h)))

@@ -130,6 +130,8 @@ class ServerConfig(C.Structure):
         ("max_batch_bytes", C.c_uint64),
         ("max_batch_docs", C.c_uint32),
         ("run_pairs", C.c_uint32),
+        ("submit_threads", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

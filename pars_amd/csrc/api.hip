@@ -108,6 +108,7 @@ struct pars_scanner {
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
 	DevBuf	  stk_res, stk_rec, stk_g1, stk_g2;	 // stack pairing: chunk residues, chunk depths, group minima
 	bool	  stack_pending = false;	 // the last enqueued R6 was the stack path (sort path if unbalanced)
+	bool	  keys_sparse	= false;	 // the document's keys are only at stack-chunk starts (a sort fills them)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
 	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0;  // tuning; 0 = default
@@ -255,7 +256,8 @@ ScanArgs make_scan_args(pars_scanner* sc, const uint8_t* d_in, uint64_t n, bool 
 	a.bitmap	 = sc->want_bitmap ? sc->bitmap.as<uint64_t>() : nullptr;
 	a.pair_pos	 = want_pairs ? sc->pair_pos.as<int32_t>() : nullptr;
 	a.bpos		 = sc->bpos.as<uint32_t>();
-	a.bsidx		 = sc->bsidx.as<uint32_t>();
+	// structural indices of brackets: read only for pair_pos and the tape's partners
+	a.bsidx		 = (want_pairs || sc->force_radix) ? sc->bsidx.as<uint32_t>() : nullptr;
 	a.bkey		 = sc->bkey.as<int32_t>();
 	a.bchar		 = sc->bchar.as<uint8_t>();
 	a.cap_struct = sc->cap_struct;
@@ -308,19 +310,26 @@ StackArgs make_stack_args(pars_scanner* sc, bool want_pairs) {
 	return a;
 }
 
+// Whether R6 of a document of n bytes starts on the stack path (enqueue_pairs);
+// its scan then writes keys only at the stack chunks' starts.
+bool stack_first(const pars_scanner* sc, uint64_t n) {
+	return !sc->radix_bits && !sc->force_radix && small_pairing_mode(n, 1) != 2;
+}
+
 // R6 of one document (or of the chunks of one): the stack path unless a
 // digit cap, a consumer of the sorted order or the one-block path asks for the
 // sort.  The stack path's kernels exit on an unbalanced document, which then
 // takes the sort once the summary shows it (finish_pairs).
 hipError_t enqueue_pairs(pars_scanner* sc, uint64_t n, bool want_pairs, hipStream_t s) {
 	const uint32_t small = small_pairing_mode(n, 1);
-	if (!sc->radix_bits && !sc->force_radix && small != 2) {
+	if (stack_first(sc, n)) {
 		sc->stack_pending = true;
 		return launch_stack_pairing(make_stack_args(sc, want_pairs), s);
 	}
 	sc->stack_pending = false;
 	PairArgs p		  = make_pair_args(sc, want_pairs);
 	p.small			  = (sc->radix_bits || sc->force_radix) ? 0u : small;
+	p.key_fill		  = sc->keys_sparse ? sc->bkey.as<int32_t>() : nullptr;
 	return launch_pairing(p, s);
 }
 
@@ -342,6 +351,7 @@ int finish_pairs(pars_scanner* sc, bool want_pairs, hipStream_t s, double* ms) {
 		return hip_fail(e, "hipEventRecord");
 	PairArgs p = make_pair_args(sc, want_pairs);
 	p.small	   = 0;
+	p.key_fill = sc->keys_sparse ? sc->bkey.as<int32_t>() : nullptr;
 	if ((e = launch_pairing(p, s)) != hipSuccess)
 		return hip_fail(e, "pairing launch");
 	if (ms && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
@@ -408,7 +418,9 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 		return hip_fail(e, "summary reset");
 	if (timed && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
-	ScanArgs a = make_scan_args(sc, d_in, n, want_pairs);
+	ScanArgs a		= make_scan_args(sc, d_in, n, want_pairs);
+	a.key_sparse	= !run_pairs || stack_first(sc, n);
+	sc->keys_sparse = a.key_sparse;
 	if ((e = launch_scan(sc, a, s)) != hipSuccess)
 		return hip_fail(e, "scan kernel launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
@@ -1151,6 +1163,11 @@ int pars_hip_scan_chunk(pars_scanner* sc, const uint8_t* d_chunk, size_t n, cons
 		return r;
 	ScanArgs a	 = make_scan_args(sc, d_chunk, n, false);
 	a.bitmap	 = nullptr;	 // the per-document bitmap is not chunk-addressed
+	a.bsidx		 = sc->bsidx.as<uint32_t>();  // pair_chunks may be asked for pair_pos
+	// R6 over the chunks starts on the stack path unless a digit cap or the tape
+	// asks for the sort; a sort after sparse keys fills them first
+	a.key_sparse	= !sc->radix_bits && !sc->force_radix;
+	sc->keys_sparse = (c.bracket_base != 0 && sc->keys_sparse) || a.key_sparse;
 	a.byte_base	 = byte_base;
 	a.out_base	 = c.out_base;
 	a.br_base	 = c.bracket_base;
@@ -1423,8 +1440,10 @@ int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t
 				return hip_fail(e, "chunk upload");
 			if (k > 0 && (e = launch_carry_next(sc->summary.as<DevSummary>(), sc->carry.as<DevCarry>(), s)) != hipSuccess)
 				return hip_fail(e, "carry");
-			ScanArgs a	= make_scan_args(sc, sc->d_chunk[slot].as<uint8_t>(), len, want_pairs != 0);
-			a.byte_base = off;
+			ScanArgs a		= make_scan_args(sc, sc->d_chunk[slot].as<uint8_t>(), len, want_pairs != 0);
+			a.key_sparse	= stack_first(sc, n);
+			sc->keys_sparse = a.key_sparse;
+			a.byte_base		= off;
 			a.bitmap	= sc->want_bitmap ? sc->bitmap.as<uint64_t>() + off / 64 : nullptr;
 			a.carry		= k > 0 ? sc->carry.as<DevCarry>() : nullptr;
 			if ((e = launch_scan(sc, a, s)) != hipSuccess)

@@ -83,6 +83,10 @@ struct DevCarry {
 	int32_t	 depth;		// depth before the chunk
 };
 
+// R6's stack path pairs brackets in chunks of this many (pair_stack.hip); the
+// scan kernel can leave keys only at these chunks' first brackets.
+constexpr uint32_t kStkChunk = 8192;  // brackets per chunk (256 threads x 32)
+
 struct ScanArgs {
 	const uint8_t*	data;
 	uint64_t		n;
@@ -98,9 +102,13 @@ struct ScanArgs {
 	uint64_t*		bitmap;	   // optional, ceil(n/64) words
 	int32_t*		pair_pos;  // optional: -1 written for every structural index
 	uint32_t*		bpos;
-	uint32_t*		bsidx;
+	uint32_t*		bsidx;	   // optional: only pair_pos and the tape's partners read it
 	int32_t*		bkey;
 	uint8_t*		bchar;
+	// one document: keys only for bracket indices that are multiples of
+	// kStkChunk (what the stack path reads); a sort of the document first fills
+	// the rest (PairArgs::key_fill).  Batches always write every key.
+	uint32_t		key_sparse;
 	uint64_t		cap_struct;
 	uint64_t		cap_br;
 	uint64_t		byte_base;	 // added to every emitted position (chunked mode)
@@ -210,6 +218,10 @@ struct PairArgs {
 	// first (the radix path then runs only if it did not apply), 2 = the only
 	// path (the caller guarantees it applies: see small_pairing_mode)
 	uint32_t		small;
+	// one document scanned with sparse keys: the key of every bracket is
+	// rebuilt here first (the same buffer as bkey) from the bracket bytes and
+	// the keys at the chunk starts
+	int32_t*		key_fill;
 };
 
 constexpr uint32_t kSmallBrackets = 8192;
@@ -228,7 +240,6 @@ hipError_t launch_pairing(const PairArgs& a, hipStream_t s);
 // R6 for balanced documents by stack matching (pair_stack.hip): the same pairs
 // the sort gives when the depth never drops below 0 and ends at 0; the kernels
 // exit at once otherwise and the sorting path runs instead.
-constexpr uint32_t kStkChunk  = 8192;		  // brackets per chunk (256 threads x 32)
 constexpr uint32_t kStackLost = 0x10u;		  // summary overflow bit: a residue close found no open (internal)
 struct StackArgs {
 	DevSummary*		sum;

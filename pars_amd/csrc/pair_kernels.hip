@@ -8,7 +8,9 @@
 // which diverges from the CPU on malformed input; here the full key range is
 // sorted stably, so the two agree on every input.
 //
-// Layout: the scan kernel leaves bkey/bchar/bsidx per bracket.  The radix sort
+// Layout: the scan kernel leaves bkey/bchar/bsidx per bracket (keys of a single
+// document only at chunk starts when it was scanned for the stack path: the
+// key pass below rebuilds the rest before a sort).  The radix sort
 // runs LSD passes over u = key - kmin with a digit width fitted to the key
 // range found on device (one 6-bit pass for every document up to depth 63,
 // at most 8 bits per pass - or a tuned narrower cap - and 1..4 passes): per-chunk digit histograms (LDS
@@ -66,6 +68,67 @@ __global__ void k_sort_prep(PairArgs a) {
 	s->n_chunks			 = (nb + kChunk - 1) / kChunk;
 	if ((uint64_t)s->n_chunks > a.max_chunks)
 		s->overflow |= 2u;
+}
+
+// ---- keys of a document scanned with sparse keys ----------------------------
+// The scan kernel left the key of every kStkChunk-th bracket only.  One block
+// per chunk rebuilds the others from the bracket bytes: the depth before the
+// chunk's first bracket follows from its key (an open's key is the depth before
+// it, a close's the depth after it), a block scan of the +1 / -1 steps gives
+// each thread's entry depth, and each thread walks its 32 brackets.
+__global__ __launch_bounds__(256) void k_key_fill(PairArgs a) {
+	__shared__ uint8_t s_open[256];
+	__shared__ int32_t s_w[4];
+	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	s_open[tid]	  = a.plan->br_flags[tid] & 1u;
+	__syncthreads();
+	const uint32_t nb  = n_brackets(a);
+	const uint32_t nch = (nb + kStkChunk - 1) / kStkChunk;
+	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+		const uint32_t base = c * kStkChunk, lim = min(nb - base, kStkChunk);
+		const uint32_t l0 = (uint32_t)tid * 32u, cnt = l0 < lim ? min(lim - l0, 32u) : 0u;
+		const uint8_t* ch = a.bchar + base + l0;
+		uint32_t	   om = 0;	// bit q: bracket q of the thread opens
+		if (cnt == 32u) {
+			uint4 v[2];
+			v[0] = reinterpret_cast<const uint4*>(ch)[0];
+			v[1] = reinterpret_cast<const uint4*>(ch)[1];
+			const uint32_t* w = reinterpret_cast<const uint32_t*>(v);
+#pragma unroll
+			for (int q = 0; q < 32; ++q) om |= (uint32_t)s_open[(w[q >> 2] >> (8 * (q & 3))) & 0xFFu] << q;
+		} else {
+			for (uint32_t q = 0; q < cnt; ++q) om |= (uint32_t)s_open[ch[q]] << q;
+		}
+		const int32_t  delta = 2 * __popc(om) - (int32_t)cnt;
+		const uint32_t inc	 = wave_incl_sum((uint32_t)delta);
+		if (lane == 63)
+			s_w[wid] = (int32_t)inc;
+		const int32_t key0 = a.key_fill[base];
+		const int32_t d0   = s_open[a.bchar[base]] ? key0 : key0 + 1;	// depth before the chunk
+		__syncthreads();
+		int32_t d = d0 + (int32_t)inc - delta;
+		for (int w2 = 0; w2 < wid; ++w2) d += s_w[w2];
+		int32_t* out = a.key_fill + base + l0;
+		if (cnt == 32u) {
+			int32_t k[32];
+#pragma unroll
+			for (int q = 0; q < 32; ++q) {
+				const bool op = (om >> q) & 1u;
+				k[q]		  = op ? d : d - 1;
+				d += op ? 1 : -1;
+			}
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				reinterpret_cast<int4*>(out)[q] = make_int4(k[4 * q], k[4 * q + 1], k[4 * q + 2], k[4 * q + 3]);
+		} else {
+			for (uint32_t q = 0; q < cnt; ++q) {
+				const bool op = (om >> q) & 1u;
+				out[q]		  = op ? d : d - 1;
+				d += op ? 1 : -1;
+			}
+		}
+		__syncthreads();  // s_w is reused by the next chunk
+	}
 }
 
 // ---- batch key bases -------------------------------------------------------
@@ -504,6 +567,10 @@ hipError_t launch_pass(const PairArgs& a, unsigned grid, hipStream_t s) {
 hipError_t launch_pairing(const PairArgs& a, hipStream_t s) {
 	if (a.cap_br == 0)
 		return hipSuccess;
+	if (a.key_fill) {
+		const uint64_t fc = (a.cap_br + kStkChunk - 1) / kStkChunk;
+		hipLaunchKernelGGL(k_key_fill, dim3((unsigned)std::min<uint64_t>(fc, 2048)), dim3(256), 0, s, a);
+	}
 	if (a.bdoc) {  // per-document key bases first
 		if (a.n_docs <= kKbaseOneBlock) {
 			hipLaunchKernelGGL(k_kbase_one, dim3(1), dim3(1024), 0, s, a);

@@ -589,7 +589,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		a.depth_base = a.carry->depth;
 	}
 	E::load(s_lds, a.plan);
-	const bool aligned = ((uintptr_t)a.data & 15u) == 0;
+	const bool aligned	 = ((uintptr_t)a.data & 15u) == 0;
+	const bool want_sidx = a.bsidx != nullptr;
+	const bool all_keys	 = BATCH || !a.key_sparse;
 	// bracket bytes from the walk's ordinal bits instead of the input
 	const bool	   bit_chr = E::kBitParallel && a.plan->br_bit_ok;
 	const uint32_t brc	   = (uint32_t)a.plan->br_chr[0][0] | ((uint32_t)a.plan->br_chr[0][1] << 8)
@@ -1084,9 +1086,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					bal += op ? 1 : -1;
 					rmin		= min(rmin, bal);
 					rmax		= max(rmax, bal);
-					s_bpos[lb]	= (uint16_t)(loff + (uint32_t)p);
-					s_bsidx[lb] = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
-					s_bkey[lb]	= (KeyT)((op ? bal0 : bal) - (BATCH ? 0 : rowd));
+					s_bpos[lb] = (uint16_t)(loff + (uint32_t)p);
+					if (want_sidx)
+						s_bsidx[lb] = (uint16_t)(cpre[r][0] + popc_below(sm[r], p));
+					s_bkey[lb] = (KeyT)((op ? bal0 : bal) - (BATCH ? 0 : rowd));
 					if constexpr (E::kBitParallel)
 						s_bch[lb] = (uint8_t)(brc >> (8 * ((op ? 2 : 0) + (int)((bo >> p) & 1ull))));
 					++lb;
@@ -1140,10 +1143,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 						const uint32_t i = i0 + u * kBlock + tid;
 						if (i < mb) {
 							const uint32_t off = s_bpos[i];
-							a.bpos[rb[1] + i]  = out_pos(off);
-							a.bsidx[rb[1] + i] = rb[0] + s_bsidx[i];
-							a.bkey[rb[1] + i]  = kb + (int32_t)s_bkey[i];
-							a.bchar[rb[1] + i] = ch[u];
+							const uint32_t g   = rb[1] + i;
+							a.bpos[g]		   = out_pos(off);
+							if (want_sidx)
+								a.bsidx[g] = rb[0] + s_bsidx[i];
+							if (all_keys || (g & (kStkChunk - 1u)) == 0)
+								a.bkey[g] = kb + (int32_t)s_bkey[i];
+							a.bchar[g] = ch[u];
 							if constexpr (BATCH)
 								a.b.bdoc[rb[1] + i] = (uint32_t)d0 + slot_of(r * kBlock + (int)(off >> 6));
 						}
@@ -1173,9 +1179,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 						if ((uint64_t)rb[1] + lb < a.cap_br) {
 							const uint32_t g = rb[1] + lb;
 							a.bpos[g]		 = pos;
-							a.bsidx[g]		 = rb[0] + li;
-							a.bkey[g]		 = (BATCH ? 0 : (int32_t)cex[2]) + (op ? bal - 1 : bal);
-							a.bchar[g]		 = ch;
+							if (want_sidx)
+								a.bsidx[g] = rb[0] + li;
+							if (all_keys || (g & (kStkChunk - 1u)) == 0)
+								a.bkey[g] = (BATCH ? 0 : (int32_t)cex[2]) + (op ? bal - 1 : bal);
+							a.bchar[g] = ch;
 							if constexpr (BATCH)
 								a.b.bdoc[g] = (uint32_t)ldoc[r];
 						}

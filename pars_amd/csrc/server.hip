@@ -3,14 +3,17 @@
 // scanned by the batched kernel with continuous batching.
 //
 // Per plan: a pinned staging ring, a region of the device ring, a copy stream,
-// a compute stream and `slots` scanners.  A worker thread loops:
+// a compute stream and `slots` scanners.  A submit reserves its staging space
+// under the server lock and copies its bytes outside it, so submitters copy in
+// parallel; the request becomes batchable when its copy is done.  A worker
+// thread loops:
 //   - retires batches whose completion event fired (their per-document counts
 //     and flags were copied to pinned memory at the end of the batch), marks
 //     their requests done and frees their ring and staging space;
 //   - frees a slot once every request of its batch is released;
 //   - for every plan with pending requests and a free slot, takes the longest
-//     run of pending requests that is contiguous in staging and fits the batch
-//     limits, the device ring and the HBM budget, and enqueues H2D copy ->
+//     run of staged pending requests that is contiguous in staging and fits the
+//     batch limits, the device ring and the HBM budget, and enqueues H2D copy ->
 //     batched scan -> D2H of the per-document results -> completion event.
 // Batches of one plan run in order on its compute stream (so ring space frees
 // in allocation order); plans run on their own streams, and a batch's copy
@@ -53,14 +56,16 @@ uint64_t align_up(uint64_t x) {
 enum ReqState : int { kPending = 0, kInBatch = 1, kDone = 2 };
 
 struct Request {
-	uint64_t id;
-	uint32_t plan;
-	uint64_t n, stage_off, reserve;
-	uint64_t t_submit, t_launch = 0, t_done = 0;
-	int		 state	= kPending;
-	int		 slot	= -1;
-	uint32_t doc	= 0;
-	bool	 polled = false;
+	uint64_t id		   = 0;
+	uint32_t plan	   = 0;
+	uint64_t n		   = 0, stage_off = 0, reserve = 0;
+	uint64_t t_submit  = 0, t_launch = 0, t_done = 0;
+	uint64_t tag	   = 0;	 // the replay driver's request index
+	int		 state	   = kPending;
+	int		 slot	   = -1;
+	uint32_t doc	   = 0;
+	bool	 polled	   = false;
+	std::atomic<bool> staged {false};  // its bytes are in staging (set after the copy, outside the lock)
 	pars_request_result res {};
 };
 
@@ -73,7 +78,7 @@ struct Slot {
 	int					  fail_status = PARS_OK;  // the batch could not be enqueued: retire it as failed
 	uint64_t			  ring_off	  = 0;		  // its inputs in the device ring, held until the slot frees
 	bool				  ring_live	  = false;
-	std::vector<uint64_t> reqs;
+	std::vector<Request*> reqs;	 // stable: the request table never moves its elements
 	uint32_t			  outstanding = 0;
 	uint64_t			  launch_ns	  = 0;
 	BatchPtrs			  ptrs {};
@@ -118,6 +123,8 @@ struct pars_server {
 	void run();
 	bool retire();
 	bool launch(uint32_t p);
+	int	 submit(uint32_t plan, const uint8_t* data, size_t n, uint64_t tag, uint64_t* id);
+	void release_locked(Request& q);  // mu held
 };
 
 namespace {
@@ -182,9 +189,11 @@ bool pars_server::retire() {
 			const uint32_t* pk	   = s.h_pack;	// packed: starts, bracket starts, depths, flags, overflow
 			const bool		failed = s.fail_status || e != hipSuccess || batch_failed_word(pk[4 * nd + 2]);
 			const int		status = s.fail_status ? s.fail_status : PARS_ERR_DEVICE;
-			std::lock_guard<std::mutex> lk(mu);
+			// results are filled outside the lock: while a request is in a batch
+			// only this thread touches it; publishing it under the lock below
+			// orders these writes before any poll that returns it
 			for (size_t d = 0; d < nd; ++d) {
-				Request&			 q = reqs[s.reqs[d]];
+				Request&			 q = *s.reqs[d];
 				pars_request_result& r = q.res;
 				r					   = pars_request_result {};
 				r.id				   = q.id;
@@ -208,7 +217,11 @@ bool pars_server::retire() {
 				r.t_done_ns			   = t;
 				r.batch_docs		   = (uint32_t)s.reqs.size();
 				q.t_done			   = t;
-				q.state				   = kDone;
+			}
+			std::lock_guard<std::mutex> lk(mu);
+			for (size_t d = 0; d < nd; ++d) {
+				Request& q = *s.reqs[d];
+				q.state	   = kDone;
 				completed.push_back(q.id);
 				st.completed++;
 				if (!failed)
@@ -233,8 +246,8 @@ bool pars_server::retire() {
 bool pars_server::launch(uint32_t p) {
 	PlanState& ps	= plans[p];
 	int		   slot = -1;
-	std::vector<uint64_t> take;
-	uint64_t			  bytes = 0, first_off = 0;
+	uint64_t   bytes = 0, first_off = 0, ring_off = 0;
+	const uint64_t t = now_ns();
 	{
 		std::lock_guard<std::mutex> lk(mu);
 		for (size_t i = 0; i < ps.slots.size(); ++i)
@@ -248,48 +261,43 @@ bool pars_server::launch(uint32_t p) {
 		}
 		if (slot < 0 || ps.pending.empty())
 			return false;
+		Slot&		   s	 = ps.slots[slot];
 		const uint64_t avail = cfg.hbm_budget > fixed ? cfg.hbm_budget - fixed : 0;
-		uint64_t	   next	 = 0;
+		uint64_t	   next = 0, held = 0;
+		s.reqs.clear();
 		for (uint64_t id : ps.pending) {
-			const Request& q = reqs[id];
-			if (take.empty())
+			Request& q = reqs[id];
+			if (!q.staged.load(std::memory_order_acquire))	// its copy is still running
+				break;
+			if (s.reqs.empty())
 				first_off = next = q.stage_off;
 			else if (q.stage_off != next)  // staging wrapped: the batch ends here
 				break;
 			const uint64_t span = align_up(q.n);
-			if (take.size() >= cfg.max_batch_docs || bytes + span > cfg.max_batch_bytes)
+			if (s.reqs.size() >= cfg.max_batch_docs || bytes + span > cfg.max_batch_bytes)
 				break;
-			if (reserved + q.reserve > avail) {
+			if (reserved + held + q.reserve > avail) {
 				st.budget_waits++;
 				break;
 			}
-			reserved += q.reserve;
-			take.push_back(id);
+			held += q.reserve;
+			s.reqs.push_back(&q);
 			bytes += span;
 			next = q.stage_off + span;
 		}
-		if (take.empty())
+		if (s.reqs.empty())
 			return false;
-	}
-	uint64_t ring_off = 0;
-	{
-		std::lock_guard<std::mutex> lk(mu);
-		if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: give the budget back and retry later
-			for (uint64_t id : take) reserved -= reqs[id].reserve;
+		if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: retry once a slot frees
+			s.reqs.clear();
 			return false;
 		}
-	}
-	Slot& s		= ps.slots[slot];
-	s.reqs		= take;
-	s.ring_off	= ring_off;
-	s.ring_live = true;
-	s.offs.resize(take.size());
-	s.lens.resize(take.size());
-	const uint64_t t = now_ns();
-	{
-		std::lock_guard<std::mutex> lk(mu);
-		for (size_t d = 0; d < take.size(); ++d) {
-			Request& q = reqs[take[d]];
+		reserved += held;
+		s.ring_off	= ring_off;
+		s.ring_live = true;
+		s.offs.resize(s.reqs.size());
+		s.lens.resize(s.reqs.size());
+		for (size_t d = 0; d < s.reqs.size(); ++d) {
+			Request& q = *s.reqs[d];
 			s.offs[d]  = q.stage_off - first_off;
 			s.lens[d]  = (uint32_t)q.n;
 			q.state	   = kInBatch;
@@ -298,9 +306,10 @@ bool pars_server::launch(uint32_t p) {
 			q.t_launch = t;
 			ps.pending.pop_front();
 		}
-		s.outstanding = (uint32_t)take.size();
+		s.outstanding = (uint32_t)s.reqs.size();
 		st.batches++;
 	}
+	Slot& s = ps.slots[slot];
 	uint8_t*   d_in = ps.ring + ring_off;
 	hipError_t e	= hipMemcpyAsync(d_in, ps.stage + first_off, bytes, hipMemcpyHostToDevice, ps.copy);
 	if (e == hipSuccess)
@@ -309,7 +318,7 @@ bool pars_server::launch(uint32_t p) {
 		e = hipStreamWaitEvent(ps.compute, s.h2d, 0);
 	int r = e == hipSuccess ? PARS_OK : PARS_ERR_DEVICE;
 	if (r == PARS_OK)
-		r = batch_enqueue(s.sc, d_in, bytes, s.offs.data(), s.lens.data(), take.size(), ps.compute,
+		r = batch_enqueue(s.sc, d_in, bytes, s.offs.data(), s.lens.data(), s.reqs.size(), ps.compute,
 						  cfg.run_pairs != 0, &s.ptrs);
 	if (r == PARS_OK) {	 // one copy brings every per-request result back
 		e = hipMemcpyAsync(s.h_pack, s.ptrs.pack, s.ptrs.pack_bytes, hipMemcpyDeviceToHost, ps.compute);
@@ -357,6 +366,62 @@ void pars_server::run() {
 	}
 }
 
+// The staging space and the request record are taken under the lock; the copy
+// runs outside it, so concurrent submitters copy in parallel.  The batch
+// former stops at the first pending request whose copy has not finished.
+int pars_server::submit(uint32_t plan, const uint8_t* data, size_t n, uint64_t tag, uint64_t* id) {
+	if (n == 0 || n > cfg.max_batch_bytes || n > 0xFFFFFFFFull)
+		return sfail(PARS_ERR_ARGUMENT, "request size must be 1 .. max_batch_bytes");
+	PlanState& ps = plans[plan];
+	Request*   q  = nullptr;
+	uint64_t   off;
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		if (!ps.stage_ring.alloc(align_up(n), off)) {
+			st.busy_rejects++;
+			return sfail(PARS_ERR_BUSY, "staging ring full");
+		}
+		const uint64_t rid = next_id++;
+		q				   = &reqs.try_emplace(rid).first->second;
+		q->id			   = rid;
+		q->plan			   = plan;
+		q->n			   = n;
+		q->stage_off	   = off;
+		q->reserve		   = n * 18;  // the request's input plus its worst-case outputs
+		q->tag			   = tag;
+		q->t_submit		   = now_ns();
+		ps.pending.push_back(rid);
+		st.submitted++;
+		*id = rid;
+	}
+	std::memcpy(ps.stage + off, data, n);
+	if (align_up(n) > n)
+		std::memset(ps.stage + off + n, ' ', align_up(n) - n);
+	q->staged.store(true, std::memory_order_release);
+	work_seq++;
+	cv_work.notify_one();
+	return PARS_OK;
+}
+
+void pars_server::release_locked(Request& q) {
+	if (q.slot >= 0) {
+		Slot& s = plans[q.plan].slots[q.slot];
+		if (s.outstanding)
+			s.outstanding--;
+		if (s.outstanding == 0 && s.state == kHeld) {
+			s.state = kFree;
+			if (s.ring_live) {	// its inputs are no longer needed
+				plans[q.plan].dev_ring.release(s.ring_off);
+				s.ring_live = false;
+			}
+		}
+	}
+	reserved -= std::min(reserved, q.reserve);
+	st.released++;
+	reqs.erase(q.id);
+	work_seq++;
+}
+
 extern "C" {
 
 int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) {
@@ -376,6 +441,9 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 		cfg.max_batch_bytes = 64ull << 20;
 	if (!cfg.max_batch_docs)
 		cfg.max_batch_docs = 16384;
+	if (!cfg.submit_threads)
+		cfg.submit_threads = 8;
+	cfg.submit_threads = std::min<uint32_t>(cfg.submit_threads, 64);
 	cfg.max_batch_bytes = align_up(cfg.max_batch_bytes);
 	const uint64_t ring_each  = cfg.ring_bytes / cfg.n_plans / kAlign * kAlign;
 	const uint64_t stage_each = cfg.staging_bytes / cfg.n_plans / kAlign * kAlign;
@@ -385,6 +453,7 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 	if (!sv)
 		return sfail(PARS_ERR_MEMORY, "host allocation failed");
 	sv->cfg	 = cfg;
+	sv->reqs.reserve(1u << 16);
 	(void)hipGetDevice(&sv->dev);
 	sv->plans.resize(cfg.n_plans);
 	int r = PARS_OK;
@@ -447,32 +516,7 @@ int pars_hip_server_destroy(pars_server* sv) {
 int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, size_t n, uint64_t* id) {
 	if (!sv || !id || (n && !data) || plan >= sv->plans.size())
 		return sfail(PARS_ERR_ARGUMENT, "bad submit arguments");
-	if (n == 0 || n > sv->cfg.max_batch_bytes || n > 0xFFFFFFFFull)
-		return sfail(PARS_ERR_ARGUMENT, "request size must be 1 .. max_batch_bytes");
-	std::lock_guard<std::mutex> lk(sv->mu);
-	PlanState&					ps = sv->plans[plan];
-	uint64_t					off;
-	if (!ps.stage_ring.alloc(align_up(n), off)) {
-		sv->st.busy_rejects++;
-		return sfail(PARS_ERR_BUSY, "staging ring full");
-	}
-	std::memcpy(ps.stage + off, data, n);
-	if (align_up(n) > n)
-		std::memset(ps.stage + off + n, ' ', align_up(n) - n);
-	Request q;
-	q.id		= sv->next_id++;
-	q.plan		= plan;
-	q.n			= n;
-	q.stage_off = off;
-	q.reserve	= n * 18;  // the request's input plus its worst-case outputs
-	q.t_submit	= now_ns();
-	sv->reqs.emplace(q.id, q);
-	ps.pending.push_back(q.id);
-	sv->st.submitted++;
-	*id = q.id;
-	sv->work_seq++;
-	sv->cv_work.notify_one();
-	return PARS_OK;
+	return sv->submit(plan, data, n, 0, id);
 }
 
 int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, size_t* n_out, int64_t wait_us) {
@@ -495,27 +539,13 @@ int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, 
 int pars_hip_server_release(pars_server* sv, uint64_t id) {
 	if (!sv)
 		return sfail(PARS_ERR_ARGUMENT, "null server");
-	std::lock_guard<std::mutex> lk(sv->mu);
-	auto						it = sv->reqs.find(id);
-	if (it == sv->reqs.end() || it->second.state != kDone)
-		return sfail(PARS_ERR_ARGUMENT, "unknown or unfinished request");
-	Request& q = it->second;
-	if (q.slot >= 0) {
-		Slot& s = sv->plans[q.plan].slots[q.slot];
-		if (s.outstanding)
-			s.outstanding--;
-		if (s.outstanding == 0 && s.state == kHeld) {
-			s.state = kFree;
-			if (s.ring_live) {	// its inputs are no longer needed
-				sv->plans[q.plan].dev_ring.release(s.ring_off);
-				s.ring_live = false;
-			}
-		}
+	{
+		std::lock_guard<std::mutex> lk(sv->mu);
+		auto						it = sv->reqs.find(id);
+		if (it == sv->reqs.end() || it->second.state != kDone)
+			return sfail(PARS_ERR_ARGUMENT, "unknown or unfinished request");
+		sv->release_locked(it->second);
 	}
-	sv->reserved -= std::min(sv->reserved, q.reserve);
-	sv->st.released++;
-	sv->reqs.erase(it);
-	sv->work_seq++;
 	sv->cv_work.notify_one();
 	return PARS_OK;
 }
@@ -540,49 +570,95 @@ int pars_hip_server_replay(pars_server* sv, size_t n, const uint32_t* plan, cons
 						   uint64_t* elapsed_ns) {
 	if (!sv || !plan || !data || !sizes || !arrival_ns || !latency_ns)
 		return sfail(PARS_ERR_ARGUMENT, "null argument");
-	std::unordered_map<uint64_t, size_t> index;	 // request id -> i
-	std::vector<pars_request_result>	 res(256);
-	size_t								 done = 0;
-	uint64_t							 last = 0;
-	const uint64_t						 t0	  = now_ns();
-	uint64_t							 progress = t0;	 // last completion (stall guard)
-	auto								 drain	  = [&](int64_t wait_us) -> int {
-		size_t k = 0;
-		int	   r = pars_hip_server_poll(sv, res.data(), res.size(), &k, wait_us);
-		if (r)
-			return r;
-		if (k)
-			progress = now_ns();
-		else if (now_ns() - progress > 30ull * 1000000000ull && done < index.size())
-			return sfail(PARS_ERR_DEVICE, "server made no progress for 30 s");
-		for (size_t j = 0; j < k; ++j) {
-			const size_t i = index[res[j].id];
-			if (res[j].status != PARS_OK)
-				return sfail(res[j].status, "a request failed");
-			latency_ns[i] = res[j].t_done_ns - (t0 + arrival_ns[i]);
-			last		  = std::max(last, res[j].t_done_ns);
-			if ((r = pars_hip_server_release(sv, res[j].id)))
-				return r;
-			++done;
-		}
+	if (n == 0) {
+		if (elapsed_ns)
+			*elapsed_ns = 0;
 		return PARS_OK;
-	};
-	int r;
-	for (size_t i = 0; i < n; ++i) {
-		while (now_ns() < t0 + arrival_ns[i])
-			if ((r = drain(0)))
-				return r;
-		uint64_t id;
-		while ((r = pars_hip_server_submit(sv, plan[i], data[i], sizes[i], &id)) == PARS_ERR_BUSY)
-			if ((r = drain(20)))
-				return r;
-		if (r)
-			return r;
-		index[id] = i;
 	}
-	while (done < n)
-		if ((r = drain(1000)))
-			return r;
+	for (size_t i = 0; i < n; ++i)
+		if (plan[i] >= sv->plans.size() || (sizes[i] && !data[i]))
+			return sfail(PARS_ERR_ARGUMENT, "bad replay request");
+	const uint32_t		 nt = (uint32_t)std::min<size_t>(sv->cfg.submit_threads, n);
+	std::atomic<int>	 err {PARS_OK};
+	std::atomic<size_t>	 submitted {0};
+	std::string			 err_msg;
+	std::mutex			 err_mu;
+	auto				 fail_with = [&](int code) {
+		int expect = PARS_OK;
+		if (err.compare_exchange_strong(expect, code)) {
+			std::lock_guard<std::mutex> lk(err_mu);
+			err_msg = last_error();
+		}
+	};
+	const uint64_t t0 = now_ns();
+	// submitters: thread k takes requests k, k + nt, ... in arrival order
+	std::vector<std::thread> th;
+	th.reserve(nt);
+	for (uint32_t k = 0; k < nt; ++k)
+		th.emplace_back([&, k] {
+			for (size_t i = k; i < n && err.load(std::memory_order_relaxed) == PARS_OK; i += nt) {
+				while (now_ns() < t0 + arrival_ns[i])
+					if (err.load(std::memory_order_relaxed) != PARS_OK)
+						return;
+					else
+						std::this_thread::yield();
+				uint64_t id;
+				int		 r;
+				while ((r = sv->submit(plan[i], data[i], sizes[i], i, &id)) == PARS_ERR_BUSY) {
+					if (err.load(std::memory_order_relaxed) != PARS_OK)
+						return;
+					std::this_thread::sleep_for(std::chrono::microseconds(10));
+				}
+				if (r) {
+					fail_with(r);
+					return;
+				}
+				submitted.fetch_add(1, std::memory_order_relaxed);
+			}
+		});
+	// this thread: completions, released in groups under one lock
+	size_t	 done = 0;
+	uint64_t last = 0, progress = now_ns();
+	while (done < n && err.load() == PARS_OK) {
+		size_t got = 0;
+		{
+			std::unique_lock<std::mutex> lk(sv->mu);
+			if (sv->completed.empty())
+				sv->cv_done.wait_for(lk, std::chrono::microseconds(200), [sv] { return !sv->completed.empty(); });
+			while (!sv->completed.empty()) {
+				auto it = sv->reqs.find(sv->completed.front());
+				sv->completed.pop_front();
+				if (it == sv->reqs.end())
+					continue;
+				Request& q = it->second;
+				if (q.res.status != PARS_OK) {
+					err.store(q.res.status);
+					std::lock_guard<std::mutex> ek(err_mu);
+					err_msg = "a request failed";
+					break;
+				}
+				const size_t i = (size_t)q.tag;
+				latency_ns[i]  = q.res.t_done_ns - (t0 + arrival_ns[i]);
+				last		   = std::max(last, q.res.t_done_ns);
+				sv->release_locked(q);
+				++got;
+			}
+		}
+		if (got) {
+			done += got;
+			progress = now_ns();
+			sv->cv_work.notify_one();
+		} else if (now_ns() - progress > 30ull * 1000000000ull) {
+			int expect = PARS_OK;
+			if (err.compare_exchange_strong(expect, PARS_ERR_DEVICE)) {
+				std::lock_guard<std::mutex> ek(err_mu);
+				err_msg = "server made no progress for 30 s";
+			}
+		}
+	}
+	for (std::thread& t : th) t.join();
+	if (err.load() != PARS_OK)
+		return sfail(err.load(), err_msg.c_str());
 	if (elapsed_ns)
 		*elapsed_ns = last - (t0 + arrival_ns[0]);
 	return PARS_OK;

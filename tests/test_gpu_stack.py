@@ -150,3 +150,45 @@ def test_stream_and_chunks_pair_through_the_stack(torch_cuda):
     compare("json", d, got2, "chunks")
     sc.close()
     sc2.close()
+
+
+def test_sort_after_sparse_keys(torch_cuda):
+    """A scan meant for the stack path leaves keys only at chunk starts; every
+    sort of such a document (an unbalanced one falling back, chunks of one,
+    a digit cap set between the chunks' scans and their pairing) rebuilds the
+    other keys first and gives the oracle's pairs."""
+    from tests.gpu_util import d2h
+
+    torch = torch_cuda
+    base = soup(123, 90_000, 300, 0.0005, p_text=0.1)
+    cases = {
+        "extra_close": np.concatenate([base[:70_001], np.frombuffer(b"]}]", np.uint8), base[70_001:]]),
+        "close_first": np.concatenate([np.frombuffer(b"]", np.uint8), base]),
+        "unclosed": np.concatenate([base, np.frombuffer(b"{[[", np.uint8)]),
+    }
+    for kind, d in cases.items():
+        sc = scanner()
+        say(f"[gpu] sparse keys, whole document: {kind} ({d.size} B)")
+        got = gpu_full(torch, sc, d, want_pairs=True)
+        assert not got["balanced"]
+        compare("json", d, got, f"sparse {kind}")
+        sc.close()
+        for radix_late in (False, True):
+            t = torch.from_numpy(d).to("cuda")
+            torch.cuda.synchronize()
+            sc2 = Scanner(Plan.load("json"))
+            sc2.reserve_struct(d.size)
+            carry = N.ChunkCarry()
+            cut = [0, 33_333, 150_000, d.size]
+            for a0, b0 in zip(cut[:-1], cut[1:]):
+                carry = sc2.scan_chunk(t.data_ptr() + a0, b0 - a0, carry, byte_base=a0)
+            if radix_late:
+                sc2.set_option(N.PARS_OPT_RADIX_BITS, 6)
+            say(f"[gpu] sparse keys, chunks: {kind} radix_late={radix_late}")
+            rr = sc2.pair_chunks(carry, want_pairs=True)
+            exp = got  # the whole-document path, checked against the oracle above
+            assert np.array_equal(d2h(rr.bracket_pair, rr.n_brackets, np.int32), exp["pair"]), kind
+            assert np.array_equal(d2h(rr.pair_pos, rr.n_struct, np.int32), exp["pair_pos"]), kind
+            assert (rr.balanced, rr.well_formed, rr.max_depth) == (exp["balanced"], exp["well_formed"],
+                                                                   exp["max_depth"])
+            sc2.close()

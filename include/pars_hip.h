@@ -249,7 +249,7 @@ typedef struct {
 	uint64_t		 max_batch_bytes; /* per launch (default 64 MiB; also the largest request) */
 	uint32_t		 max_batch_docs;  /* per launch (default 16384) */
 	uint32_t		 run_pairs;		  /* run R6 (default 1) */
-	uint32_t		 submit_threads;  /* threads pars_hip_server_replay submits from (default 8) */
+	uint32_t		 submit_threads;  /* threads pars_hip_server_replay submits from (default 4) */
 	uint32_t		 reserved;
 } pars_server_config;
 
@@ -284,6 +284,10 @@ typedef struct {
 	uint64_t inflight_reserved;	 /* budget held by admitted, unreleased requests */
 	uint64_t ring_wraps;		 /* device ring allocations that wrapped to offset 0 past live data */
 	uint64_t staging_wraps;		 /* the same for the pinned staging ring */
+	uint64_t worker_launch_ns;	 /* host time the worker spent forming and enqueueing batches */
+	uint64_t worker_retire_ns;	 /* host time the worker spent retiring batches */
+	uint64_t submit_ns;			 /* host time inside submit, summed over the submitting threads */
+	uint64_t submit_lock_ns;	 /* of which waiting for the server lock */
 } pars_server_stats;
 
 int pars_hip_server_create(const pars_server_config* cfg, pars_server** out);

@@ -343,7 +343,7 @@ class Server:
 
     def __init__(self, plans, slots_per_plan: int = 2, ring_bytes: int = 4 << 30, staging_bytes: int = 1 << 30,
                  hbm_budget: int = 64 << 30, max_batch_bytes: int = 64 << 20, max_batch_docs: int = 16384,
-                 run_pairs: bool = True, submit_threads: int = 8):
+                 run_pairs: bool = True, submit_threads: int = 4):
         import numpy as np
 
         self.plans = [Plan.load(p) if isinstance(p, str) else p for p in plans]

@@ -171,6 +171,10 @@ class ServerStats(C.Structure):
         ("inflight_reserved", C.c_uint64),
         ("ring_wraps", C.c_uint64),
         ("staging_wraps", C.c_uint64),
+        ("worker_launch_ns", C.c_uint64),
+        ("worker_retire_ns", C.c_uint64),
+        ("submit_ns", C.c_uint64),
+        ("submit_lock_ns", C.c_uint64),
     ]
 
 

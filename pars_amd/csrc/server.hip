@@ -5,8 +5,9 @@
 // Per plan: a pinned staging ring, a region of the device ring, a copy stream,
 // a compute stream and `slots` scanners.  A submit reserves its staging space
 // under the server lock and copies its bytes outside it, so submitters copy in
-// parallel; the request becomes batchable when its copy is done.  A worker
-// thread loops:
+// parallel; the request becomes batchable when its copy is done.  One worker
+// thread per plan (batch enqueueing is host-bound: a batch is ~20 launches)
+// loops:
 //   - retires batches whose completion event fired (their per-document counts
 //     and flags were copied to pinned memory at the end of the batch), marks
 //     their requests done and frees their ring and staging space;
@@ -24,6 +25,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <emmintrin.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -34,7 +37,7 @@
 #include <new>
 #include <string>
 #include <thread>
-#include <unordered_map>
+#include <memory>
 #include <vector>
 
 using namespace pars_hip;
@@ -53,10 +56,56 @@ uint64_t align_up(uint64_t x) {
 	return (x + kAlign - 1) / kAlign * kAlign;
 }
 
+// A request's bytes into staging, padded with spaces to the next 64 bytes.
+// Staging is read only by the copy engine, so the stores bypass the caches
+// (non-temporal: no read-for-ownership of the destination lines); the fence
+// orders them before the request is marked staged.  dst is 64-byte aligned.
+void stage_copy(uint8_t* dst, const uint8_t* src, size_t n) {
+	const size_t body = n & ~(size_t)63;
+	for (size_t o = 0; o < body; o += 64) {
+		const __m128i a = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + o));
+		const __m128i b = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + o + 16));
+		const __m128i c = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + o + 32));
+		const __m128i d = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + o + 48));
+		_mm_stream_si128(reinterpret_cast<__m128i*>(dst + o), a);
+		_mm_stream_si128(reinterpret_cast<__m128i*>(dst + o + 16), b);
+		_mm_stream_si128(reinterpret_cast<__m128i*>(dst + o + 32), c);
+		_mm_stream_si128(reinterpret_cast<__m128i*>(dst + o + 48), d);
+	}
+	if (body < n) {
+		alignas(64) uint8_t tail[64];
+		std::memset(tail, ' ', sizeof tail);
+		std::memcpy(tail, src + body, n - body);
+		for (int q = 0; q < 4; ++q)
+			_mm_stream_si128(reinterpret_cast<__m128i*>(dst + body + 16 * q),
+							 _mm_load_si128(reinterpret_cast<const __m128i*>(tail + 16 * q)));
+	}
+	_mm_sfence();
+}
+
+// The server lock guards short critical sections (ring reservations, queue
+// pushes, a request record) taken by every submitter, the drain and the plan
+// workers at up to ~10^6 per second: a spin lock that pauses, then yields,
+// hands it over in tens of nanoseconds where a futex mutex would put waiters
+// to sleep.
+struct SpinLock {
+	std::atomic<bool> held {false};
+	void			  lock() {
+		 for (int spins = 0; held.exchange(true, std::memory_order_acquire);)
+			 while (held.load(std::memory_order_relaxed))
+				 if (++spins < 256)
+					 _mm_pause();
+				 else
+					 std::this_thread::yield();
+	}
+	bool try_lock() { return !held.load(std::memory_order_relaxed) && !held.exchange(true, std::memory_order_acquire); }
+	void unlock() { held.store(false, std::memory_order_release); }
+};
+
 enum ReqState : int { kPending = 0, kInBatch = 1, kDone = 2 };
 
 struct Request {
-	uint64_t id		   = 0;
+	uint64_t id		   = 0;	 // (generation << 32) | index in the request pool; 0 while free
 	uint32_t plan	   = 0;
 	uint64_t n		   = 0, stage_off = 0, reserve = 0;
 	uint64_t t_submit  = 0, t_launch = 0, t_done = 0;
@@ -94,7 +143,9 @@ struct PlanState {
 	uint8_t*		  ring = nullptr;  // device
 	Ring			  dev_ring;
 	hipStream_t		  copy = nullptr, compute = nullptr;
-	std::deque<uint64_t> pending;  // staged request ids in staging order
+	std::deque<Request*> pending;  // submitted requests in staging order (stable pointers into the table)
+	std::condition_variable_any cv;  // wakes this plan's idle worker
+	std::atomic<bool>		idle {false};
 	std::vector<Slot>	 slots;
 	std::deque<int>		 running;  // slots in launch order
 };
@@ -104,27 +155,40 @@ struct PlanState {
 struct pars_server {
 	pars_server_config			  cfg {};
 	std::vector<PlanState>		  plans;
-	std::mutex					  mu;
-	std::condition_variable		  cv_done;
-	std::condition_variable		  cv_work;	// a request was submitted or released (wakes an idle worker)
+	SpinLock					  mu;
+	std::condition_variable_any	  cv_done;
 	std::atomic<uint64_t>		  work_seq {0};
-	std::unordered_map<uint64_t, Request> reqs;
+	// request pool: chunks of kPoolChunk records that never move, a free list of
+	// indices; an id carries its slot's generation, so a stale id finds nothing
+	static constexpr uint32_t			  kPoolChunk = 4096;
+	std::vector<std::unique_ptr<Request[]>> pool;
+	std::vector<uint32_t>				  pool_gen;
+	std::vector<uint32_t>				  pool_free;
+	Request*							  req_alloc();			 // mu held
+	Request*							  req_find(uint64_t id);	 // mu held; nullptr if not live
+	void								  req_free(Request& q);	 // mu held
 	std::deque<uint64_t>		  completed;
-	uint64_t					  next_id  = 1;
 	uint64_t					  reserved = 0;	 // budget held by admitted, unreleased requests
 	uint64_t					  fixed	   = 0;	 // device bytes held for rings and arenas
 	pars_server_stats			  st {};
-	std::thread					  worker;
+	std::vector<std::thread>	  workers;	// one per plan
+	std::atomic<uint64_t>		  launch_ns {0}, retire_ns {0};
 	std::atomic<bool>			  stop {false};
+	std::atomic<uint64_t>		  submit_ns {0}, submit_lock_ns {0};
 	std::string					  error;
 	int							  dev = 0;
 
 	~pars_server();
-	void run();
-	bool retire();
+	void run(uint32_t p);
+	bool retire(uint32_t p);
 	bool launch(uint32_t p);
 	int	 submit(uint32_t plan, const uint8_t* data, size_t n, uint64_t tag, uint64_t* id);
 	void release_locked(Request& q);  // mu held
+	void wake_idle() {				  // a release frees slots, ring space and budget: any plan may launch
+		for (PlanState& p : plans)
+			if (p.idle.load())
+				p.cv.notify_one();
+	}
 };
 
 namespace {
@@ -141,12 +205,13 @@ int shipfail(hipError_t e, const char* what) {
 
 pars_server::~pars_server() {
 	{
-		std::lock_guard<std::mutex> lk(mu);
+		std::lock_guard<SpinLock> lk(mu);
 		stop = true;
 	}
-	cv_work.notify_all();
-	if (worker.joinable())
-		worker.join();
+	wake_idle();
+	for (std::thread& w : workers)
+		if (w.joinable())
+			w.join();
 	for (PlanState& p : plans) {
 		if (p.compute)
 			(void)hipStreamSynchronize(p.compute);
@@ -172,9 +237,9 @@ pars_server::~pars_server() {
 }
 
 // Batches whose completion event fired: fill their requests' results.
-bool pars_server::retire() {
+bool pars_server::retire(uint32_t p) {
 	bool any = false;
-	for (uint32_t p = 0; p < plans.size(); ++p) {
+	{
 		PlanState& ps = plans[p];
 		while (!ps.running.empty()) {
 			Slot& s = ps.slots[ps.running.front()];
@@ -218,7 +283,7 @@ bool pars_server::retire() {
 				r.batch_docs		   = (uint32_t)s.reqs.size();
 				q.t_done			   = t;
 			}
-			std::lock_guard<std::mutex> lk(mu);
+			std::lock_guard<SpinLock> lk(mu);
 			for (size_t d = 0; d < nd; ++d) {
 				Request& q = *s.reqs[d];
 				q.state	   = kDone;
@@ -249,7 +314,7 @@ bool pars_server::launch(uint32_t p) {
 	uint64_t   bytes = 0, first_off = 0, ring_off = 0;
 	const uint64_t t = now_ns();
 	{
-		std::lock_guard<std::mutex> lk(mu);
+		std::lock_guard<SpinLock> lk(mu);
 		for (size_t i = 0; i < ps.slots.size(); ++i)
 			if (ps.slots[i].state == kFree || (ps.slots[i].state == kHeld && ps.slots[i].outstanding == 0)) {
 				slot = (int)i;
@@ -265,8 +330,8 @@ bool pars_server::launch(uint32_t p) {
 		const uint64_t avail = cfg.hbm_budget > fixed ? cfg.hbm_budget - fixed : 0;
 		uint64_t	   next = 0, held = 0;
 		s.reqs.clear();
-		for (uint64_t id : ps.pending) {
-			Request& q = reqs[id];
+		for (Request* qp : ps.pending) {
+			Request& q = *qp;
 			if (!q.staged.load(std::memory_order_acquire))	// its copy is still running
 				break;
 			if (s.reqs.empty())
@@ -335,7 +400,7 @@ bool pars_server::launch(uint32_t p) {
 		// older and stay untouched).
 		(void)hipStreamSynchronize(ps.copy);
 		(void)hipStreamSynchronize(ps.compute);
-		std::lock_guard<std::mutex> lk(mu);
+		std::lock_guard<SpinLock> lk(mu);
 		error		  = std::string("batch launch failed: ") + last_error();
 		s.fail_status = r;
 	}
@@ -345,24 +410,41 @@ bool pars_server::launch(uint32_t p) {
 	return true;
 }
 
-void pars_server::run() {
+// Plan p's worker: retire its finished batches, launch what can go, then poll
+// the running batches' events (yielding for a while after the last progress,
+// then in short sleeps), or sleep until a submit or release wakes it.
+void pars_server::run(uint32_t p) {
 	(void)hipSetDevice(dev);
+	PlanState& ps	  = plans[p];
+	uint64_t   t_prog = now_ns();
 	while (!stop) {
-		const uint64_t seen = work_seq.load();
-		bool		   busy = retire();
-		for (uint32_t p = 0; p < plans.size(); ++p)
-			while (launch(p)) busy = true;
-		bool running = false;
-		for (const PlanState& ps : plans) running = running || !ps.running.empty();
-		if (busy)
-			continue;
-		if (running) {	// batches on the device: poll their events
-			std::this_thread::sleep_for(std::chrono::microseconds(5));
+		const uint64_t seen		= work_seq.load();
+		const uint64_t t0		= now_ns();
+		const bool	   retired	= retire(p);
+		const uint64_t t1		= now_ns();
+		bool		   launched = false;
+		while (launch(p)) launched = true;
+		if (retired)
+			retire_ns.fetch_add(t1 - t0, std::memory_order_relaxed);
+		if (launched)
+			launch_ns.fetch_add(now_ns() - t1, std::memory_order_relaxed);
+		if (retired || launched) {
+			t_prog = now_ns();
 			continue;
 		}
-		// idle: sleep until a submit or a release changes what can launch
-		std::unique_lock<std::mutex> lk(mu);
-		cv_work.wait_for(lk, std::chrono::microseconds(200), [this, seen] { return stop || work_seq.load() != seen; });
+		if (!ps.running.empty()) {	// batches on the device: poll their events
+			if (now_ns() - t_prog < 200000)
+				std::this_thread::yield();
+			else
+				std::this_thread::sleep_for(std::chrono::microseconds(20));
+			continue;
+		}
+		// idle: sleep until a submit or a release changes what can launch (the
+		// submitter bumps work_seq before it looks at the idle flag)
+		std::unique_lock<SpinLock> lk(mu);
+		ps.idle.store(true);
+		ps.cv.wait_for(lk, std::chrono::microseconds(500), [this, seen] { return stop || work_seq.load() != seen; });
+		ps.idle.store(false);
 	}
 }
 
@@ -372,35 +454,79 @@ void pars_server::run() {
 int pars_server::submit(uint32_t plan, const uint8_t* data, size_t n, uint64_t tag, uint64_t* id) {
 	if (n == 0 || n > cfg.max_batch_bytes || n > 0xFFFFFFFFull)
 		return sfail(PARS_ERR_ARGUMENT, "request size must be 1 .. max_batch_bytes");
-	PlanState& ps = plans[plan];
-	Request*   q  = nullptr;
-	uint64_t   off;
+	PlanState&	   ps = plans[plan];
+	Request*	   q  = nullptr;
+	uint64_t	   off;
+	const uint64_t t0 = now_ns();
 	{
-		std::lock_guard<std::mutex> lk(mu);
+		std::lock_guard<SpinLock> lk(mu);
+		submit_lock_ns.fetch_add(now_ns() - t0, std::memory_order_relaxed);
 		if (!ps.stage_ring.alloc(align_up(n), off)) {
 			st.busy_rejects++;
 			return sfail(PARS_ERR_BUSY, "staging ring full");
 		}
-		const uint64_t rid = next_id++;
-		q				   = &reqs.try_emplace(rid).first->second;
-		q->id			   = rid;
+		q = req_alloc();
+		if (!q)
+			return sfail(PARS_ERR_MEMORY, "host allocation failed");
+		const uint64_t rid = q->id;
 		q->plan			   = plan;
 		q->n			   = n;
 		q->stage_off	   = off;
 		q->reserve		   = n * 18;  // the request's input plus its worst-case outputs
 		q->tag			   = tag;
 		q->t_submit		   = now_ns();
-		ps.pending.push_back(rid);
+		ps.pending.push_back(q);
 		st.submitted++;
 		*id = rid;
 	}
-	std::memcpy(ps.stage + off, data, n);
-	if (align_up(n) > n)
-		std::memset(ps.stage + off + n, ' ', align_up(n) - n);
+	stage_copy(ps.stage + off, data, n);
 	q->staged.store(true, std::memory_order_release);
 	work_seq++;
-	cv_work.notify_one();
+	if (ps.idle.load())
+		ps.cv.notify_one();
+	submit_ns.fetch_add(now_ns() - t0, std::memory_order_relaxed);
 	return PARS_OK;
+}
+
+Request* pars_server::req_alloc() {
+	if (pool_free.empty()) {
+		const uint32_t base = (uint32_t)pool_gen.size();
+		if (base > 0xFFFFFFFFu - kPoolChunk)
+			return nullptr;
+		std::unique_ptr<Request[]> c(new (std::nothrow) Request[kPoolChunk]);
+		if (!c)
+			return nullptr;
+		pool.push_back(std::move(c));
+		pool_gen.resize(base + kPoolChunk, 0u);
+		for (uint32_t i = kPoolChunk; i-- > 0;) pool_free.push_back(base + i);
+	}
+	const uint32_t idx = pool_free.back();
+	pool_free.pop_back();
+	Request& q = pool[idx / kPoolChunk][idx % kPoolChunk];
+	const uint32_t g = ++pool_gen[idx] ? pool_gen[idx] : ++pool_gen[idx];	// never 0: id 0 means free
+	q.id			 = ((uint64_t)g << 32) | idx;
+	q.t_launch = q.t_done = 0;
+	q.state				 = kPending;
+	q.slot				 = -1;
+	q.doc				 = 0;
+	q.polled			 = false;
+	q.staged.store(false, std::memory_order_relaxed);
+	q.res = pars_request_result {};
+	return &q;
+}
+
+Request* pars_server::req_find(uint64_t id) {
+	const uint32_t idx = (uint32_t)id;
+	if (id == 0 || idx >= pool_gen.size())
+		return nullptr;
+	Request& q = pool[idx / kPoolChunk][idx % kPoolChunk];
+	return q.id == id ? &q : nullptr;
+}
+
+void pars_server::req_free(Request& q) {
+	const uint32_t idx = (uint32_t)q.id;
+	q.id			   = 0;
+	pool_free.push_back(idx);
 }
 
 void pars_server::release_locked(Request& q) {
@@ -418,7 +544,7 @@ void pars_server::release_locked(Request& q) {
 	}
 	reserved -= std::min(reserved, q.reserve);
 	st.released++;
-	reqs.erase(q.id);
+	req_free(q);
 	work_seq++;
 }
 
@@ -442,7 +568,7 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 	if (!cfg.max_batch_docs)
 		cfg.max_batch_docs = 16384;
 	if (!cfg.submit_threads)
-		cfg.submit_threads = 8;
+		cfg.submit_threads = 4;
 	cfg.submit_threads = std::min<uint32_t>(cfg.submit_threads, 64);
 	cfg.max_batch_bytes = align_up(cfg.max_batch_bytes);
 	const uint64_t ring_each  = cfg.ring_bytes / cfg.n_plans / kAlign * kAlign;
@@ -453,9 +579,8 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 	if (!sv)
 		return sfail(PARS_ERR_MEMORY, "host allocation failed");
 	sv->cfg	 = cfg;
-	sv->reqs.reserve(1u << 16);
 	(void)hipGetDevice(&sv->dev);
-	sv->plans.resize(cfg.n_plans);
+	sv->plans = std::vector<PlanState>(cfg.n_plans);  // constructed in place (PlanState does not move)
 	int r = PARS_OK;
 	for (uint32_t p = 0; p < cfg.n_plans && r == PARS_OK; ++p) {
 		PlanState& ps = sv->plans[p];
@@ -470,6 +595,9 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 			r = shipfail(e, "server rings");
 			break;
 		}
+		// map every staging page now: the kernel would otherwise fault them in on
+		// the first submits' copies (a minor fault per 4 KiB page)
+		for (uint64_t o = 0; o < stage_each; o += 4096) ps.stage[o] = 0;
 		ps.stage_ring.cap = stage_each;
 		ps.dev_ring.cap	  = ring_each;
 		sv->fixed += ring_each;
@@ -503,7 +631,7 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 		return r;
 	}
 	sv->st.device_bytes = sv->fixed;
-	sv->worker			= std::thread([sv] { sv->run(); });
+	for (uint32_t p = 0; p < cfg.n_plans; ++p) sv->workers.emplace_back([sv, p] { sv->run(p); });
 	*out				= sv;
 	return PARS_OK;
 }
@@ -522,12 +650,17 @@ int pars_hip_server_submit(pars_server* sv, uint32_t plan, const uint8_t* data, 
 int pars_hip_server_poll(pars_server* sv, pars_request_result* out, size_t cap, size_t* n_out, int64_t wait_us) {
 	if (!sv || !n_out || (cap && !out))
 		return sfail(PARS_ERR_ARGUMENT, "bad poll arguments");
-	std::unique_lock<std::mutex> lk(sv->mu);
+	std::unique_lock<SpinLock> lk(sv->mu);
 	if (sv->completed.empty() && wait_us > 0)
 		sv->cv_done.wait_for(lk, std::chrono::microseconds(wait_us), [sv] { return !sv->completed.empty(); });
 	size_t k = 0;
 	while (k < cap && !sv->completed.empty()) {
-		Request& q = sv->reqs[sv->completed.front()];
+		Request* qp = sv->req_find(sv->completed.front());
+		if (!qp) {	// released without being polled
+			sv->completed.pop_front();
+			continue;
+		}
+		Request& q = *qp;
 		sv->completed.pop_front();
 		q.polled = true;
 		out[k++] = q.res;
@@ -540,24 +673,28 @@ int pars_hip_server_release(pars_server* sv, uint64_t id) {
 	if (!sv)
 		return sfail(PARS_ERR_ARGUMENT, "null server");
 	{
-		std::lock_guard<std::mutex> lk(sv->mu);
-		auto						it = sv->reqs.find(id);
-		if (it == sv->reqs.end() || it->second.state != kDone)
+		std::lock_guard<SpinLock> lk(sv->mu);
+		Request*					q = sv->req_find(id);
+		if (!q || q->state != kDone)
 			return sfail(PARS_ERR_ARGUMENT, "unknown or unfinished request");
-		sv->release_locked(it->second);
+		sv->release_locked(*q);
 	}
-	sv->cv_work.notify_one();
+	sv->wake_idle();
 	return PARS_OK;
 }
 
 int pars_hip_server_stats(pars_server* sv, pars_server_stats* out) {
 	if (!sv || !out)
 		return sfail(PARS_ERR_ARGUMENT, "null argument");
-	std::lock_guard<std::mutex> lk(sv->mu);
+	std::lock_guard<SpinLock> lk(sv->mu);
 	*out				   = sv->st;
 	out->inflight_reserved = sv->reserved;
 	out->ring_wraps		   = 0;
 	out->staging_wraps	   = 0;
+	out->submit_ns		   = sv->submit_ns.load();
+	out->worker_launch_ns  = sv->launch_ns.load();
+	out->worker_retire_ns  = sv->retire_ns.load();
+	out->submit_lock_ns	   = sv->submit_lock_ns.load();
 	for (const PlanState& p : sv->plans) {
 		out->ring_wraps += p.dev_ring.wraps;
 		out->staging_wraps += p.stage_ring.wraps;
@@ -597,11 +734,17 @@ int pars_hip_server_replay(pars_server* sv, size_t n, const uint32_t* plan, cons
 	for (uint32_t k = 0; k < nt; ++k)
 		th.emplace_back([&, k] {
 			for (size_t i = k; i < n && err.load(std::memory_order_relaxed) == PARS_OK; i += nt) {
-				while (now_ns() < t0 + arrival_ns[i])
+				// sleep through long waits (spinning submitters would take CPU from
+				// the worker), spin through the last stretch
+				for (uint64_t now = now_ns(); now < t0 + arrival_ns[i]; now = now_ns()) {
 					if (err.load(std::memory_order_relaxed) != PARS_OK)
 						return;
+					const uint64_t wait = t0 + arrival_ns[i] - now;
+					if (wait > 200000)
+						std::this_thread::sleep_for(std::chrono::nanoseconds(wait - 150000));
 					else
 						std::this_thread::yield();
+				}
 				uint64_t id;
 				int		 r;
 				while ((r = sv->submit(plan[i], data[i], sizes[i], i, &id)) == PARS_ERR_BUSY) {
@@ -622,15 +765,15 @@ int pars_hip_server_replay(pars_server* sv, size_t n, const uint32_t* plan, cons
 	while (done < n && err.load() == PARS_OK) {
 		size_t got = 0;
 		{
-			std::unique_lock<std::mutex> lk(sv->mu);
+			std::unique_lock<SpinLock> lk(sv->mu);
 			if (sv->completed.empty())
 				sv->cv_done.wait_for(lk, std::chrono::microseconds(200), [sv] { return !sv->completed.empty(); });
 			while (!sv->completed.empty()) {
-				auto it = sv->reqs.find(sv->completed.front());
+				Request* qp = sv->req_find(sv->completed.front());
 				sv->completed.pop_front();
-				if (it == sv->reqs.end())
+				if (!qp)
 					continue;
-				Request& q = it->second;
+				Request& q = *qp;
 				if (q.res.status != PARS_OK) {
 					err.store(q.res.status);
 					std::lock_guard<std::mutex> ek(err_mu);
@@ -647,7 +790,7 @@ int pars_hip_server_replay(pars_server* sv, size_t n, const uint32_t* plan, cons
 		if (got) {
 			done += got;
 			progress = now_ns();
-			sv->cv_work.notify_one();
+			sv->wake_idle();
 		} else if (now_ns() - progress > 30ull * 1000000000ull) {
 			int expect = PARS_OK;
 			if (err.compare_exchange_strong(expect, PARS_ERR_DEVICE)) {

@@ -138,3 +138,59 @@ def test_server_replay_poisson(torch_cuda):
     st = sv.stats()
     assert st.completed == len(docs) and st.released == len(docs)
     sv.close()
+
+
+def test_concurrent_submitters(torch_cuda):
+    """Four threads submit at once (the copies into staging run outside the
+    server lock, the request becomes batchable when its copy is done) while
+    this thread polls and releases: every result bit-exact, ids unique, and
+    the staging and device rings, budget and request pool back to empty."""
+    import threading
+    import time
+
+    docs = request_docs(17, 600)
+    sv = Server(PLANS, ring_bytes=96 << 20, staging_bytes=48 << 20, max_batch_bytes=2 << 20, max_batch_docs=256)
+    ids: dict[int, int] = {}
+    lock = threading.Lock()
+    errors: list[str] = []
+
+    def submitter(k: int):
+        try:
+            for i in range(k, len(docs), 4):
+                t0 = time.monotonic()
+                while (rid := sv.submit(docs[i][0], docs[i][1])) is None:
+                    if time.monotonic() - t0 > 30:
+                        raise RuntimeError("staging stayed full for 30 s")
+                    time.sleep(0.0005)
+                with lock:
+                    assert rid not in ids, "an id was handed out twice"
+                    ids[rid] = i
+        except Exception as e:  # reported by the main thread
+            errors.append(repr(e))
+
+    th = [threading.Thread(target=submitter, args=(k,)) for k in range(4)]
+    for t in th:
+        t.start()
+    results, last = {}, time.monotonic()
+    while len(results) < len(docs) and not errors:
+        assert time.monotonic() - last < 30, "server made no progress for 30 s"
+        for r in sv.poll(wait_us=2000):
+            assert r.status == 0
+            while True:  # the submitter records the id right after submit returns
+                with lock:
+                    i = ids.get(r.id)
+                if i is not None:
+                    break
+                time.sleep(0.0001)
+            results[i] = fetch(r)
+            sv.release(r.id)
+            last = time.monotonic()
+    for t in th:
+        t.join()
+    assert not errors, errors
+    say(f"server: {len(docs)} requests from 4 threads in {sv.stats().batches} batches")
+    for i, (p, d) in enumerate(docs):
+        check(PLANS[p], d, results[i], f"request {i} (concurrent submit)")
+    st = sv.stats()
+    assert st.submitted == st.completed == st.released == len(docs) and st.inflight_reserved == 0
+    sv.close()

@@ -583,7 +583,7 @@ def main() -> int:
         torch.cuda.synchronize()
         sc3, r3, m3, _ = bench_document(torch, "clike6", d3.data_ptr(), n3, max(args.steps // 2, 3), args.warmup,
                                         args.knobs, not args.no_tune, "k_scan_clike6_1gib.md",
-                                        "k_scan<GeneralEval<5, true>, 1> (R1-R5)")
+                                        "k_scan<EventEval<5, true>, 2> (R1-R5)")
         line3 = {
             "workload": f"clike6-{n3 >> 20}MiB plain pipeline (R1-R6, device-resident, N1b; six modes, 5 descriptors)",
             "bytes": n3,

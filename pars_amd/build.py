@@ -23,8 +23,8 @@ OBJ = os.path.join(HERE, "_obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_SOURCES = [
-    "scan_kernels.hip", "batch_kernels.hip", "pair_kernels.hip", "pair_stack.hip", "dom_kernels.hip", "api.hip",
-    "server.hip",
+    "scan_kernels.hip", "batch_kernels.hip", "event_kernels.hip", "pair_kernels.hip", "pair_stack.hip",
+    "dom_kernels.hip", "api.hip", "server.hip",
 ]
 CXX_SOURCES = ["plan_dev.cpp"]
 HIP_FLAGS = [
@@ -112,6 +112,15 @@ def build_ring_check() -> str:
     return out
 
 
+def build_event_check() -> str:
+    """tests/cpp/event_check: the event path's tables (csrc/plan_dev.cpp) and its
+    stepping, against byte-by-byte DFA steps, host-only."""
+    src = os.path.join(ROOT, "tests", "cpp", "event_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "event_check")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", src, os.path.join(CSRC, "plan_dev.cpp"), "-o", out])
+    return out
+
+
 def build_all() -> None:
     build_gen()
     build_oracle()
@@ -119,6 +128,7 @@ def build_all() -> None:
     build_facade_check()
     build_gap_check()
     build_ring_check()
+    build_event_check()
 
 
 if __name__ == "__main__":

@@ -407,6 +407,8 @@ hipError_t launch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
 		return launch_scan_escape(a, sc->rows, s);
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
 		return launch_scan_parity(a, sc->rows, s);
+	if (sc->path == 0 && event_path_applies(sc->hplan))
+		return launch_scan_event(a, sc->hplan.n_descs, sc->hplan.wide, s);
 	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 
@@ -490,6 +492,8 @@ hipError_t launch_batch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s)
 		return launch_batch_escape(a, sc->rows, s);
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
 		return launch_batch_parity(a, sc->rows, s);
+	if (sc->path == 0 && event_path_applies(sc->hplan))
+		return launch_batch_event(a, sc->hplan.n_descs, sc->hplan.wide, s);
 	return launch_batch_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 

@@ -155,6 +155,8 @@ struct GeneralEval {
 		}
 	}
 
+	__device__ uint64_t masked(const Lds&, const uint32_t (&s0)[NDA]) const { return masked(s0); }
+
 	__device__ uint64_t masked(const uint32_t (&s0)[NDA]) const {
 		uint64_t m = 0;
 #pragma unroll
@@ -328,6 +330,8 @@ struct EscapeEval {
 		T[0].lo = (tO << (8 * f0)) | (tI << (8 * f1)) | (tO << (8 * f2)) | (tE << (8 * f3));
 	}
 
+	__device__ uint64_t masked(const Lds&, const uint32_t (&s0)[1]) const { return masked(s0); }
+
 	__device__ uint64_t masked(const uint32_t (&s0)[1]) const {
 		// blend with masks, not a select (a select of members lowers to scratch)
 		const uint64_t kI = 0ull - (uint64_t)(s0[0] == st_in), kE = 0ull - (uint64_t)(s0[0] == st_esc);
@@ -397,6 +401,8 @@ struct ParityEval {
 		T[0].lo = (fe << (8 * f_out)) | (fo << (8 * f_in)) | (fe << (8 * f_cl)) | (3u << 24);
 	}
 
+	__device__ uint64_t masked(const Lds&, const uint32_t (&s0)[1]) const { return masked(s0); }
+
 	__device__ uint64_t masked(const uint32_t (&s0)[1]) const {
 		const uint64_t odd = s0[0] == f_in ? ~Pe : Pe;
 		if (!doubled)
@@ -419,6 +425,146 @@ struct ParityEval {
 		op = OP;
 	}
 
+};
+
+// ---------------------------------------------------------------------------
+// EventEval<ND, WIDE>: plans of two or more descriptors whose background runs
+// act as a single byte (the DevPlan::ev_* tables).  A lane finds its event
+// bytes (a byte that is not background for some descriptor) among its class
+// lookups, then
+//   walk:   composes each descriptor's map at the events only (the background
+//           run before an event is folded into the event's map, a run to the
+//           lane's end is one more step): the lane's maps for every entry
+//           state, as the per-byte walk would give them;
+//   masked: once the lane's entry states are known, steps the joint state of
+//           all descriptors, one table lookup per event, and sets the masked
+//           bits of each background run (its first byte, the rest) and of the
+//           event byte.
+// Exact for every input: inside a run the per-byte walk would apply the same
+// idempotent map again and again, so every run byte after the first sees the
+// same states.
+// ---------------------------------------------------------------------------
+template<int ND, bool WIDE>
+struct EventEval {
+	static constexpr int  kND		   = ND;
+	static constexpr bool kWide		   = WIDE;
+	static constexpr bool kBitParallel = false;
+	using M							   = MapT<WIDE>;
+	using Map						   = typename M::type;
+
+	struct Lds {
+		uint2	 cls[8 * 256];			   // DevPlan::evc
+		uint2	 map[kEvIds * 2 * ND];	   // [(id * 2 + g) * ND + d]: ev_map lo, hi
+		uint16_t joint[kMaxJoint * kEvIds * 2];  // [(J * kEvIds + id) * 2 + g]: ev_joint
+		uint16_t stride[ND];
+	};
+
+	__device__ static void load(Lds& L, const DevPlan* P) {
+		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) {
+			const uint32_t* e = P->evc[i >> 8][i & 255];
+			L.cls[i]		  = make_uint2(e[0], e[1]);
+		}
+		for (int i = threadIdx.x; i < kEvIds * 2 * ND; i += kBlock) {
+			const int d = i % ND, g = (i / ND) & 1, id = i / (2 * ND);
+			L.map[i]	= make_uint2(P->ev_map[id][g][d][0], P->ev_map[id][g][d][1]);
+		}
+		const int nj = (int)P->ev_n_joint * kEvIds * 2;
+		for (int i = threadIdx.x; i < nj; i += kBlock) L.joint[i] = (&P->ev_joint[0][0][0])[i];
+		if (threadIdx.x < ND)
+			L.stride[threadIdx.x] = P->ev_stride[threadIdx.x];
+	}
+
+	Map		 T[ND];
+	uint64_t S, OC, OP;
+	uint64_t E0, E1, E2;  // event id bit planes
+	uint32_t nv;		  // valid bytes of the lane
+
+	__device__ uint32_t event_id(int p) const {
+		return ((uint32_t)(E0 >> p) & 1u) | (((uint32_t)(E1 >> p) & 1u) << 1) | (((uint32_t)(E2 >> p) & 1u) << 2);
+	}
+
+	__device__ static Map map_at(const uint2& e) {
+		if constexpr (WIDE)
+			return Map {e.x, e.y};
+		else
+			return Map {e.x};
+	}
+
+	__device__ void walk(const Lds& L, const DevPlan*, const uint32_t (&x)[16], uint32_t valid) {
+		uint32_t clo[8], chi[8];
+		lookup_groups(L.cls, x, clo, chi);
+		uint64_t e0;
+		gather_classes(clo, S, OC, OP, e0);
+		const uint64_t vm = valid_mask(valid);	// bytes past the lane's end have no class
+		S &= vm;
+		OC &= vm;
+		OP &= vm;
+		E0 = e0 & vm;
+		E1 = gather_byte0(chi) & vm;
+		E2 = gather_byte1(chi) & vm;
+		nv = valid;
+#pragma unroll
+		for (int d = 0; d < ND; ++d) T[d] = M::id();
+		uint64_t u = E0 | E1 | E2;
+		int		 q = -1;  // the last event (or -1)
+		while (u) {
+			const int p = __builtin_ctzll(u);
+			u &= u - 1ull;
+			const uint2* mp = &L.map[(event_id(p) * 2 + (p > q + 1 ? 1 : 0)) * ND];
+#pragma unroll
+			for (int d = 0; d < ND; ++d) T[d] = M::compose(map_at(mp[d]), T[d]);
+			q = p;
+		}
+		if (q < (int)valid - 1) {  // a background run to the lane's end
+			const uint2* mp = &L.map[1 * ND];
+#pragma unroll
+			for (int d = 0; d < ND; ++d) T[d] = M::compose(map_at(mp[d]), T[d]);
+		}
+	}
+
+	// bits a .. b-1 (0 <= a < b <= 64)
+	__device__ static uint64_t span(int a, int b) {
+		const uint64_t hi = b >= 64 ? ~0ull : ((1ull << b) - 1ull);
+		return hi & ~((1ull << a) - 1ull);
+	}
+
+	// the masked bits of the background run q+1 .. end-1 from its table entry
+	__device__ static uint64_t run_bits(uint32_t e, int q, int end) {
+		uint64_t m = 0;
+		if (end > q + 1) {
+			if (e & (1u << 12))
+				m |= 1ull << (q + 1);
+			if ((e & (1u << 13)) && end > q + 2)
+				m |= span(q + 2, end);
+		}
+		return m;
+	}
+
+	__device__ uint64_t masked(const Lds& L, const uint32_t (&s0)[ND]) const {
+		uint32_t J = 0;
+#pragma unroll
+		for (int d = 0; d < ND; ++d) J += s0[d] * (uint32_t)L.stride[d];
+		uint64_t m = 0, u = E0 | E1 | E2;
+		int		 q = -1;
+		while (u) {
+			const int p = __builtin_ctzll(u);
+			u &= u - 1ull;
+			const uint32_t e = L.joint[(J * kEvIds + event_id(p)) * 2 + (p > q + 1 ? 1u : 0u)];
+			m |= run_bits(e, q, p);
+			m |= (uint64_t)((e >> 14) & 1u) << p;
+			J = e & 0xFFFu;
+			q = p;
+		}
+		if (q < (int)nv - 1)
+			m |= run_bits(L.joint[(J * kEvIds) * 2 + 1], q, (int)nv);
+		return m;
+	}
+
+	__device__ void classes(uint64_t& s, uint64_t& oc, uint64_t& op) const {
+		s  = S;
+		oc = OC;
+		op = OP;
+	}
 };
 
 // ---------------------------------------------------------------------------

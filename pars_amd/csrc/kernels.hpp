@@ -137,6 +137,11 @@ hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, int ro
 hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s);
 // R1-R5 for plans whose one descriptor is quote parity (CSV doubled quotes).
 hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
+// R1-R5 for plans of two to five descriptors whose background runs act as one
+// byte (DevPlan::ev_ok): the lanes step at their event bytes only.
+bool	   event_path_applies(const DevPlan& p);
+hipError_t launch_scan_event(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
+hipError_t launch_batch_event(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
 // The same three over a batch (a.b filled in); the document heads must be set.
 hipError_t launch_batch_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 hipError_t launch_batch_escape(const ScanArgs& a, int rows, hipStream_t s);

@@ -106,6 +106,108 @@ bool is_parity_desc(const pars_lex_desc& d, uint8_t& dl, uint8_t st[4], uint8_t&
 	return true;
 }
 
+// The event path's tables (DevPlan::evc, ev_map, ev_joint); false when the
+// plan does not fit it (fewer than two descriptors, a background map that is
+// not idempotent, more than 7 event ids or kMaxJoint joint states).
+bool lower_events(const pars_plan& S, DevPlan& P) {
+	const unsigned nd = S.n_descs;
+	if (nd < 2)
+		return false;
+	uint8_t bg[kMaxDescs];
+	for (unsigned d = 0; d < nd; ++d) {
+		const pars_lex_desc& D = S.descs[d];
+		int					 cnt[PARS_MAX_DFA_CLASSES] = {0};
+		for (int c = 0; c < 256; ++c) cnt[D.rep_of[c]]++;
+		bg[d] = 0;
+		for (int k = 1; k < D.n_classes; ++k)
+			if (cnt[k] > cnt[bg[d]])
+				bg[d] = (uint8_t)k;
+		for (int s = 0; s < D.n_states; ++s) {
+			const uint8_t f = D.trans[s][bg[d]];
+			if (D.trans[f][bg[d]] != f)
+				return false;
+		}
+	}
+	// event ids: distinct class tuples of the event bytes, numbered from 1 in
+	// byte order
+	uint8_t ev_id[256];
+	uint8_t cls_of[kEvIds][kMaxDescs];	// per id, each descriptor's class
+	int		n_ids = 0;
+	for (unsigned d = 0; d < nd; ++d) cls_of[0][d] = bg[d];
+	for (int c = 0; c < 256; ++c) {
+		bool event = false;
+		for (unsigned d = 0; d < nd; ++d) event = event || S.descs[d].rep_of[c] != bg[d];
+		if (!event) {
+			ev_id[c] = 0;
+			continue;
+		}
+		int id = -1;
+		for (int i = 1; i <= n_ids && id < 0; ++i) {
+			bool same = true;
+			for (unsigned d = 0; d < nd; ++d) same = same && cls_of[i][d] == S.descs[d].rep_of[c];
+			if (same)
+				id = i;
+		}
+		if (id < 0) {
+			if (n_ids == kEvIds - 1)
+				return false;
+			id = ++n_ids;
+			for (unsigned d = 0; d < nd; ++d) cls_of[id][d] = S.descs[d].rep_of[c];
+		}
+		ev_id[c] = (uint8_t)id;
+	}
+	uint32_t n_joint = 1;
+	for (unsigned d = 0; d < nd; ++d) {
+		P.ev_stride[d] = (uint16_t)n_joint;
+		n_joint *= S.descs[d].n_states;
+		if (n_joint > (uint32_t)kMaxJoint)
+			return false;
+	}
+	for (int j = 0; j < 8; ++j)
+		for (int c = 0; c < 256; ++c) {
+			const uint32_t id = ev_id[c];
+			P.evc[j][c][0]	  = P.cls[j][c] | ((id & 1u) << (24 + j));
+			P.evc[j][c][1]	  = (((id >> 1) & 1u) << j) | (((id >> 2) & 1u) << (8 + j));
+		}
+	for (int i = 0; i < kEvIds; ++i)
+		for (int g = 0; g < 2; ++g)
+			for (unsigned d = 0; d < nd; ++d) {
+				const pars_lex_desc& D = S.descs[d];
+				uint8_t				 m[8];
+				for (int s = 0; s < 8; ++s) {
+					if (s >= D.n_states || i > n_ids) {
+						m[s] = (uint8_t)s;
+						continue;
+					}
+					const uint8_t s1 = g ? D.trans[s][bg[d]] : (uint8_t)s;
+					m[s]			 = D.trans[s1][cls_of[i][d]];
+				}
+				P.ev_map[i][g][d][0] = (uint32_t)m[0] | ((uint32_t)m[1] << 8) | ((uint32_t)m[2] << 16) | ((uint32_t)m[3] << 24);
+				P.ev_map[i][g][d][1] = (uint32_t)m[4] | ((uint32_t)m[5] << 8) | ((uint32_t)m[6] << 16) | ((uint32_t)m[7] << 24);
+			}
+	for (uint32_t J = 0; J < n_joint; ++J)
+		for (int i = 0; i < kEvIds; ++i)
+			for (int g = 0; g < 2; ++g) {
+				uint32_t next = 0, first = 0, rest = 0, ev = 0;
+				for (unsigned d = 0; d < nd && i <= n_ids; ++d) {
+					const pars_lex_desc& D = S.descs[d];
+					const uint32_t		 s = (J / P.ev_stride[d]) % D.n_states;
+					uint32_t			 t = s;	 // state before the event byte
+					if (g) {
+						first |= D.masked[s][bg[d]];
+						t = D.trans[s][bg[d]];
+						rest |= D.masked[t][bg[d]];
+					}
+					ev |= D.masked[t][cls_of[i][d]];
+					next += D.trans[t][cls_of[i][d]] * P.ev_stride[d];
+				}
+				P.ev_joint[J][i][g] = (uint16_t)(next | (first << 12) | (rest << 13) | (ev << 14));
+			}
+	P.ev_n_joint = (uint16_t)n_joint;
+	P.ev_n_ids	 = (uint8_t)n_ids;
+	return true;
+}
+
 }  // namespace
 
 int lower_plan(const pars_plan& S, DevPlan& P) {
@@ -187,6 +289,7 @@ int lower_plan(const pars_plan& S, DevPlan& P) {
 		}
 	}
 	P.br_bit_ok = n_open <= 2 && n_close <= 2;
+	P.ev_ok		= lower_events(S, P) ? 1 : 0;
 	for (int j = 0; j < 8; ++j)
 		for (int c = 0; c < 256; ++c) {
 			const uint32_t q = P.fast_kind != kFastNone && c == P.fk_delim;

@@ -15,6 +15,8 @@
 namespace pars_hip {
 
 constexpr int kMaxDescs = 8;
+constexpr int kMaxJoint = 512;	// joint states (product of the descriptors' state counts) of the event path
+constexpr int kEvIds	= 8;	// event ids 1..7 (0 = a byte every descriptor treats as background)
 
 // Per-byte class bits, pre-shifted by the byte's position within a group of 8
 // so a lane ORs eight lookups into one dword: byte 0 = structural, byte 1 =
@@ -49,6 +51,27 @@ struct DevPlan {
 	// bytes (br_bit_ok); otherwise bracket bytes are read from the input.
 	uint8_t br_bit_ok;
 	uint8_t br_chr[2][2];
+	// Event path (EventEval, plans of two or more descriptors).  Each
+	// descriptor's most common class is its background, and a run of
+	// background bytes acts as one (its map f_d is idempotent: f_d(f_d(s)) =
+	// f_d(s)).  A byte that is not background for some descriptor is an event;
+	// its event id (1..7) numbers the distinct tuples of per-descriptor classes
+	// among event bytes.  A lane then steps only at its events.
+	//   evc[j][c]: cls[j][c] | (id bit 0) << (24 + j) in [0], (id bit 1) << j |
+	//     (id bit 2) << (8 + j) in [1] (pre-shifted like cls).
+	//   ev_map[i][g][d]: descriptor d's map over event id i, after a background
+	//     run when g = 1 (trans[.][class] after f_d), packed as dt's lo/hi.
+	//   ev_joint[J][i][g]: for the joint state J = sum_d s_d ev_stride[d]: the
+	//     joint state after the step (bits 0-11), whether the run's first byte
+	//     is masked (bit 12), its later bytes (bit 13), the event byte (bit 14);
+	//     id 0 with g = 1 is a background run to the lane's end.
+	uint32_t evc[8][256][2];
+	uint32_t ev_map[kEvIds][2][kMaxDescs][2];
+	uint16_t ev_joint[kMaxJoint][kEvIds][2];
+	uint16_t ev_stride[kMaxDescs];
+	uint16_t ev_n_joint;
+	uint8_t	 ev_ok;	 // the event path applies
+	uint8_t	 ev_n_ids;
 };
 
 enum FastKind : uint8_t {

@@ -928,7 +928,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 #pragma unroll
 				for (int d = 0; d < ND; ++d) lexit[r] |= M::apply(ev[r].T[d], s0[d]) << (3 * d);
 			}
-			const uint64_t masked = ND > 0 ? ev[r].masked(s0) : 0ull;
+			const uint64_t masked = ND > 0 ? ev[r].masked(s_lds, s0) : 0ull;
 			uint64_t	   S, OC, OP;
 			ev[r].classes(S, OC, OP);
 			sm[r]  = S & ~masked;

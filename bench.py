@@ -209,6 +209,55 @@ def bench_document(torch, plan: str, d_ptr: int, n: int, steps: int, warmup: int
     return sc, r, out, sp
 
 
+# paper/plots/data/pars.csv, 512 MB rows (A100): N1a = R1-R5, N1b = R1-R6 GB/s, and tokens
+REF_FORMATS_512 = {
+    "json": (40.181, 19.997, 165941931), "csv": (141.189, 141.192, 68536716),
+    "toml": (36.215, 30.604, 52123390), "clike": (49.299, 49.500, 29020050),
+    "ini": (61.932, 61.937, 103244410), "xml": (34.601, 42.457, 89478487),
+}
+
+
+def bench_formats(torch, mib: int, reps: int) -> dict:
+    """The reference's per-format documents (bench_boundary.cu:312-354: a
+    record repeated to the target size) through the device-resident tiers:
+    N1a = R1-R5 (run_pairs off), N1b = R1-R6.  Tokens must equal the
+    reference's published count for the same document (512 MiB)."""
+    import numpy as np
+
+    from pars_amd import Plan, Scanner
+    from pars_amd import workloads as W
+
+    out = {}
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    for fmt, (ref_a, ref_b, ref_tok) in REF_FORMATS_512.items():
+        doc = W.format_doc(fmt, mib << 20)
+        d = torch.from_numpy(doc).to("cuda")
+        torch.cuda.synchronize()
+        sc = Scanner(Plan.load(fmt))
+        sc.autotune(d.data_ptr(), doc.size, stream=sp, reps=2)
+        line = {"bytes": int(doc.size)}
+        for tier, pairs in (("n1a", False), ("n1b", True)):
+            r = sc.scan_device(d.data_ptr(), doc.size, sp, want_pairs=False, run_pairs=pairs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = sc.scan_device(d.data_ptr(), doc.size, sp, want_pairs=False, run_pairs=pairs)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / reps * 1e3
+            line[tier + "_ms"] = round(ms, 4)
+            line[tier + "_gb_s"] = round(doc.size / (ms / 1e3) / 1e9, 1)
+        line["tokens"] = int(r.n_struct)
+        if mib == 512:
+            line["tokens_equal_reference"] = int(r.n_struct) == ref_tok
+        line["reference_a100_gb_s"] = {"n1a": ref_a, "n1b": ref_b}
+        out[fmt] = line
+        log(f"formats: {fmt} {line}")
+        sc.close()
+        del d
+    return out
+
+
 def bench_stream(sc, doc, stream: int, reps: int = 3) -> dict:
     """The host boundary: the same document handed over in host memory and
     streamed through pinned staging (64 MiB chunks, 8 copy threads), uploads and
@@ -543,6 +592,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch knobs (no autotuner)")
     ap.add_argument("--no-stream", action="store_true", help="skip the host-boundary (streaming) measurement")
     ap.add_argument("--no-tape", action="store_true", help="skip the JSON tape (stage-2 hook) measurement")
+    ap.add_argument("--no-formats", action="store_true",
+                    help="skip the reference's six per-format documents (512 MiB each)")
+    ap.add_argument("--formats-mib", type=int, default=512)
+    ap.add_argument("--only-formats", action="store_true", help="run only the six per-format documents")
     ap.add_argument("--tape-cpu-mib", type=int, default=256, help="CPU baseline sample of the JSON tape")
     ap.add_argument("--knobs", default="",
                     help="fixed launch knobs for the document scans instead of tuning, e.g. lb_window=16,rows=4 "
@@ -559,6 +612,13 @@ def main() -> int:
     base = None if args.no_cpu else Baseline()
     if base is not None:
         log(f"cpu baseline build: {base.flags}; host {base.host}")
+
+    if args.only_formats:
+        import torch
+
+        torch.cuda.init()
+        print(json.dumps({"formats": bench_formats(torch, args.formats_mib, max(args.steps // 2, 3))}), flush=True)
+        return 0
 
     if args.only_serve or args.only_batch:
         import torch
@@ -685,6 +745,9 @@ def main() -> int:
     torch.cuda.empty_cache()
     if not args.no_clike:
         line["clike6"] = clike_line(torch)
+    if not args.no_formats:
+        torch.cuda.empty_cache()
+        line["formats"] = bench_formats(torch, args.formats_mib, max(args.steps // 2, 3))
     if not args.no_batch:
         line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, base,
                                       tune=not args.no_tune)

@@ -408,7 +408,7 @@ hipError_t launch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
 		return launch_scan_parity(a, sc->rows, s);
 	if (sc->path == 0 && event_path_applies(sc->hplan))
-		return launch_scan_event(a, sc->hplan.n_descs, sc->hplan.wide, s);
+		return launch_scan_event(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 
@@ -493,7 +493,7 @@ hipError_t launch_batch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s)
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
 		return launch_batch_parity(a, sc->rows, s);
 	if (sc->path == 0 && event_path_applies(sc->hplan))
-		return launch_batch_event(a, sc->hplan.n_descs, sc->hplan.wide, s);
+		return launch_batch_event(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 	return launch_batch_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 

@@ -136,7 +136,7 @@ uint32_t batch_tile_rows(const DevPlan& p, int path, int rows) {
 	if (path == 0 && (p.fast_kind == kFastEscape || p.fast_kind == kFastParity))
 		return pick(rows, 4);
 	if (path == 0 && event_path_applies(p))
-		return 2u;	// launch_event's fixed shape
+		return event_tile_rows(rows);
 	if (!p.wide && p.n_descs == 1)
 		return pick(rows, 2);
 	return p.n_descs == 0 && !p.wide ? 2u : 1u;	 // launch_general's fixed shapes

@@ -140,8 +140,9 @@ hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
 // R1-R5 for plans of two to five descriptors whose background runs act as one
 // byte (DevPlan::ev_ok): the lanes step at their event bytes only.
 bool	   event_path_applies(const DevPlan& p);
-hipError_t launch_scan_event(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
-hipError_t launch_batch_event(const ScanArgs& a, int n_descs, bool wide, hipStream_t s);
+uint32_t   event_tile_rows(int rows);	// 16 KiB rows per tile of the event launchers (1 or 2)
+hipError_t launch_scan_event(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
+hipError_t launch_batch_event(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 // The same three over a batch (a.b filled in); the document heads must be set.
 hipError_t launch_batch_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 hipError_t launch_batch_escape(const ScanArgs& a, int rows, hipStream_t s);

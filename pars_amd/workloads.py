@@ -25,6 +25,34 @@ ALPHABETS = {
 }
 
 
+# The reference's per-format benchmark documents (bench/bench_boundary.cu:
+# 312-337): a record repeated until the document reaches the target size
+# (repeat_rec, :35-40); JSON is an array of one record, comma-separated
+# (:312-320).  Kept byte for byte: the reference's published token counts at
+# each size (paper/plots/data/pars.csv) then pin the scan of these documents.
+FORMAT_RECORDS = {
+    "json": b'{"id":123,"name":"x \\"y\\"","a":[1,2,3],"n":{"z":null}}',
+    "csv": b'alpha,beta,"quoted, field","has ""q""",42,true\n',
+    "toml": b"[server]\nhost = \"localhost\"\nport = 8080\nname = 'a # b'\n"
+            b"# comment { } , = with \"quotes\"\narr = [1, 2, 3]\n",
+    "clike": b"int x; /* c { ; } */ y = 1; // { ; }\n",
+    "ini": b"[section]\nkey = value\nnum = 123\n; comment\nother = x\n",
+    "xml": b'<item id="1" name="x &amp; y">alpha</item>',
+}
+
+
+def format_doc(fmt: str, target: int) -> np.ndarray:
+    """The reference's benchmark document of `fmt` for a target of `target`
+    bytes (the reference's scale_mb * 2^20)."""
+    rec = FORMAT_RECORDS[fmt]
+    if fmt == "json":  # '[' rec (',' rec)* ']': records appended while shorter than the target
+        reps = 0 if target <= 1 else 1 + max(0, -(-(target - 1 - len(rec)) // (1 + len(rec))))
+        out = b"[" + b",".join([rec] * reps) + b"]"
+    else:
+        out = rec * max(1, -(-target // len(rec)))
+    return np.frombuffer(out, dtype=np.uint8).copy()
+
+
 def _buf(n: int) -> np.ndarray:
     return np.empty(n, dtype=np.uint8)
 

@@ -109,8 +109,18 @@ typedef enum {
 	PARS_OPT_LOOKBACK_WINDOW = 6, /* predecessor records per look-back round trip: 0 = default (32), 8..64 */
 	PARS_OPT_BLOCKS_PER_CU	 = 7, /* resident scan blocks per CU (4 waves each): 0 = as many as fit */
 	PARS_OPT_RADIX_BITS		 = 8, /* pairing sort: digit width cap per pass, 0 = default (8), 1..8 */
-	PARS_OPT_CHASER			 = 9  /* one-descriptor plans: one block publishes inclusive prefixes ahead (default 0) */
+	PARS_OPT_CHASER			 = 9, /* one-descriptor plans: one block publishes inclusive prefixes ahead (default 0) */
+	PARS_OPT_LAST_PAIRING	 = 10 /* read-only (pars_hip_get_option): how R6 of the last scan_device /
+								   * scan_host / scan_batch call ran, a pars_pairing value */
 } pars_option;
+
+/* How R6 paired brackets (PARS_OPT_LAST_PAIRING). */
+typedef enum {
+	PARS_PAIRING_NONE	   = 0, /* no pairing ran */
+	PARS_PAIRING_ONE_BLOCK = 1, /* one block, bitonic sort in LDS (small inputs) */
+	PARS_PAIRING_STACK	   = 2, /* stack matching: a balanced document, or a batch of balanced documents */
+	PARS_PAIRING_SORT	   = 3	/* stable radix sort by (document,) depth */
+} pars_pairing;
 
 /* One document of a batch: its bytes are d_in[offset, offset + size). */
 typedef struct {

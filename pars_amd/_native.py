@@ -32,6 +32,8 @@ PARS_OPT_LOOKBACK_WINDOW = 6
 PARS_OPT_BLOCKS_PER_CU = 7
 PARS_OPT_RADIX_BITS = 8
 PARS_OPT_CHASER = 9
+PARS_OPT_LAST_PAIRING = 10  # read-only
+PARS_PAIRING_NONE, PARS_PAIRING_ONE_BLOCK, PARS_PAIRING_STACK, PARS_PAIRING_SORT = 0, 1, 2, 3
 
 
 class StageTimes(C.Structure):

@@ -108,6 +108,8 @@ struct pars_scanner {
 	DevBuf	  bpos, bsidx, bkey, bchar, ukeys0, ukeys1, order0, order1, rank, pair, hist, hist_part;
 	DevBuf	  stk_res, stk_rec, stk_g1, stk_g2;	 // stack pairing: chunk residues, chunk depths, group minima
 	bool	  stack_pending = false;	 // the last enqueued R6 was the stack path (sort path if unbalanced)
+	bool	  pairs_ran		 = false;	 // the last scan_device / scan_host / scan_batch call ran R6
+	bool	  stack_launched = false;	 // ... and launched the stack path first
 	bool	  keys_sparse	= false;	 // the document's keys are only at stack-chunk starts (a sort fills them)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
@@ -323,7 +325,8 @@ bool stack_first(const pars_scanner* sc, uint64_t n) {
 hipError_t enqueue_pairs(pars_scanner* sc, uint64_t n, bool want_pairs, hipStream_t s) {
 	const uint32_t small = small_pairing_mode(n, 1);
 	if (stack_first(sc, n)) {
-		sc->stack_pending = true;
+		sc->stack_pending  = true;
+		sc->stack_launched = true;
 		return launch_stack_pairing(make_stack_args(sc, want_pairs), s);
 	}
 	sc->stack_pending = false;
@@ -427,7 +430,9 @@ int enqueue_document(pars_scanner* sc, const uint8_t* d_in, uint64_t n, hipStrea
 		return hip_fail(e, "scan kernel launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
-	sc->stack_pending = false;
+	sc->stack_pending  = false;
+	sc->pairs_ran	   = run_pairs;
+	sc->stack_launched = false;
 	if (run_pairs && (e = enqueue_pairs(sc, n, want_pairs, s)) != hipSuccess)
 		return hip_fail(e, "pairing launch");
 	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
@@ -518,6 +523,8 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 		return hip_fail(e, "batch scan launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
+	sc->pairs_ran	   = run_pairs;
+	sc->stack_launched = false;
 	if (run_pairs) {
 		PairArgs p = make_pair_args(sc, want_pairs);
 		p.bdoc	   = b.bdoc;
@@ -530,6 +537,19 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 		p.doc_kbase = sc->doc_kbase.as<uint32_t>();
 		p.kb_part	= sc->kb_part.as<uint64_t>();
 		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, n_docs);
+		// every document balanced: stack matching pairs the batch and marks the
+		// summary so the sorting chain after it exits at once; otherwise the
+		// stack kernels exit and the sort pairs it (decided on the device)
+		if (p.small == 0 && !sc->radix_bits) {
+			StackArgs k = make_stack_args(sc, want_pairs);
+			k.bdoc		= b.bdoc;
+			k.br_start	= b.br_start;
+			k.st_start	= b.st_start;
+			k.doc_bad	= sc->doc_bad.as<uint32_t>();
+			if ((e = launch_stack_pairing(k, s)) != hipSuccess)
+				return hip_fail(e, "stack pairing launch");
+			sc->stack_launched = true;
+		}
 		if ((e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 	}
@@ -637,7 +657,7 @@ int batch_enqueue(pars_scanner* sc, const uint8_t* d_in, uint64_t n, const uint6
 }
 
 bool batch_failed_word(uint32_t overflow) {
-	return (overflow & 0x8000000Du) != 0;
+	return (overflow & (0x8000000Du | kStackLost)) != 0;
 }
 
 bool batch_failed(const void* h_summary) {
@@ -840,6 +860,7 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 		sc->radix_bits = (int)value;
 		return PARS_OK;
 	case PARS_OPT_CHASER: sc->chase = value != 0; return PARS_OK;
+	case PARS_OPT_LAST_PAIRING: return fail(PARS_ERR_ARGUMENT, "option %d is read-only", option);
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }
@@ -999,6 +1020,8 @@ int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const p
 	}
 	if (sc->h_sum->overflow & 4u)  // more than 2^32 distinct (document, depth) keys
 		return fail(PARS_ERR_ARGUMENT, "batch pairing needs more than 2^32 (document, depth) keys; split the batch");
+	if (sc->h_sum->overflow & kStackLost)
+		return fail(PARS_ERR_DEVICE, "stack pairing found an unmatched residue on a balanced batch (internal error)");
 	if (st) {
 		float ms = 0;
 		if (graph) {
@@ -1251,6 +1274,17 @@ int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value) {
 	case PARS_OPT_BLOCKS_PER_CU: *value = sc->blocks_per_cu; return PARS_OK;
 	case PARS_OPT_RADIX_BITS: *value = sc->radix_bits; return PARS_OK;
 	case PARS_OPT_CHASER: *value = sc->chase; return PARS_OK;
+	case PARS_OPT_LAST_PAIRING: {
+		// read from the last call's final summary: a sort that ran set its pass
+		// count; the one-block path (and a batch's stack path) set small_done
+		const DevSummary& h = *sc->h_sum;
+		*value				= !sc->pairs_ran			? PARS_PAIRING_NONE
+							  : h.sort_passes > 0		? PARS_PAIRING_SORT
+							  : sc->stack_launched		? PARS_PAIRING_STACK
+							  : h.small_done			? PARS_PAIRING_ONE_BLOCK
+														: PARS_PAIRING_NONE;
+		return PARS_OK;
+	}
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }

@@ -42,6 +42,7 @@ __global__ void k_batch_prep(BatchArgs b, DevSummary* sum, uint32_t* doc_bad, ui
 		sum->n_chunks	  = 0;
 		sum->overflow	  = 0;
 		sum->small_done	  = 0;
+		sum->doc_unbal	  = 0;
 		sum->key_total	  = 0;
 		const uint32_t e  = (sum->epoch + 1u) & 0x3FFFFFFFu;
 		sum->epoch		  = e ? e : 1u;

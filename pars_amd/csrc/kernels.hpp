@@ -43,8 +43,8 @@ struct DevSummary {
 	// (never reset itself), so records of earlier launches - also of earlier
 	// replays of one captured graph - read as unpublished.
 	uint32_t epoch;
-	uint32_t small_done;  // R6 was completed by the one-block path (k_pair_small)
-	uint32_t pad;
+	uint32_t small_done;  // R6 is complete before the sort: the one-block path or the stack path paired it
+	uint32_t doc_unbal;	  // batch: some document ends at a depth other than 0
 	// batch pairing: distinct (document, key) values, the sum over documents
 	// of (max depth - min depth + 2)
 	uint64_t key_total;
@@ -262,6 +262,14 @@ struct StackArgs {
 	unsigned long long* gmin1;	// [max_chunks / 64 + 1]
 	unsigned long long* gmin2;	// [max_chunks / 4096 + 1]
 	uint64_t		max_chunks;
+	// batch mode (null for one document): pairs and pair_pos values are
+	// document-relative, a mismatched pair marks its document in doc_bad, and
+	// the path applies when every document is balanced; it then marks the
+	// summary's small_done so the sorting chain launched after it exits at once
+	const uint32_t* bdoc;
+	const uint32_t* br_start;
+	const uint32_t* st_start;
+	uint32_t*		doc_bad;
 };
 hipError_t launch_stack_pairing(const StackArgs& a, hipStream_t s);
 

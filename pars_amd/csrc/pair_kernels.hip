@@ -178,6 +178,8 @@ __device__ __forceinline__ void store_kbase(const PairArgs& a, uint64_t d, uint6
 constexpr uint32_t kKbaseOneBlock = 16384;
 
 __global__ __launch_bounds__(1024) void k_kbase_one(PairArgs a) {
+	if (a.sum->small_done)  // a batch paired by stack matching needs no key bases
+		return;
 	__shared__ uint64_t s_w[16];
 	const uint64_t		nd	= a.n_docs;
 	uint64_t			run = 0;
@@ -201,6 +203,8 @@ __device__ __forceinline__ void kb_range(uint64_t len, uint32_t b, uint64_t& lo,
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_kbase_partial(PairArgs a) {
+	if (a.sum->small_done)  // a batch paired by stack matching needs no key bases
+		return;
 	__shared__ uint64_t s_w[4];
 	uint64_t			lo, hi;
 	kb_range(a.n_docs, blockIdx.x, lo, hi);
@@ -213,6 +217,8 @@ __global__ __launch_bounds__(kSortBlock) void k_kbase_partial(PairArgs a) {
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_kbase_top(PairArgs a) {
+	if (a.sum->small_done)  // a batch paired by stack matching needs no key bases
+		return;
 	__shared__ uint64_t s_w[4];
 	uint64_t			tot;
 	const uint64_t		v  = a.kb_part[threadIdx.x];  // kScanGrid == kSortBlock
@@ -223,6 +229,8 @@ __global__ __launch_bounds__(kSortBlock) void k_kbase_top(PairArgs a) {
 }
 
 __global__ __launch_bounds__(kSortBlock) void k_kbase_apply(PairArgs a) {
+	if (a.sum->small_done)  // a batch paired by stack matching needs no key bases
+		return;
 	__shared__ uint64_t s_w[4];
 	uint64_t			lo, hi;
 	kb_range(a.n_docs, blockIdx.x, lo, hi);

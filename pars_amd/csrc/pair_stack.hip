@@ -28,7 +28,11 @@
 //
 // An unbalanced document (min depth < 0 or final depth != 0) takes the sorting
 // path (pair_kernels.hip) instead: these kernels exit at once, and the host
-// runs that path after reading the summary.  Outputs and flags are the radix
+// runs that path after reading the summary.  A batch whose documents are all
+// balanced pairs here too (document-relative outputs, mismatches marked per
+// document); otherwise these kernels exit and the batch's sorting chain, always
+// launched after them, runs - without a host round trip, so a batch stays one
+// captured graph.  Outputs and flags are the radix
 // path's: pair[] in the CPU convention (-1 for a pair whose bytes do not
 // match, nest.hpp:80-88), pair_pos[] in the structural-index convention, and
 // the summary's mismatch flag.
@@ -55,8 +59,28 @@ __device__ __forceinline__ uint32_t stack_nb(const StackArgs& a) {
 	return (uint64_t)n > a.cap_br ? (uint32_t)a.cap_br : n;
 }
 
-__device__ __forceinline__ bool stack_applies(const DevSummary* s) {
-	return s->min_bal >= 0 && s->last_bal == 0;
+// One document: balanced.  A batch: every document balanced - then the
+// concatenated bracket stream is balanced too and returns to depth 0 at every
+// document boundary, so no stack pair crosses a document and each document's
+// pairs are those of its own depth sort.
+__device__ __forceinline__ bool stack_applies(const StackArgs& a) {
+	const DevSummary* s = a.sum;
+	return s->min_bal >= 0 && (a.bdoc ? s->doc_unbal == 0 : s->last_bal == 0);
+}
+
+// Bracket i's partner v (a bracket index, or -1 for none or a mismatch) in the
+// output conventions: pair[] by bracket index, pair_pos[] by structural index
+// (when `pos`), both document-relative in a batch.
+__device__ __forceinline__ void put_pair(const StackArgs& a, uint32_t i, int32_t v, bool pos) {
+	uint32_t b0 = 0, s0 = 0;
+	if (a.bdoc) {
+		const uint32_t d = a.bdoc[i];
+		b0				 = a.br_start[d];
+		s0				 = a.st_start[d];
+	}
+	a.pair[i] = v >= 0 ? v - (int32_t)b0 : -1;
+	if (pos && a.pair_pos)
+		a.pair_pos[a.bsidx[i]] = v >= 0 ? (int32_t)(a.bsidx[v] - s0) : -1;
 }
 
 // The j-th set bit (from 0) of m, which has more than j set bits.
@@ -108,9 +132,11 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 	__shared__ uint8_t	s_flags[256], s_cof[256];
 	__shared__ int32_t	s_wsum[kStkThreads / 64];
 	__shared__ int32_t	s_entry;
-	if (!stack_applies(a.sum))
+	if (!stack_applies(a))
 		return;
 	const int	   tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	if (a.bdoc && blockIdx.x == 0 && tid == 0)	// the batch's sorting chain, launched next, has nothing to do
+		a.sum->small_done = 1u;
 	const uint32_t nb  = stack_nb(a);
 	const uint32_t nch = (nb + kStkChunk - 1) / kStkChunk;
 	const uint32_t epoch = a.sum->epoch;
@@ -253,22 +279,16 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 				const bool	   ok = (s_flags[cl] & 1u) ? s_cof[cl] == cv : s_cof[cv] == cl;
 				bad				  = bad || !ok;
 				pv				  = ok ? (int32_t)(base + v) : -1;
+				if (!ok && a.doc_bad)
+					a.doc_bad[a.bdoc[base + l]] = 1u;
 			}
-			a.pair[base + l] = pv;
-			if (a.pair_pos && v != kPartLater)
-				a.pair_pos[a.bsidx[base + l]] = pv >= 0 ? (int32_t)a.bsidx[pv] : -1;
+			put_pair(a, base + l, pv, v != kPartLater);
 		}
 	}
 	if (__ballot(bad) && lane == 0)
 		atomicOr(&a.sum->mismatched, 1u);
 	if (__ballot(lost) && lane == 0)
 		atomicOr(&a.sum->overflow, kStackLost);
-}
-
-__device__ __forceinline__ void write_pair(const StackArgs& a, uint32_t i, int32_t v) {
-	a.pair[i] = v;
-	if (a.pair_pos)
-		a.pair_pos[a.bsidx[i]] = v >= 0 ? (int32_t)a.bsidx[v] : -1;
 }
 
 // The nearest chunk p <= cur whose minimum is below thr, or -1: one wave,
@@ -318,7 +338,7 @@ __device__ int64_t prev_below(const StackArgs& a, uint32_t epoch, int64_t cur, i
 // chunk whose minimum is below the highest level still open; its opens close
 // the levels from max(its minimum, ours) up; repeat below it.
 __global__ __launch_bounds__(kResolveBlock) void k_stack_resolve(StackArgs a) {
-	if (!stack_applies(a.sum))
+	if (!stack_applies(a))
 		return;
 	const uint32_t nb	 = stack_nb(a);
 	const uint32_t nch	 = (nb + kStkChunk - 1) / kStkChunk;
@@ -354,8 +374,10 @@ __global__ __launch_bounds__(kResolveBlock) void k_stack_resolve(StackArgs a) {
 				}
 				const bool ok = P->br_close_of[a.bchar[go]] == a.bchar[gc];
 				bad			  = bad || !ok;
-				write_pair(a, gc, ok ? (int32_t)go : -1);
-				write_pair(a, go, ok ? (int32_t)gc : -1);
+				if (!ok && a.doc_bad)
+					a.doc_bad[a.bdoc[gc]] = 1u;
+				put_pair(a, gc, ok ? (int32_t)go : -1, true);
+				put_pair(a, go, ok ? (int32_t)gc : -1, true);
 			}
 			rm	= lo;
 			cur = p - 1;

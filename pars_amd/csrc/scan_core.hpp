@@ -1051,8 +1051,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 						a.b.br_start[ldoc[r]] = rb[1] + cpre[r][1];
 					}
 					if (llast[r]) {
+						const int32_t fin	   = dl + (int32_t)cnt[r][2];
 						a.b.doc_final[ldoc[r]] = lexit[r];
-						a.b.doc_last[ldoc[r]]  = dl + (int32_t)cnt[r][2];
+						a.b.doc_last[ldoc[r]]  = fin;
+						if (fin != 0)  // the batch cannot pair by stack matching
+							atomicOr(&a.sum->doc_unbal, 1u);
 					}
 				}
 			}

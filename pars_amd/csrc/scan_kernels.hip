@@ -20,6 +20,7 @@ __global__ void k_summary_reset(DevSummary* s, SummaryInit in) {
 		s->n_chunks		= 0;
 		s->overflow		= 0;
 		s->small_done	= 0;
+		s->doc_unbal	= 0;
 		s->key_total	= 0;
 		const uint32_t e = (s->epoch + 1u) & 0x3FFFFFFFu;
 		s->epoch		 = e ? e : 1u;

@@ -8,7 +8,9 @@ Batches mix tiny documents (one byte, lane and tile boundaries) with
 documents spanning many tiles, adversarial random-alphabet documents
 (unterminated strings, unbalanced and mismatched brackets), and run on the
 fast evaluators, the general path, one- and two-descriptor plans, each tile
-height, and through the captured HIP graph on repeated calls.
+height, and through the captured HIP graph on repeated calls.  Batches whose
+documents are all balanced pair by stack matching (mismatched closing bytes
+included); one unbalanced document sends the whole batch through the sort.
 """
 from __future__ import annotations
 
@@ -183,3 +185,54 @@ def test_batch_rejects_bad_spans(torch_cuda):
         sc.scan_batch(ptr, buf.size, bad, torch.cuda.current_stream().cuda_stream)
     got = sc.scan_batch(ptr, buf.size, spans, torch.cuda.current_stream().cuda_stream)
     assert got.n_docs == 2 and got.n_struct == 4
+
+
+def balanced_docs(seed: int, count: int) -> list[np.ndarray]:
+    """Balanced JSON-grammar documents over 1 MiB in all (so the batch pairs on
+    the device-wide paths): valid JSON of log-normal sizes, bracket soups whose
+    closing bytes are sometimes swapped (balanced but not well formed), and two
+    soups deep and long enough that their residues cross stack chunks."""
+    from tests.test_gpu_stack import soup
+
+    rng = np.random.default_rng(seed)
+    docs = []
+    for i in range(count):
+        if i % 5 == 4:
+            docs.append(soup(seed * 1000 + i, int(rng.integers(1, 4000)), int(rng.integers(1, 300)), 0.01,
+                             p_text=0.3))
+        else:
+            n = int(min(max(rng.lognormal(np.log(4096), 1.0), 200), 300_000))
+            docs.append(W.make_doc("json", W.doc_seed(seed, i), n))
+    docs.append(soup(seed + 1, 40_000, 20_000, 0.0005))
+    docs.append(soup(seed + 2, 30_000, 9_000, 0.0))
+    order = rng.permutation(len(docs))
+    return [docs[k] for k in order]
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_batch_of_balanced_documents_pairs_by_stack(torch_cuda, graph):
+    docs = balanced_docs(61, 300)
+    total = sum(d.size for d in docs)
+    assert total > (1 << 20)
+    sc = Scanner("json")
+    sc.set_option(N.PARS_OPT_GRAPH, graph)
+    s = torch_cuda.cuda.Stream() if graph else None
+    for call in range(3 if graph else 1):  # graph: eager, capture, replay
+        say(f"batch stack path: {len(docs)} docs, {total} B, graph={graph} call {call}")
+        got = run_batch(torch_cuda, sc, docs, stream=s.cuda_stream if s else None)
+        assert sc.get_option(N.PARS_OPT_LAST_PAIRING) == N.PARS_PAIRING_STACK
+        check_batch("json", docs, got, f"stack graph={graph} call {call}")
+    assert any(not (f & 2) for f in got["flags"]), "the swapped closing bytes leave some document ill-formed"
+
+
+@pytest.mark.parametrize("kind", ["extra_close", "unclosed", "below_zero"])
+def test_batch_with_one_unbalanced_document_sorts(torch_cuda, kind):
+    docs = balanced_docs(62, 200)
+    bad = {"extra_close": b"[1,{}]]", "unclosed": b"[[{}", "below_zero": b"}{"}[kind]
+    docs.insert(97, np.frombuffer(bad, np.uint8).copy())
+    sc = Scanner("json")
+    say(f"batch stack fallback: {kind}")
+    got = run_batch(torch_cuda, sc, docs)
+    assert sc.get_option(N.PARS_OPT_LAST_PAIRING) == N.PARS_PAIRING_SORT
+    check_batch("json", docs, got, f"fallback {kind}")
+    assert not (got["flags"][97] & 1)

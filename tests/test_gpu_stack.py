@@ -81,6 +81,7 @@ def test_deep_nest_all_residue(torch_cuda):
     got = gpu_full(torch_cuda, sc, d, want_pairs=True)
     compare("json", d, got, "deep nest")
     assert got["max_depth"] == depth + 1 and got["well_formed"]
+    assert sc.get_option(N.PARS_OPT_LAST_PAIRING) == N.PARS_PAIRING_STACK
     sc.close()
 
 
@@ -100,6 +101,7 @@ def test_unbalanced_documents_take_the_sort(torch_cuda, kind):
     got = gpu_full(torch_cuda, sc, d, want_pairs=True)
     assert not got["balanced"]
     compare("json", d, got, kind)
+    assert sc.get_option(N.PARS_OPT_LAST_PAIRING) == N.PARS_PAIRING_SORT
     sc.close()
 
 

@@ -215,13 +215,19 @@ REF_FORMATS_512 = {
     "toml": (36.215, 30.604, 52123390), "clike": (49.299, 49.500, 29020050),
     "ini": (61.932, 61.937, 103244410), "xml": (34.601, 42.457, 89478487),
 }
+# the same rows' N2 tier (host input: H2D + R1-R6, GB/s, pars.csv:184-209), and
+# cuJSON's R1-R6 with the input transfer at 512 MB (baselines.csv:50)
+REF_N2_512 = {"json": 6.983, "csv": 11.896, "toml": 8.628, "clike": 10.245, "ini": 10.643, "xml": 10.087}
+CUJSON_N2_512 = 10.876
 
 
 def bench_formats(torch, mib: int, reps: int) -> dict:
     """The reference's per-format documents (bench_boundary.cu:312-354: a
     record repeated to the target size) through the device-resident tiers:
-    N1a = R1-R5 (run_pairs off), N1b = R1-R6.  Tokens must equal the
-    reference's published count for the same document (512 MiB)."""
+    N1a = R1-R5 (run_pairs off), N1b = R1-R6, and N2 = the same from host
+    memory (pars_hip_scan_stream: PCIe upload overlapped with the scan).
+    Tokens must equal the reference's published count for the same document
+    (512 MiB)."""
     import numpy as np
 
     from pars_amd import Plan, Scanner
@@ -250,7 +256,21 @@ def bench_formats(torch, mib: int, reps: int) -> dict:
         line["tokens"] = int(r.n_struct)
         if mib == 512:
             line["tokens_equal_reference"] = int(r.n_struct) == ref_tok
-        line["reference_a100_gb_s"] = {"n1a": ref_a, "n1b": ref_b}
+        # N2: the document handed over in host memory (H2D + R1-R6) through the
+        # streaming boundary (64 MiB pinned chunks, 8 copy threads)
+        walls = []
+        for i in range(reps + 1):
+            t0 = time.perf_counter()
+            r2 = sc.scan_stream(doc, chunk_bytes=64 << 20, copy_threads=8, stream=sp)
+            if i:
+                walls.append(time.perf_counter() - t0)
+        assert int(r2.n_struct) == int(r.n_struct)
+        w = float(np.median(walls))
+        line["n2_ms"] = round(w * 1e3, 3)
+        line["n2_gb_s"] = round(doc.size / w / 1e9, 2)
+        line["reference_a100_gb_s"] = {"n1a": ref_a, "n1b": ref_b, "n2": REF_N2_512[fmt]}
+        if fmt == "json":
+            line["cujson_a100_n2_gb_s"] = CUJSON_N2_512
         out[fmt] = line
         log(f"formats: {fmt} {line}")
         sc.close()

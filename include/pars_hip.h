@@ -151,8 +151,38 @@ typedef struct {
 	const uint64_t* struct_bitmap;	/* bit per byte of d_in (bytes outside documents are 0), or NULL */
 } pars_batch_result;
 
+/* ---- device memory: an injectable allocator and a shared budget ------------
+ * Reference: DeviceCache (pipeline.cuh:562-714) cudaMallocs its buffers and
+ * exits the process when an allocation fails (:520-555).  Here a scanner gets
+ * its buffers from the caller's allocator (default: hipMalloc / hipFree),
+ * charged against an optional budget that several scanners and servers can
+ * share; an allocation that fails or would pass the budget returns
+ * PARS_ERR_MEMORY, and the scanner stays usable for later calls. */
+typedef struct {
+	/* device memory of at least `bytes`, or NULL when none can be had; called on
+	 * the calling host thread, outside any kernel, never during graph capture
+	 * of a replayed batch (buffers are sized before capture) */
+	void* (*alloc)(void* ctx, size_t bytes);
+	/* returns what alloc gave (same pointer and size) */
+	void (*release)(void* ctx, void* p, size_t bytes);
+	void* ctx;
+} pars_allocator;
+
+typedef struct pars_budget pars_budget;
+
+/* A limit of `limit_bytes` on the device memory its users hold at once. */
+int pars_hip_budget_create(uint64_t limit_bytes, pars_budget** out);
+/* PARS_ERR_ARGUMENT while a scanner or server still uses it. */
+int pars_hip_budget_destroy(pars_budget* b);
+/* Bytes charged now, the most ever charged at once, and the limit (any may be NULL). */
+int pars_hip_budget_usage(const pars_budget* b, uint64_t* in_use, uint64_t* peak, uint64_t* limit);
+
 /* ---- scanner lifetime (reference: gpu::scanner<S> constructor/destructor) ---- */
 int pars_hip_scanner_create(const pars_plan* plan, pars_scanner** out);
+/* The same with the scanner's device memory from `alloc` (NULL: hipMalloc)
+ * and charged to `budget` (NULL: no limit). */
+int pars_hip_scanner_create_ex(const pars_plan* plan, const pars_allocator* alloc, pars_budget* budget,
+							   pars_scanner** out);
 int pars_hip_scanner_destroy(pars_scanner* sc);
 int pars_hip_set_option(pars_scanner* sc, int option, int64_t value);
 
@@ -261,6 +291,11 @@ typedef struct {
 	uint32_t		 run_pairs;		  /* run R6 (default 1) */
 	uint32_t		 submit_threads;  /* threads pars_hip_server_replay submits from (default 4) */
 	uint32_t		 reserved;
+	/* optional, shared with other scanners and servers: charged with the rings,
+	 * the scanners' buffers and, from a batch's launch until release, each
+	 * request's worst-case footprint (18 bytes per input byte); a batch waits
+	 * for budget like for hbm_budget */
+	pars_budget*	 budget;
 } pars_server_config;
 
 typedef struct {

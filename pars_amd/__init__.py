@@ -212,18 +212,89 @@ def pack_batch(docs, align: int = 64):
     return buf, spans
 
 
-class Scanner:
-    """gpu::scanner<S> (pipeline.cuh:1164-1258) for a runtime plan."""
+class Budget:
+    """A limit on the device memory its scanners and servers hold at once
+    (pars_hip_budget_*): growth past it fails with PARS_ERR_MEMORY, a server's
+    batches wait for it."""
 
-    def __init__(self, plan: Plan | str | bytes):
+    _teardown_order = 2  # after the scanners and servers that draw on it
+
+    def __init__(self, limit_bytes: int):
+        self._lib = N.lib()
+        h = C.c_void_p()
+        _check(self._lib.pars_hip_budget_create(limit_bytes, C.byref(h)))
+        self._h = h
+        N.track(self)
+
+    def usage(self) -> dict:
+        u, p, lim = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(self._lib.pars_hip_budget_usage(self._h, C.byref(u), C.byref(p), C.byref(lim)))
+        return {"in_use": u.value, "peak": p.value, "limit": lim.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _check(self._lib.pars_hip_budget_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TorchAllocator:
+    """A pars_allocator over PyTorch's caching allocator, so a scanner's
+    buffers come from (and return to) the pool the rest of the program uses.
+    `fail_after` (for tests) makes every allocation past that many return NULL."""
+
+    def __init__(self, fail_after: int | None = None):
+        import torch
+
+        self._torch = torch
+        self.fail_after = fail_after
+        self.allocs = self.releases = 0
+        self.live: dict[int, int] = {}
+        self._alloc = N.ALLOC_FN(self._on_alloc)
+        self._release = N.RELEASE_FN(self._on_release)
+        self.struct = N.Allocator(self._alloc, self._release, None)
+
+    def _on_alloc(self, ctx, n):
+        if self.fail_after is not None and self.allocs >= self.fail_after:
+            return None
+        p = self._torch.cuda.caching_allocator_alloc(int(n))
+        self.allocs += 1
+        self.live[p] = int(n)
+        return p
+
+    def _on_release(self, ctx, p, n):
+        self.releases += 1
+        assert self.live.pop(p) == int(n), "released with another size"
+        self._torch.cuda.caching_allocator_delete(p)
+
+
+class Scanner:
+    """gpu::scanner<S> (pipeline.cuh:1164-1258) for a runtime plan; device
+    memory from `allocator` (default hipMalloc), charged to `budget`."""
+
+    def __init__(self, plan: Plan | str | bytes, allocator: "TorchAllocator | None" = None,
+                 budget: Budget | None = None):
         if isinstance(plan, str):
             plan = Plan.load(plan)
         elif isinstance(plan, (bytes, bytearray)):
             plan = Plan(bytes(plan))
         self.plan = plan
         self._lib = N.lib()
+        self._allocator, self._budget = allocator, budget  # kept alive as long as the scanner
         h = C.c_void_p()
-        _check(self._lib.pars_hip_scanner_create(plan.blob, C.byref(h)))
+        if allocator is None and budget is None:
+            _check(self._lib.pars_hip_scanner_create(plan.blob, C.byref(h)))
+        else:
+            _check(self._lib.pars_hip_scanner_create_ex(
+                plan.blob, C.byref(allocator.struct) if allocator is not None else None,
+                budget._h if budget is not None else None, C.byref(h)))
         self._h = h
         N.track(self)
 
@@ -343,7 +414,7 @@ class Server:
 
     def __init__(self, plans, slots_per_plan: int = 2, ring_bytes: int = 4 << 30, staging_bytes: int = 1 << 30,
                  hbm_budget: int = 64 << 30, max_batch_bytes: int = 64 << 20, max_batch_docs: int = 16384,
-                 run_pairs: bool = True, submit_threads: int = 4):
+                 run_pairs: bool = True, submit_threads: int = 4, budget: Budget | None = None):
         import numpy as np
 
         self.plans = [Plan.load(p) if isinstance(p, str) else p for p in plans]
@@ -351,7 +422,8 @@ class Server:
         self._plan_buf = np.frombuffer(blob, dtype=np.uint8).copy()
         cfg = N.ServerConfig(self._plan_buf.ctypes.data, len(self.plans), slots_per_plan, ring_bytes,
                              staging_bytes, hbm_budget, max_batch_bytes, max_batch_docs, int(run_pairs),
-                             submit_threads, 0)
+                             submit_threads, 0, budget._h if budget is not None else None)
+        self._budget = budget
         self._lib = N.lib()
         h = C.c_void_p()
         _check(self._lib.pars_hip_server_create(C.byref(cfg), C.byref(h)))

@@ -134,6 +134,7 @@ class ServerConfig(C.Structure):
         ("run_pairs", C.c_uint32),
         ("submit_threads", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("budget", C.c_void_p),
     ]
 
 
@@ -198,8 +199,25 @@ class TuneResult(C.Structure):
         return {k: (round(getattr(self, k), 4) if t is C.c_float else getattr(self, k)) for k, t in self._fields_}
 
 
+# pars_allocator: device memory from the caller (alloc returns NULL when none is left)
+ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t)
+RELEASE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_size_t)
+
+
+class Allocator(C.Structure):
+    _fields_ = [("alloc", ALLOC_FN), ("release", RELEASE_FN), ("ctx", C.c_void_p)]
+
+
 # name -> (restype, argtypes); every symbol include/pars_hip.h declares
 PROTOTYPES = {
+    "pars_hip_budget_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "pars_hip_budget_destroy": (C.c_int, [C.c_void_p]),
+    "pars_hip_budget_usage": (
+        C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    ),
+    "pars_hip_scanner_create_ex": (
+        C.c_int, [C.c_char_p, C.POINTER(Allocator), C.c_void_p, C.POINTER(C.c_void_p)]
+    ),
     "pars_hip_scanner_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "pars_hip_scanner_destroy": (C.c_int, [C.c_void_p]),
     "pars_hip_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
@@ -304,7 +322,8 @@ def shutdown() -> None:
     no HIP call is left for the process's static destructors."""
     global _lib
     objs = list(_live)
-    objs.sort(key=lambda o: 0 if hasattr(o, "submit") else 1)
+    # servers, then scanners, then the budgets they drew on
+    objs.sort(key=lambda o: 0 if hasattr(o, "submit") else getattr(o, "_teardown_order", 1))
     for o in objs:
         try:
             o.close()

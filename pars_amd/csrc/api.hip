@@ -11,6 +11,7 @@
 #include "../../include/pars_hip.h"
 #include "internal.hpp"
 #include "kernels.hpp"
+#include "memory.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -49,31 +50,34 @@ int hip_fail(hipError_t e, const char* what) {
 	return fail(PARS_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
+// One device buffer of a scanner, from the scanner's memory source.
 struct DevBuf {
-	void*  p   = nullptr;
-	size_t cap = 0;
+	void*	   p   = nullptr;
+	size_t	   cap = 0;
+	MemSource* mem = nullptr;  // set for every buffer when the scanner is made
 
 	void release() {
 		if (p)
-			(void)hipFree(p);
+			mem->put(p, cap);
 		p	= nullptr;
 		cap = 0;
 	}
 
 	// Grow-only; `zero` clears a fresh allocation (look-back records, tickets).
+	// A failed growth leaves the buffer empty; the next call grows it again.
 	int ensure(size_t bytes, bool zero = false) {
 		if (bytes <= cap && p)
 			return PARS_OK;
+		if (!mem)
+			return fail(PARS_ERR_DEVICE, "a scanner buffer without a memory source (internal error)");
 		release();
 		const size_t want = bytes ? bytes : 64;
-		hipError_t	 e	  = hipMalloc(&p, want);
-		if (e != hipSuccess) {
-			p = nullptr;
-			return fail(PARS_ERR_MEMORY, "hipMalloc(%zu bytes): %s", want, hipGetErrorString(e));
-		}
+		int			 r	  = mem->get(&p, want);
+		if (r != PARS_OK)
+			return r;
 		cap = want;
 		if (zero) {
-			e = hipMemset(p, 0, want);
+			const hipError_t e = hipMemset(p, 0, want);
 			if (e != hipSuccess)
 				return hip_fail(e, "hipMemset");
 		}
@@ -89,6 +93,40 @@ struct DevBuf {
 }  // namespace
 
 namespace pars_hip {
+int MemSource::get(void** p, size_t bytes) {
+	*p = nullptr;
+	if (budget && !budget->charge(bytes))
+		return fail(PARS_ERR_MEMORY,
+					"device memory budget exceeded: %zu more bytes wanted, %llu of %llu in use", bytes,
+					(unsigned long long)budget->in_use.load(), (unsigned long long)budget->limit);
+	if (alloc.alloc) {
+		*p = alloc.alloc(alloc.ctx, bytes);
+		if (!*p) {
+			if (budget)
+				budget->uncharge(bytes);
+			return fail(PARS_ERR_MEMORY, "the caller's allocator returned no memory for %zu bytes", bytes);
+		}
+		return PARS_OK;
+	}
+	const hipError_t e = hipMalloc(p, bytes);
+	if (e != hipSuccess) {
+		*p = nullptr;
+		if (budget)
+			budget->uncharge(bytes);
+		return fail(PARS_ERR_MEMORY, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+	}
+	return PARS_OK;
+}
+
+void MemSource::put(void* p, size_t bytes) {
+	if (alloc.alloc)
+		alloc.release(alloc.ctx, p, bytes);
+	else
+		(void)hipFree(p);
+	if (budget)
+		budget->uncharge(bytes);
+}
+
 int set_error(int code, const std::string& msg) {
 	g_err = msg;
 	return code;
@@ -147,19 +185,36 @@ struct pars_scanner {
 		dom_parent, dom_close, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
 	DomSummary* h_dom = nullptr;  // pinned
 	size_t		h_dom_cap = 0;
+	MemSource	mem;  // every DevBuf above grows through it
+
+	// Every device buffer of the scanner.
+	template<class F>
+	void each_buf(F&& f) {
+		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
+						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &stk_res,
+						  &stk_rec, &stk_g1, &stk_g2, &d_in, &doc_tab, &tile_doc, &head_bits, &st_start, &br_start,
+						  &doc_final, &doc_min, &doc_max, &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &doc_kbase,
+						  &kb_part, &d_chunk[0], &d_chunk[1], &carry, &dom_tinfo, &dom_gbeg, &dom_gend, &dom_gdec,
+						  &dom_sub, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum, &tape_kind,
+						  &tape_beg, &tape_end, &tape_soff, &tape_arena})
+			f(*b);
+	}
+
+	pars_scanner(const pars_allocator* a, pars_budget* b) {
+		if (a)
+			mem.alloc = *a;
+		mem.budget = b;
+		if (b)
+			b->users.fetch_add(1);
+		each_buf([&](DevBuf& d) { d.mem = &mem; });
+	}
 
 	~pars_scanner() {
 		if (gexec)
 			(void)hipGraphExecDestroy(gexec);
-		for (DevBuf* b : {&stk_res, &stk_rec, &stk_g1, &stk_g2})
-			b->release();
-		for (DevBuf* b : {&dplan, &recs, &ticket, &summary, &positions, &chars, &bitmap, &pair_pos, &bpos, &bsidx,
-						  &bkey, &bchar, &ukeys0, &ukeys1, &order0, &order1, &rank, &pair, &hist, &hist_part, &d_in,
-						  &doc_tab, &tile_doc, &head_bits, &st_start, &br_start, &doc_final, &doc_min, &doc_max,
-						  &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &doc_kbase, &kb_part, &dom_tinfo, &dom_gbeg, &dom_gend,
-						  &dom_gdec, &dom_sub, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum,
-						  &tape_kind, &tape_beg, &tape_end, &tape_soff, &tape_arena})
-			b->release();
+		each_buf([](DevBuf& d) { d.release(); });
+		if (mem.budget)
+			mem.budget->users.fetch_sub(1);
 		if (h_dom)
 			(void)hipHostFree(h_dom);
 		if (h_doctab)
@@ -167,13 +222,11 @@ struct pars_scanner {
 		for (int i = 0; i < 2; ++i) {
 			if (h_chunk[i])
 				(void)hipHostFree(h_chunk[i]);
-			d_chunk[i].release();
 			if (ev_h2d[i])
 				(void)hipEventDestroy(ev_h2d[i]);
 			if (ev_done[i])
 				(void)hipEventDestroy(ev_done[i]);
 		}
-		carry.release();
 		if (copy_stream)
 			(void)hipStreamDestroy(copy_stream);
 		if (h_sum)
@@ -791,11 +844,49 @@ const char* pars_hip_last_error(void) {
 	return g_err.c_str();
 }
 
+int pars_hip_budget_create(uint64_t limit_bytes, pars_budget** out) {
+	if (!out)
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	*out = new (std::nothrow) pars_budget();
+	if (!*out)
+		return fail(PARS_ERR_MEMORY, "host allocation failed");
+	(*out)->limit = limit_bytes;
+	return PARS_OK;
+}
+
+int pars_hip_budget_destroy(pars_budget* b) {
+	if (!b)
+		return PARS_OK;
+	if (b->users.load())
+		return fail(PARS_ERR_ARGUMENT, "the budget is still used by %u scanners or servers", b->users.load());
+	delete b;
+	return PARS_OK;
+}
+
+int pars_hip_budget_usage(const pars_budget* b, uint64_t* in_use, uint64_t* peak, uint64_t* limit) {
+	if (!b)
+		return fail(PARS_ERR_ARGUMENT, "null budget");
+	if (in_use)
+		*in_use = b->in_use.load();
+	if (peak)
+		*peak = b->peak.load();
+	if (limit)
+		*limit = b->limit;
+	return PARS_OK;
+}
+
 int pars_hip_scanner_create(const pars_plan* plan, pars_scanner** out) {
+	return pars_hip_scanner_create_ex(plan, nullptr, nullptr, out);
+}
+
+int pars_hip_scanner_create_ex(const pars_plan* plan, const pars_allocator* alloc, pars_budget* budget,
+							   pars_scanner** out) {
 	if (!plan || !out)
 		return fail(PARS_ERR_ARGUMENT, "null argument");
-	*out				= nullptr;
-	pars_scanner* sc	= new (std::nothrow) pars_scanner();
+	*out = nullptr;
+	if (alloc && (!alloc->alloc || !alloc->release))
+		return fail(PARS_ERR_ARGUMENT, "an allocator needs both alloc and release");
+	pars_scanner* sc = new (std::nothrow) pars_scanner(alloc, budget);
 	if (!sc)
 		return fail(PARS_ERR_MEMORY, "host allocation failed");
 	std::memcpy(&sc->plan, plan, sizeof(pars_plan));

@@ -21,6 +21,7 @@
 // overlaps the previous batch's kernels.  No counterpart in the reference.
 #include "../../include/pars_hip.h"
 #include "internal.hpp"
+#include "memory.hpp"
 #include "ring.hpp"
 
 #include <hip/hip_runtime.h>
@@ -170,6 +171,7 @@ struct pars_server {
 	std::deque<uint64_t>		  completed;
 	uint64_t					  reserved = 0;	 // budget held by admitted, unreleased requests
 	uint64_t					  fixed	   = 0;	 // device bytes held for rings and arenas
+	uint64_t					  ring_charged = 0;	// ring bytes charged to the shared budget
 	pars_server_stats			  st {};
 	std::vector<std::thread>	  workers;	// one per plan
 	std::atomic<uint64_t>		  launch_ns {0}, retire_ns {0};
@@ -229,6 +231,12 @@ pars_server::~pars_server() {
 			(void)hipHostFree(p.stage);
 		if (p.ring)
 			(void)hipFree(p.ring);
+	}
+	if (pars_budget* b = cfg.budget) {	// rings, and requests still held by their callers
+		b->uncharge(ring_charged + reserved);
+		b->users.fetch_sub(1);
+	}
+	for (PlanState& p : plans) {
 		if (p.copy)
 			(void)hipStreamDestroy(p.copy);
 		if (p.compute)
@@ -352,7 +360,14 @@ bool pars_server::launch(uint32_t p) {
 		}
 		if (s.reqs.empty())
 			return false;
+		if (cfg.budget && !cfg.budget->charge(held)) {	// the shared budget: wait like for hbm_budget
+			st.budget_waits++;
+			s.reqs.clear();
+			return false;
+		}
 		if (!ps.dev_ring.alloc(bytes, ring_off)) {	// ring full: retry once a slot frees
+			if (cfg.budget)
+				cfg.budget->uncharge(held);
 			s.reqs.clear();
 			return false;
 		}
@@ -454,6 +469,8 @@ void pars_server::run(uint32_t p) {
 int pars_server::submit(uint32_t plan, const uint8_t* data, size_t n, uint64_t tag, uint64_t* id) {
 	if (n == 0 || n > cfg.max_batch_bytes || n > 0xFFFFFFFFull)
 		return sfail(PARS_ERR_ARGUMENT, "request size must be 1 .. max_batch_bytes");
+	if (cfg.budget && n * 18 > cfg.budget->limit)
+		return sfail(PARS_ERR_BUDGET, "the request's worst-case footprint exceeds the shared budget");
 	PlanState&	   ps = plans[plan];
 	Request*	   q  = nullptr;
 	uint64_t	   off;
@@ -542,7 +559,10 @@ void pars_server::release_locked(Request& q) {
 			}
 		}
 	}
-	reserved -= std::min(reserved, q.reserve);
+	const uint64_t back = std::min(reserved, q.reserve);
+	reserved -= back;
+	if (cfg.budget)
+		cfg.budget->uncharge(back);
 	st.released++;
 	req_free(q);
 	work_seq++;
@@ -579,6 +599,8 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 	if (!sv)
 		return sfail(PARS_ERR_MEMORY, "host allocation failed");
 	sv->cfg	 = cfg;
+	if (cfg.budget)
+		cfg.budget->users.fetch_add(1);
 	(void)hipGetDevice(&sv->dev);
 	sv->plans = std::vector<PlanState>(cfg.n_plans);  // constructed in place (PlanState does not move)
 	int r = PARS_OK;
@@ -601,9 +623,16 @@ int pars_hip_server_create(const pars_server_config* cfg_in, pars_server** out) 
 		ps.stage_ring.cap = stage_each;
 		ps.dev_ring.cap	  = ring_each;
 		sv->fixed += ring_each;
+		if (cfg.budget) {
+			if (!cfg.budget->charge(ring_each)) {
+				r = sfail(PARS_ERR_BUDGET, "the server's device ring exceeds the shared budget");
+				break;
+			}
+			sv->ring_charged += ring_each;
+		}
 		ps.slots.resize(cfg.slots_per_plan);
 		for (Slot& s : ps.slots) {
-			if ((r = pars_hip_scanner_create(&cfg.plans[p], &s.sc)))
+			if ((r = pars_hip_scanner_create_ex(&cfg.plans[p], nullptr, cfg.budget, &s.sc)))
 				break;
 			if ((r = pars_hip_set_option(s.sc, PARS_OPT_WANT_BITMAP, 0)))
 				break;

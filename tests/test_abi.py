@@ -133,3 +133,27 @@ def test_no_alloc_limits_refused_before_the_device():
     rc = lib.pars_hip_scan_device_noalloc(dummy, dummy, 1024, None, 1, dummy, 1024, None, dummy, dummy, dummy,
                                           (1 << 20) + 1, C.byref(r))
     assert rc == 1 and b"brackets" in lib.pars_hip_last_error()
+
+
+def test_budget_accounting_without_a_device():
+    """pars_hip_budget_*: host-only objects (no HIP call)."""
+    lib = N.lib()
+    h = C.c_void_p()
+    assert lib.pars_hip_budget_create(1 << 20, C.byref(h)) == N.PARS_OK and h.value
+    u, p, lim = C.c_uint64(7), C.c_uint64(7), C.c_uint64(0)
+    assert lib.pars_hip_budget_usage(h, C.byref(u), C.byref(p), C.byref(lim)) == N.PARS_OK
+    assert (u.value, p.value, lim.value) == (0, 0, 1 << 20)
+    assert lib.pars_hip_budget_usage(h, None, None, None) == N.PARS_OK  # every output optional
+    assert lib.pars_hip_budget_usage(None, C.byref(u), None, None) == 1  # PARS_ERR_ARGUMENT
+    assert lib.pars_hip_budget_create(5, None) == 1
+    assert lib.pars_hip_budget_destroy(h) == N.PARS_OK
+    assert lib.pars_hip_budget_destroy(None) == N.PARS_OK
+
+
+def test_allocator_needs_both_callbacks():
+    lib = N.lib()
+    a = N.Allocator(N.ALLOC_FN(lambda ctx, n: None), N.RELEASE_FN(), None)  # release missing
+    out = C.c_void_p()
+    assert lib.pars_hip_scanner_create_ex(Plan.load("json").blob, C.byref(a), None, C.byref(out)) == 1
+    assert not out.value
+    assert b"allocator" in lib.pars_hip_last_error()

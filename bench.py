@@ -154,7 +154,8 @@ def bench_document(torch, plan: str, d_ptr: int, n: int, steps: int, warmup: int
     tuned = None
     if knobs:
         opts = {"rows": N.PARS_OPT_ROWS_PER_TILE, "lb_window": N.PARS_OPT_LOOKBACK_WINDOW,
-                "blocks_per_cu": N.PARS_OPT_BLOCKS_PER_CU, "radix_bits": N.PARS_OPT_RADIX_BITS}
+                "blocks_per_cu": N.PARS_OPT_BLOCKS_PER_CU, "radix_bits": N.PARS_OPT_RADIX_BITS,
+                "waves": N.PARS_OPT_WAVES_PER_BLOCK}
         tuned = {}
         for kv in knobs.split(","):
             k, v = kv.split("=")
@@ -184,6 +185,9 @@ def bench_document(torch, plan: str, d_ptr: int, n: int, steps: int, warmup: int
     b_alg = n + n / 8 + 4 * k + 4 * kb
     t_scan = float(np.mean(scan_ms)) / 1e3
     achieved = b_alg / t_scan / 1e9
+    # the kernel as launched: rows per tile and waves per block as set or tuned
+    kernel = kernel.format(rows=sc.get_option(N.PARS_OPT_ROWS_PER_TILE) or 4,
+                           waves=sc.get_option(N.PARS_OPT_WAVES_PER_BLOCK) or 4)
     out = {
         "ms_per_step": ms_step,
         "value": n / (ms_step / 1e3) / 1e9,
@@ -619,7 +623,7 @@ def main() -> int:
     ap.add_argument("--tape-cpu-mib", type=int, default=256, help="CPU baseline sample of the JSON tape")
     ap.add_argument("--knobs", default="",
                     help="fixed launch knobs for the document scans instead of tuning, e.g. lb_window=16,rows=4 "
-                         "(rows, lb_window, blocks_per_cu, radix_bits); profiles use this")
+                         "(rows, lb_window, blocks_per_cu, radix_bits, waves); profiles use this")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -663,7 +667,7 @@ def main() -> int:
         torch.cuda.synchronize()
         sc3, r3, m3, _ = bench_document(torch, "clike6", d3.data_ptr(), n3, max(args.steps // 2, 3), args.warmup,
                                         args.knobs, not args.no_tune, "k_scan_clike6_1gib.md",
-                                        "k_scan<EventEval<5, true>, 2> (R1-R5)")
+                                        "k_scan<EventEval<5, true>, 2, false, 4> (R1-R5)")
         line3 = {
             "workload": f"clike6-{n3 >> 20}MiB plain pipeline (R1-R6, device-resident, N1b; six modes, 5 descriptors)",
             "bytes": n3,
@@ -720,7 +724,7 @@ def main() -> int:
         log(f"copy calibration: 2 device copies of {n} bytes")
 
     sc, r, m, sp = bench_document(torch, "json", d_doc.data_ptr(), n, args.steps, args.warmup, args.knobs,
-                                  not args.no_tune, "k_scan_json_2gib.md", "k_scan<EscapeEval, 4> (R1-R5)")
+                                  not args.no_tune, "k_scan_json_2gib.md", "k_scan<EscapeEval, {rows}, false, {waves}> (R1-R5)")
     line = {
         "metric": METRIC,
         "value": round(m["value"], 3),

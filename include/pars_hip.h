@@ -110,8 +110,12 @@ typedef enum {
 	PARS_OPT_BLOCKS_PER_CU	 = 7, /* resident scan blocks per CU (4 waves each): 0 = as many as fit */
 	PARS_OPT_RADIX_BITS		 = 8, /* pairing sort: digit width cap per pass, 0 = default (8), 1..8 */
 	PARS_OPT_CHASER			 = 9, /* one-descriptor plans: one block publishes inclusive prefixes ahead (default 0) */
-	PARS_OPT_LAST_PAIRING	 = 10 /* read-only (pars_hip_get_option): how R6 of the last scan_device /
-								   * scan_host / scan_batch call ran, a pars_pairing value */
+	PARS_OPT_LAST_PAIRING	 = 10, /* read-only (pars_hip_get_option): how R6 of the last scan_device /
+									* scan_host / scan_batch call ran, a pars_pairing value */
+	PARS_OPT_WAVES_PER_BLOCK = 11  /* scan blocks of one document under the bit-parallel evaluators
+									* (JSON- and CSV-style plans): 0 or 4 (default), 6, 8 waves; the
+									* row grows with the block (64 bytes per lane); other paths and
+									* batches keep four */
 } pars_option;
 
 /* How R6 paired brackets (PARS_OPT_LAST_PAIRING). */
@@ -367,8 +371,8 @@ int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value);
  * Times the scanner on the caller's input - one document, or a batch when docs
  * is non-NULL - over its launch knobs and keeps the fastest setting on the
  * scanner: rows per tile (bytes per lane and tile = 64 x rows), look-back
- * window, resident blocks per CU (waves per SIMD), and the pairing sort's
- * radix digit cap.  Coordinate descent from the current setting; a candidate
+ * window, resident blocks per CU (waves per SIMD), waves per block (one
+ * document, bit-parallel evaluators), and the pairing sort's radix digit cap.  Coordinate descent from the current setting; a candidate
  * must beat the best so far by 2 %; each is the median of `reps` runs (1..15)
  * after a warm-up.  Overwrites the scanner's outputs; blocks. */
 typedef struct {
@@ -377,6 +381,7 @@ typedef struct {
 	float	 base_scan_ms, base_pair_ms;				  /* with the setting before tuning */
 	uint32_t trials;									  /* runs timed */
 	uint32_t max_blocks_per_cu;							  /* the scan kernel's occupancy limit */
+	int32_t	 waves_per_block;							  /* chosen option value (0 = default) */
 } pars_tune_result;
 
 int pars_hip_autotune(

@@ -33,6 +33,7 @@ PARS_OPT_BLOCKS_PER_CU = 7
 PARS_OPT_RADIX_BITS = 8
 PARS_OPT_CHASER = 9
 PARS_OPT_LAST_PAIRING = 10  # read-only
+PARS_OPT_WAVES_PER_BLOCK = 11
 PARS_PAIRING_NONE, PARS_PAIRING_ONE_BLOCK, PARS_PAIRING_STACK, PARS_PAIRING_SORT = 0, 1, 2, 3
 
 
@@ -193,6 +194,7 @@ class TuneResult(C.Structure):
         ("base_pair_ms", C.c_float),
         ("trials", C.c_uint32),
         ("max_blocks_per_cu", C.c_uint32),
+        ("waves_per_block", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -23,7 +23,7 @@ OBJ = os.path.join(HERE, "_obj")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 HIP_SOURCES = [
-    "scan_kernels.hip", "batch_kernels.hip", "event_kernels.hip", "event_batch_kernels.hip",
+    "scan_kernels.hip", "scan_wide_kernels.hip", "batch_kernels.hip", "event_kernels.hip", "event_batch_kernels.hip",
     "pair_kernels.hip", "pair_stack.hip",
     "dom_kernels.hip", "api.hip", "server.hip",
 ]

@@ -151,7 +151,7 @@ struct pars_scanner {
 	bool	  keys_sparse	= false;	 // the document's keys are only at stack-chunk starts (a sort fills them)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
-	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0;  // tuning; 0 = default
+	int		  lb_window = 0, blocks_per_cu = 0, radix_bits = 0, waves = 0;	// tuning; 0 = default
 	int		  chase		= 0;
 	DevSummary* h_sum	 = nullptr;	 // pinned
 	hipEvent_t	ev[3]	 = {nullptr, nullptr, nullptr};
@@ -460,9 +460,9 @@ void fill_result(pars_scanner* sc, uint64_t n, bool want_pairs, bool run_pairs, 
 // R1-R5 through the fastest evaluator that is exact for the plan.
 hipError_t launch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s) {
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastEscape)
-		return launch_scan_escape(a, sc->rows, s);
+		return launch_scan_escape(a, sc->rows, sc->waves, s);
 	if (sc->path == 0 && sc->hplan.fast_kind == kFastParity)
-		return launch_scan_parity(a, sc->rows, s);
+		return launch_scan_parity(a, sc->rows, sc->waves, s);
 	if (sc->path == 0 && event_path_applies(sc->hplan))
 		return launch_scan_event(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 	return launch_scan_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
@@ -952,6 +952,11 @@ int pars_hip_set_option(pars_scanner* sc, int option, int64_t value) {
 		return PARS_OK;
 	case PARS_OPT_CHASER: sc->chase = value != 0; return PARS_OK;
 	case PARS_OPT_LAST_PAIRING: return fail(PARS_ERR_ARGUMENT, "option %d is read-only", option);
+	case PARS_OPT_WAVES_PER_BLOCK:
+		if (value != 0 && value != 4 && value != 6 && value != 8)
+			return fail(PARS_ERR_ARGUMENT, "waves per block must be 0 (default 4), 4, 6 or 8");
+		sc->waves = (int)value;
+		return PARS_OK;
 	}
 	return fail(PARS_ERR_ARGUMENT, "unknown option %d", option);
 }
@@ -1261,6 +1266,8 @@ int pars_hip_scan_chunk(pars_scanner* sc, const uint8_t* d_chunk, size_t n, cons
 						pars_stream stream, size_t byte_base, pars_chunk_carry* out) {
 	if (!sc || !in || !out || (n && !d_chunk))
 		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if ((uint64_t)n > 0xFFFFFFFFull || (uint64_t)byte_base + n > 0xFFFFFFFFull)
+		return fail(PARS_ERR_ARGUMENT, "chunks of a document end at or below 2^32-1 bytes (uint32 positions)");
 	hipStream_t		 s	 = (hipStream_t)stream;
 	pars_chunk_carry c	 = *in;
 	const bool		 pad = (c.scan >> 31) & 1u;
@@ -1365,6 +1372,7 @@ int pars_hip_get_option(const pars_scanner* sc, int option, int64_t* value) {
 	case PARS_OPT_BLOCKS_PER_CU: *value = sc->blocks_per_cu; return PARS_OK;
 	case PARS_OPT_RADIX_BITS: *value = sc->radix_bits; return PARS_OK;
 	case PARS_OPT_CHASER: *value = sc->chase; return PARS_OK;
+	case PARS_OPT_WAVES_PER_BLOCK: *value = sc->waves; return PARS_OK;
 	case PARS_OPT_LAST_PAIRING: {
 		// read from the last call's final summary: a sort that ran set its pass
 		// count; the one-block path (and a batch's stack path) set small_done
@@ -1462,6 +1470,11 @@ int pars_hip_autotune(pars_scanner* sc, const uint8_t* d_in, size_t n, const par
 		for (int b = 1; b < occ; ++b) bpc.push_back(b);
 		sweep(sc->blocks_per_cu, bpc, false);
 	}
+	// waves per block: one document through a bit-parallel evaluator (batches
+	// and the descriptor walks keep four); the launcher caps blocks per CU at
+	// what the wider blocks can keep resident
+	if (!docs && sc->path == 0 && sc->hplan.fast_kind != kFastNone)
+		sweep(sc->waves, {6, 8}, false);
 	if (n_brackets)
 		sweep(sc->radix_bits, {8, 7, 6, 5, 4}, true);
 	if ((r = measure(sm, pm)) == PARS_OK) {
@@ -1473,6 +1486,7 @@ int pars_hip_autotune(pars_scanner* sc, const uint8_t* d_in, size_t n, const par
 	out->lb_window		= sc->lb_window;
 	out->blocks_per_cu	= sc->blocks_per_cu;
 	out->radix_bits		= sc->radix_bits;
+	out->waves_per_block = sc->waves;
 	return r;
 }
 

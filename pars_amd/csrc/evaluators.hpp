@@ -100,8 +100,8 @@ struct GeneralEval {
 	};
 
 	__device__ static void load(Lds& L, const DevPlan* P) {
-		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) L.cls[i] = (&P->cls[0][0])[i];
-		for (int i = threadIdx.x; i < ND * 256; i += kBlock) {
+		for (int i = threadIdx.x; i < 8 * 256; i += (int)blockDim.x) L.cls[i] = (&P->cls[0][0])[i];
+		for (int i = threadIdx.x; i < ND * 256; i += (int)blockDim.x) {
 			const uint32_t* e = P->dt[i >> 8][i & 255];
 			if constexpr (WIDE)
 				L.dt[i] = make_uint4(e[0], e[1], e[2], e[3]);
@@ -272,7 +272,7 @@ struct EscapeEval {
 	};
 
 	__device__ static void load(Lds& L, const DevPlan* P) {
-		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) {
+		for (int i = threadIdx.x; i < 8 * 256; i += (int)blockDim.x) {
 			const uint32_t* e = P->cls8[i >> 8][i & 255];
 			L.cls[i]		  = make_uint2(e[0], e[1]);
 		}
@@ -460,16 +460,16 @@ struct EventEval {
 	};
 
 	__device__ static void load(Lds& L, const DevPlan* P) {
-		for (int i = threadIdx.x; i < 8 * 256; i += kBlock) {
+		for (int i = threadIdx.x; i < 8 * 256; i += (int)blockDim.x) {
 			const uint32_t* e = P->evc[i >> 8][i & 255];
 			L.cls[i]		  = make_uint2(e[0], e[1]);
 		}
-		for (int i = threadIdx.x; i < kEvIds * 2 * ND; i += kBlock) {
+		for (int i = threadIdx.x; i < kEvIds * 2 * ND; i += (int)blockDim.x) {
 			const int d = i % ND, g = (i / ND) & 1, id = i / (2 * ND);
 			L.map[i]	= make_uint2(P->ev_map[id][g][d][0], P->ev_map[id][g][d][1]);
 		}
 		const int nj = (int)P->ev_n_joint * kEvIds * 2;
-		for (int i = threadIdx.x; i < nj; i += kBlock) L.joint[i] = (&P->ev_joint[0][0][0])[i];
+		for (int i = threadIdx.x; i < nj; i += (int)blockDim.x) L.joint[i] = (&P->ev_joint[0][0][0])[i];
 		if (threadIdx.x < ND)
 			L.stride[threadIdx.x] = P->ev_stride[threadIdx.x];
 	}
@@ -570,16 +570,17 @@ struct EventEval {
 // ---------------------------------------------------------------------------
 // Persistent launch: as many blocks as fit on the device at once (never more
 // than there are tiles); each draws tiles until none is left.
-template<class E, int W, bool BATCH>
+template<class E, int W, bool BATCH, int NWV = kBlock / kWave>
 hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
-	static int cus = 0, per_cu = 0;	 // per kernel instance
+	constexpr int NT  = NWV * kWave;  // threads per block
+	static int	  cus = 0, per_cu = 0;	// per kernel instance
 	if (!cus) {
 		int		   dev = 0, c = 0, p = 0;
 		hipError_t e = hipGetDevice(&dev);
 		if (e == hipSuccess)
 			e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
 		if (e == hipSuccess)
-			e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_scan<E, W, BATCH>, kBlock, 0);
+			e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_scan<E, W, BATCH, NWV>, NT, 0);
 		if (e != hipSuccess)
 			return e;
 		per_cu = p > 0 ? p : 1;
@@ -588,7 +589,7 @@ hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
 	set_last_scan_occupancy(per_cu);
 	const int resident = cus * (in.blocks_per_cu > 0 && (int)in.blocks_per_cu < per_cu ? (int)in.blocks_per_cu : per_cu);
 	ScanArgs  a		   = in;
-	const uint64_t tbytes = (uint64_t)W * kTile;
+	const uint64_t tbytes = (uint64_t)W * NT * kBPL;
 	a.n_tiles			  = (uint32_t)((a.n + tbytes - 1) / tbytes);
 	if (a.n_tiles == 0)
 		return hipSuccess;
@@ -602,7 +603,7 @@ hipError_t launch_eval(const ScanArgs& in, hipStream_t s) {
 	const uint32_t grid = a.n_tiles < (uint32_t)resident ? a.n_tiles : (uint32_t)resident;
 	// a chaser pays off once tiles outnumber the resident blocks several times
 	a.chase = E::kND == 1 && !BATCH && in.chase != 0 && grid >= 2 && a.n_tiles >= 4 * grid;
-	hipLaunchKernelGGL((k_scan<E, W, BATCH>), dim3(grid), dim3(kBlock), 0, s, a);
+	hipLaunchKernelGGL((k_scan<E, W, BATCH, NWV>), dim3(grid), dim3(NT), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -616,6 +617,21 @@ hipError_t launch_rows(const ScanArgs& a, int rows, int def, hipStream_t s) {
 	case 4: return launch_eval<E, 4, BATCH>(a, s);
 	default: return launch_eval<E, 2, BATCH>(a, s);
 	}
+}
+
+// One document through a bit-parallel evaluator with `waves` waves per block
+// (0 or 4: the four-wave block; 6 or 8: rows of 24 or 32 KiB - the same tile
+// geometry scaled, two blocks of six waves or one of eight per CU at the
+// evaluators' register budget).  Rows per tile as launch_rows; the wider
+// blocks take two or four rows.
+template<class E>
+hipError_t launch_waves(const ScanArgs& a, int rows, int def, int waves, hipStream_t s) {
+	const int w = rows ? rows : def;
+	if (waves == 6)
+		return w >= 3 ? launch_eval<E, 4, false, 6>(a, s) : launch_eval<E, 2, false, 6>(a, s);
+	if (waves == 8)
+		return w >= 3 ? launch_eval<E, 4, false, 8>(a, s) : launch_eval<E, 2, false, 8>(a, s);
+	return launch_rows<E>(a, rows, def, s);
 }
 
 template<bool BATCH>

@@ -134,9 +134,12 @@ int last_scan_occupancy();
 // rows per tile (1..4; 0 = the default); tuned per plan.
 hipError_t launch_scan_general(const ScanArgs& a, int n_descs, bool wide, int rows, hipStream_t s);
 // R1-R5 for plans whose one descriptor is an affine escape automaton (JSON).
-hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s);
+hipError_t launch_scan_escape(const ScanArgs& a, int rows, int waves, hipStream_t s);
 // R1-R5 for plans whose one descriptor is quote parity (CSV doubled quotes).
-hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s);
+hipError_t launch_scan_parity(const ScanArgs& a, int rows, int waves, hipStream_t s);
+// The escape / parity evaluators with six- or eight-wave blocks (waves 6, 8;
+// one document): scan_wide_kernels.hip.
+hipError_t launch_scan_wide(const ScanArgs& a, bool parity, int rows, int waves, hipStream_t s);
 // R1-R5 for plans of two to five descriptors whose background runs act as one
 // byte (DevPlan::ev_ok): the lanes step at their event bytes only.
 bool	   event_path_applies(const DevPlan& p);

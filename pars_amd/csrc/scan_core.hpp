@@ -543,37 +543,39 @@ constexpr int kCapB = (!BATCH && W >= 3) ? 2048 : 1536;  // staged bracket outpu
 template<class E, int W>
 constexpr int kMinWaves = (E::kND >= 3 && E::kWide) ? 2 : (E::kBitParallel && W >= 3) ? 3 : 1;
 
-template<class E, int W, bool BATCH>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWaves<E, W>))) void k_scan(ScanArgs a) {
+template<class E, int W, bool BATCH, int NWV = kBlock / kWave>
+__global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMinWaves<E, W>))) void k_scan(ScanArgs a) {
+	constexpr int	   NT	= NWV * kWave;	// threads per block
+	constexpr uint32_t RowB = NT * kBPL;	// bytes per row
 	constexpr int  ND	 = E::kND;
 	constexpr int  NDA	 = ND > 0 ? ND : 1;
 	constexpr bool FUSED = ND == 1;
 	constexpr int  NS	 = E::kWide ? 8 : 4;  // entry states a one-descriptor tile is summarised for
 	constexpr int  NPK	 = NS / 4;			  // count dwords per field, four states each
-	constexpr int  NL	 = W * kBlock;		  // lanes per tile
+	constexpr int  NL	 = W * NT;			  // lanes per tile
 	constexpr int  NHW	 = NL / 64;			  // head-bitmap words per tile
 	using M				 = MapT<E::kWide>;
 	using Map			 = typename M::type;
 	using KeyT			 = typename std::conditional<BATCH, int32_t, int16_t>::type;
-	constexpr uint64_t kTileBytes = (uint64_t)W * kRowBytes;
-	constexpr int	   CAPP		  = kCapP<W, BATCH>;
-	constexpr int	   CAPB		  = kCapB<W, BATCH>;
+	constexpr uint64_t kTileBytes = (uint64_t)W * RowB;
+	constexpr int	   CAPP		  = kCapP<W, BATCH> * NWV / 4;	// staging scales with the row
+	constexpr int	   CAPB		  = kCapB<W, BATCH> * NWV / 4;
 
 	__shared__ typename E::Lds s_lds;
-	__shared__ Map			   s_wagg[W][4][NDA];
+	__shared__ Map			   s_wagg[W][NWV][NDA];
 	__shared__ uint32_t		   s_misc[8];
-	__shared__ uint32_t		   s_wcnt[W][4][3];
-	__shared__ uint32_t		   s_vsum[4][3][2 * NPK];  // one-chain: per-wave 16-bit-pair count sums
-	__shared__ int32_t		   s_vpost[4];			   // one-chain batch: per-wave depth after the last head
+	__shared__ uint32_t		   s_wcnt[W][NWV][3];
+	__shared__ uint32_t		   s_vsum[NWV][3][2 * NPK];	// one-chain: per-wave 16-bit-pair count sums
+	__shared__ int32_t		   s_vpost[NWV];		   // one-chain batch: per-wave depth after the last head
 	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
-	__shared__ int32_t		   s_mm[4][2];
+	__shared__ int32_t		   s_mm[NWV][2];
 	__shared__ uint16_t		   s_pos[CAPP];	 // row-local byte offset
 	__shared__ uint16_t		   s_bpos[CAPB];	 // row-local byte offset
 	__shared__ uint16_t		   s_bsidx[CAPB];	 // row-local structural index
 	__shared__ KeyT			   s_bkey[CAPB];	 // key: relative to the row's entry depth / document-relative
 	// bit-parallel evaluators: each lane's bracket ordinal bits per row (from the
 	// walk) and the staged bracket bytes rebuilt from them
-	__shared__ uint64_t		   s_bo[E::kBitParallel ? W : 1][E::kBitParallel ? kBlock : 1];
+	__shared__ uint64_t		   s_bo[E::kBitParallel ? W : 1][E::kBitParallel ? NT : 1];
 	__shared__ uint8_t		   s_bch[E::kBitParallel ? CAPB : 1];
 	__shared__ uint64_t		   s_hw[BATCH ? NHW : 1];		// the tile's head bits
 	__shared__ uint32_t		   s_hpre[BATCH ? NHW : 1];		// heads in the words before
@@ -656,8 +658,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					last_head = q * 64 + 63 - (int)__clzll((long long)s_hw[q]);
 #pragma unroll
 			for (int r = 0; r < W; ++r) {
-				const int	   L	  = r * kBlock + tid;
-				const uint64_t lstart = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
+				const int	   L	  = r * NT + tid;
+				const uint64_t lstart = tbase + (uint64_t)r * RowB + (uint64_t)tid * kBPL;
 				lhead[r]			  = (s_hw[L >> 6] >> (L & 63)) & 1ull;
 				ldoc[r]				  = d0 + (int32_t)slot_of(L);
 				vdoc[r]				  = 0;
@@ -681,7 +683,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		uint32_t valid[W];
 		uint32_t xb[W][16];
 		auto	 load_row = [&](int r, uint32_t (&x)[16]) {
-			const uint64_t base = tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL;
+			const uint64_t base = tbase + (uint64_t)r * RowB + (uint64_t)tid * kBPL;
 			valid[r]			= base >= a.n ? 0u : (uint32_t)min<uint64_t>((uint64_t)kBPL, a.n - base);
 			if (valid[r] == kBPL && aligned) {
 				const uint4* p = reinterpret_cast<const uint4*>(a.data + base);
@@ -748,7 +750,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				for (int r = 0; r < W; ++r) {
 					Map wp = M::id();
 #pragma unroll
-					for (int w2 = 0; w2 < 4; ++w2) {
+					for (int w2 = 0; w2 < NWV; ++w2) {
 						if (w2 == wid)
 							lpre[r][d] = M::compose(exc[r][d], wp);
 						wp = M::compose(s_wagg[r][w2][d], wp);
@@ -811,7 +813,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 							acc16[f][2 * h + 1] += (g[h] >> 8) & 0x00FF00FFu;
 						}
 						if constexpr (BATCH)
-							if (f == 2 && r * kBlock + tid >= last_head && last_head >= 0)
+							if (f == 2 && r * NT + tid >= last_head && last_head >= 0)
 								dpost += (int32_t)(g[0] & 0xFFu) - 64;	// entry-independent after a head
 					}
 					rowpre = M::compose(ragg[r][0], rowpre);
@@ -831,15 +833,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				}
 				__syncthreads();
 				if constexpr (BATCH)
-					post = (uint32_t)(s_vpost[0] + s_vpost[1] + s_vpost[2] + s_vpost[3]);
+					for (int w2 = 0; w2 < NWV; ++w2) post += (uint32_t)s_vpost[w2];
 				if (tid < 3 * NS) {	 // thread (f, e): field e of the four wave sums
 					const int f = tid / NS, e = tid % NS;
 					const int q = 2 * (e >> 2) + (e & 1), sh = (e & 2) ? 16 : 0;
 					uint32_t  t = 0;
 #pragma unroll
-					for (int w2 = 0; w2 < 4; ++w2) t += (s_vsum[w2][f][q] >> sh) & 0xFFFFu;
+					for (int w2 = 0; w2 < NWV; ++w2) t += (s_vsum[w2][f][q] >> sh) & 0xFFFFu;
 					if (f == 2)
-						t -= 64u * (uint32_t)(kBlock * W);
+						t -= 64u * (uint32_t)(NT * W);
 					if (BATCH && f == 2 && last_head >= 0)
 						t = post;
 					s_v[f][e] = t;
@@ -934,8 +936,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 			sm[r]  = S & ~masked;
 			om[r]  = OC & ~masked;
 			opm[r] = OP & ~masked;
-			if (a.bitmap && (tbase + (uint64_t)r * kRowBytes + (uint64_t)tid * kBPL) < a.n)
-				a.bitmap[(tbase + (uint64_t)r * kRowBytes) / 64 + tid] = sm[r];
+			if (a.bitmap && (tbase + (uint64_t)r * RowB + (uint64_t)tid * kBPL) < a.n)
+				a.bitmap[(tbase + (uint64_t)r * RowB) / 64 + tid] = sm[r];
 			cnt[r][0] = popc64(sm[r]);
 			cnt[r][1] = popc64(om[r]);
 			cnt[r][2] = 2u * popc64(opm[r]) - cnt[r][1];
@@ -958,7 +960,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				for (int f = 0; f < 3; ++f) {
 					uint32_t p = 0, t = 0;
 #pragma unroll
-					for (int w2 = 0; w2 < 4; ++w2) {
+					for (int w2 = 0; w2 < NWV; ++w2) {
 						const uint32_t v = s_wcnt[r][w2][f];
 						if (w2 < wid)
 							p += v;
@@ -1025,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 			}
 		}
 		if (a.pair_pos)	 // -1 for every structural index of the tile; R6 overwrites brackets
-			for (uint32_t i = tid; i < ctot[0]; i += kBlock)
+			for (uint32_t i = tid; i < ctot[0]; i += NT)
 				if ((uint64_t)cex[0] + i < a.cap_struct)
 					a.pair_pos[cex[0] + i] = -1;
 
@@ -1037,7 +1039,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		int32_t	 lmin = INT_MAX, lmax = INT_MIN;
 #pragma unroll
 		for (int r = 0; r < W; ++r) {
-			const uint64_t rowb	 = tbase + (uint64_t)r * kRowBytes;	 // data index of the row
+			const uint64_t rowb	 = tbase + (uint64_t)r * RowB;	 // data index of the row
 			const uint32_t loff	 = (uint32_t)tid * kBPL;			 // lane's offset in the row
 			const uint64_t pbase = a.byte_base + rowb;
 			int32_t		   dl	 = rowd + (int32_t)cpre[r][2];	// the lane's entry depth
@@ -1108,21 +1110,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 				constexpr int U = 4;
 				auto		  out_pos = [&](uint32_t off) -> uint32_t {
 					 if constexpr (BATCH)
-						 return s_lpos0[r * kBlock + (off >> 6)] + (off & 63u);
+						 return s_lpos0[r * NT + (off >> 6)] + (off & 63u);
 					 else
 						 return (uint32_t)(pbase + off);
 				};
-				for (uint32_t i0 = 0; i0 < mp; i0 += U * kBlock) {
+				for (uint32_t i0 = 0; i0 < mp; i0 += U * NT) {
 					uint8_t ch[U];
 					if (a.chars)
 #pragma unroll
 						for (int u = 0; u < U; ++u) {
-							const uint32_t i = i0 + u * kBlock + tid;
+							const uint32_t i = i0 + u * NT + tid;
 							ch[u]			 = i < mp ? a.data[rowb + s_pos[i]] : 0;
 						}
 #pragma unroll
 					for (int u = 0; u < U; ++u) {
-						const uint32_t i = i0 + u * kBlock + tid;
+						const uint32_t i = i0 + u * NT + tid;
 						if (i < mp) {
 							a.positions[rb[0] + i] = out_pos(s_pos[i]);
 							if (a.chars)
@@ -1131,11 +1133,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					}
 				}
 				const int32_t kb = BATCH ? 0 : (int32_t)rb[2];
-				for (uint32_t i0 = 0; i0 < mb; i0 += U * kBlock) {
+				for (uint32_t i0 = 0; i0 < mb; i0 += U * NT) {
 					uint8_t ch[U];
 #pragma unroll
 					for (int u = 0; u < U; ++u) {
-						const uint32_t i = i0 + u * kBlock + tid;
+						const uint32_t i = i0 + u * NT + tid;
 						if constexpr (E::kBitParallel)
 							ch[u] = i >= mb ? 0 : bit_chr ? s_bch[i] : a.data[rowb + s_bpos[i]];
 						else
@@ -1143,7 +1145,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					}
 #pragma unroll
 					for (int u = 0; u < U; ++u) {
-						const uint32_t i = i0 + u * kBlock + tid;
+						const uint32_t i = i0 + u * NT + tid;
 						if (i < mb) {
 							const uint32_t off = s_bpos[i];
 							const uint32_t g   = rb[1] + i;
@@ -1154,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 								a.bkey[g] = kb + (int32_t)s_bkey[i];
 							a.bchar[g] = ch[u];
 							if constexpr (BATCH)
-								a.b.bdoc[rb[1] + i] = (uint32_t)d0 + slot_of(r * kBlock + (int)(off >> 6));
+								a.b.bdoc[rb[1] + i] = (uint32_t)d0 + slot_of(r * NT + (int)(off >> 6));
 						}
 					}
 				}
@@ -1167,7 +1169,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 					const int	   p   = __builtin_ctzll(m);
 					uint32_t	   pos = (uint32_t)(pbase + loff + (uint64_t)p);
 					if constexpr (BATCH)
-						pos = s_lpos0[r * kBlock + tid] + (uint32_t)p;
+						pos = s_lpos0[r * NT + tid] + (uint32_t)p;
 					const uint8_t ch = a.data[rowb + loff + p];
 					if ((uint64_t)rb[0] + li < a.cap_struct) {
 						a.positions[rb[0] + li] = pos;
@@ -1240,7 +1242,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMinWave
 		if (tid == 0) {
 			int32_t mn = s_mm[0][0], mx = s_mm[0][1];
 #pragma unroll
-			for (int w2 = 1; w2 < 4; ++w2) {
+			for (int w2 = 1; w2 < NWV; ++w2) {
 				mn = min(mn, s_mm[w2][0]);
 				mx = max(mx, s_mm[w2][1]);
 			}

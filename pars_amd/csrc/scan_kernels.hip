@@ -62,15 +62,19 @@ hipError_t launch_summary_reset(DevSummary* sum, const SummaryInit& in, hipStrea
 	return hipGetLastError();
 }
 
-hipError_t launch_scan_escape(const ScanArgs& a, int rows, hipStream_t s) {
+hipError_t launch_scan_escape(const ScanArgs& a, int rows, int waves, hipStream_t s) {
 	if (a.n == 0)
 		return hipSuccess;
+	if (waves == 6 || waves == 8)
+		return launch_scan_wide(a, false, rows, waves, s);
 	return launch_rows<EscapeEval>(a, rows, 4, s);
 }
 
-hipError_t launch_scan_parity(const ScanArgs& a, int rows, hipStream_t s) {
+hipError_t launch_scan_parity(const ScanArgs& a, int rows, int waves, hipStream_t s) {
 	if (a.n == 0)
 		return hipSuccess;
+	if (waves == 6 || waves == 8)
+		return launch_scan_wide(a, true, rows, waves, s);
 	return launch_rows<ParityEval>(a, rows, 4, s);
 }
 

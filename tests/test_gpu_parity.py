@@ -108,6 +108,42 @@ def test_rows_per_tile_and_paths(torch_cuda, g, rows, path):
     sc.close()
 
 
+@pytest.mark.parametrize("waves", [6, 8])
+@pytest.mark.parametrize("rows", [2, 4])
+@pytest.mark.parametrize("g", ["json", "csv"])
+def test_waves_per_block(torch_cuda, g, rows, waves):
+    """Six- and eight-wave blocks (24 and 32 KiB rows) of the bit-parallel
+    evaluators agree with the oracle, also with the chunked boundary."""
+    sc = Scanner(PLANS[g])
+    sc.set_option(N.PARS_OPT_WANT_CHARS, 1)
+    sc.set_option(N.PARS_OPT_ROWS_PER_TILE, rows)
+    sc.set_option(N.PARS_OPT_WAVES_PER_BLOCK, waves)
+    gen = {"json": lambda s, n: W.json(s, n, 12, 0.3), "csv": W.csv}[g]
+    for n in (100, 5 * 24576 + 7, (1 << 20) + 77, 6 << 20):
+        for d in (gen(n, n), W.alpha(n + 1, n, W.ALPHABETS[g])):
+            say(f"[gpu] waves={waves} rows={rows} {g} n={n}")
+            compare(g, d, gpu_full(torch_cuda, sc, d, want_pairs=True), f"waves={waves} rows={rows} n={n}")
+    assert sc.get_option(N.PARS_OPT_WAVES_PER_BLOCK) == waves
+    sc.close()
+    # chunks of one document with the carry, cut inside rows of the wider blocks
+    n = 900_001
+    d = gen(n, n)
+    ck = Scanner(PLANS[g])
+    ck.set_option(N.PARS_OPT_ROWS_PER_TILE, rows)
+    ck.set_option(N.PARS_OPT_WAVES_PER_BLOCK, waves)
+    ck.reserve_struct(n)
+    t, base = to_device(torch_cuda, d)
+    carry = N.ChunkCarry()
+    bounds = [0, 24576, 24577, 32768 * 9, 300_003, n]
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        carry = ck.scan_chunk(base + a, b - a, carry, None, a)
+    exp = O.full(PLANS[g].blob, d)
+    assert np.array_equal(d2h(ck.struct_ptr(), carry.out_base, np.uint32), exp["positions"])
+    r = ck.pair_chunks(carry, None, want_pairs=True)
+    assert np.array_equal(d2h(r.bracket_pair, r.n_brackets, np.int32), exp["pair"])
+    ck.close()
+
+
 def test_unaligned_inputs(torch_cuda):
     d = W.json(31, 200_000)
     for off in (1, 3, 5, 8, 13):
@@ -166,6 +202,11 @@ def test_scan_chunk_any_boundaries(torch_cuda, g):
     for a, b in zip(bounds[:-1], bounds[1:]):
         say(f"[gpu] chunk {g} [{a},{b})")
         carry = sc.scan_chunk(base + a, b - a, carry, None, a)
+    # a length or end past 2^32-1 is refused before anything is launched (a
+    # negative length, wrapped to size_t, once slipped through)
+    for bad_n, bad_base in ((2**64 - 5091, 0), (100, 2**32 - 50)):
+        with pytest.raises(Exception):
+            sc.scan_chunk(base, bad_n, carry, None, bad_base)
     exp = O.full(PLANS[g].blob, d)
     assert carry.out_base == exp["positions"].size
     got = d2h(sc.struct_ptr(), carry.out_base, np.uint32)

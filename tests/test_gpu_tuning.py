@@ -74,7 +74,9 @@ def test_radix_digit_cap_batch_and_refusal(torch_cuda):
 def test_option_bounds(torch_cuda):
     sc = Scanner("json")
     for opt, bad in [(N.PARS_OPT_LOOKBACK_WINDOW, 7), (N.PARS_OPT_LOOKBACK_WINDOW, 65),
-                     (N.PARS_OPT_BLOCKS_PER_CU, -1), (N.PARS_OPT_RADIX_BITS, 9)]:
+                     (N.PARS_OPT_BLOCKS_PER_CU, -1), (N.PARS_OPT_RADIX_BITS, 9),
+                     (N.PARS_OPT_WAVES_PER_BLOCK, 5), (N.PARS_OPT_WAVES_PER_BLOCK, 16),
+                     (N.PARS_OPT_LAST_PAIRING, 1)]:
         with pytest.raises(Exception):
             sc.set_option(opt, bad)
 
@@ -89,6 +91,8 @@ def test_autotune_document_keeps_parity(torch_cuda):
     assert res["rows"] in (0, 1, 2, 3, 4) and res["lb_window"] in (0, 16, 32, 48, 64)
     assert 0 <= res["blocks_per_cu"] < max(res["max_blocks_per_cu"], 1) or res["blocks_per_cu"] == 0
     assert res["radix_bits"] in (0, 4, 5, 6, 7, 8)
+    assert res["waves_per_block"] in (0, 4, 6, 8)
+    assert sc.get_option(N.PARS_OPT_WAVES_PER_BLOCK) == res["waves_per_block"]
     assert res["trials"] > 10 and res["scan_ms"] > 0
     assert sc.get_option(N.PARS_OPT_LOOKBACK_WINDOW) == res["lb_window"]
     del t

@@ -148,6 +148,9 @@ struct pars_scanner {
 	bool	  stack_pending = false;	 // the last enqueued R6 was the stack path (sort path if unbalanced)
 	bool	  pairs_ran		 = false;	 // the last scan_device / scan_host / scan_batch call ran R6
 	bool	  stack_launched = false;	 // ... and launched the stack path first
+	// the batch chain enqueued last (also what its captured graph replays) tried
+	// the stack path and left the sort to the host, if that path refused
+	bool	  batch_defers_sort = false;
 	bool	  keys_sparse	= false;	 // the document's keys are only at stack-chunk starts (a sort fills them)
 	uint64_t  cap_struct = 0, cap_br = 0, br_hint = 0;
 	int		  want_chars = 0, want_bitmap = 1, path = 0, rows = 0;
@@ -555,9 +558,39 @@ hipError_t launch_batch_scan(pars_scanner* sc, const ScanArgs& a, hipStream_t s)
 	return launch_batch_general(a, sc->hplan.n_descs, sc->hplan.wide, sc->rows, s);
 }
 
+PairArgs make_batch_pair_args(pars_scanner* sc, const BatchArgs& b, bool want_pairs) {
+	PairArgs p	= make_pair_args(sc, want_pairs);
+	p.bdoc		= b.bdoc;
+	p.br_start	= b.br_start;
+	p.st_start	= b.st_start;
+	p.n_docs	= b.n_docs;
+	p.doc_bad	= sc->doc_bad.as<uint32_t>();
+	p.doc_min	= b.doc_min;
+	p.doc_max	= b.doc_max;
+	p.doc_kbase = sc->doc_kbase.as<uint32_t>();
+	p.kb_part	= sc->kb_part.as<uint64_t>();
+	return p;
+}
+
+hipError_t launch_finish(pars_scanner* sc, const BatchArgs& b, bool run_pairs, hipStream_t s) {
+	const uint32_t nd = sc->hplan.n_descs;
+	const uint32_t m  = nd ? ((1u << (3 * nd)) - 1u) : 0u;
+	return launch_batch_finish(b, sc->doc_bad.as<uint32_t>(), m, sc->hplan.start_packed, run_pairs ? 1 : 0,
+							   sc->doc_flags.as<uint8_t>(), sc->summary.as<DevSummary>(), s);
+}
+
 // The whole batch: no host sync inside, so it can be captured into a graph.
+// A batch over 1 MiB pairs by stack matching when every document is
+// balanced; whether it is shows on the device only.  `defer_sort` (the
+// scan_batch call, which reads the summary back anyway): the sorting chain is
+// not launched, and finish_batch_pairs runs it after the read-back if the
+// stack path refused - one more round trip for a batch holding an unbalanced
+// document, ~20 launches fewer for every other.  Otherwise (the serving
+// runtime, which never waits) the sorting chain follows the stack kernels and
+// exits at once when they paired the batch.
 int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_docs, hipStream_t s,
 				  bool want_pairs, bool run_pairs, bool timed, bool packed = false) {
+	const bool defer_sort = !packed;
 	hipError_t e;
 	if ((e = hipMemcpyAsync(sc->doc_tab.p, sc->h_doctab, n_docs * 12, hipMemcpyHostToDevice, s)) != hipSuccess)
 		return hip_fail(e, "document table H2D");
@@ -576,19 +609,11 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 		return hip_fail(e, "batch scan launch");
 	if (timed && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
-	sc->pairs_ran	   = run_pairs;
-	sc->stack_launched = false;
+	sc->pairs_ran		  = run_pairs;
+	sc->stack_launched	  = false;
+	sc->batch_defers_sort = false;
 	if (run_pairs) {
-		PairArgs p = make_pair_args(sc, want_pairs);
-		p.bdoc	   = b.bdoc;
-		p.br_start = b.br_start;
-		p.st_start = b.st_start;
-		p.n_docs   = b.n_docs;
-		p.doc_bad  = sc->doc_bad.as<uint32_t>();
-		p.doc_min	= b.doc_min;
-		p.doc_max	= b.doc_max;
-		p.doc_kbase = sc->doc_kbase.as<uint32_t>();
-		p.kb_part	= sc->kb_part.as<uint64_t>();
+		PairArgs p = make_batch_pair_args(sc, b, want_pairs);
 		p.small	   = sc->radix_bits ? 0u : small_pairing_mode(n, n_docs);
 		// every document balanced: stack matching pairs the batch and marks the
 		// summary so the sorting chain after it exits at once; otherwise the
@@ -601,16 +626,13 @@ int enqueue_batch(pars_scanner* sc, const uint8_t* d_in, uint64_t n, uint64_t n_
 			k.doc_bad	= sc->doc_bad.as<uint32_t>();
 			if ((e = launch_stack_pairing(k, s)) != hipSuccess)
 				return hip_fail(e, "stack pairing launch");
-			sc->stack_launched = true;
+			sc->stack_launched	  = true;
+			sc->batch_defers_sort = defer_sort;
 		}
-		if ((e = launch_pairing(p, s)) != hipSuccess)
+		if (!sc->batch_defers_sort && (e = launch_pairing(p, s)) != hipSuccess)
 			return hip_fail(e, "pairing launch");
 	}
-	const uint32_t nd = sc->hplan.n_descs;
-	const uint32_t m  = nd ? ((1u << (3 * nd)) - 1u) : 0u;
-	if ((e = launch_batch_finish(b, sc->doc_bad.as<uint32_t>(), m, sc->hplan.start_packed, run_pairs ? 1 : 0,
-								 sc->doc_flags.as<uint8_t>(), sc->summary.as<DevSummary>(), s))
-		!= hipSuccess)
+	if ((e = launch_finish(sc, b, run_pairs, s)) != hipSuccess)
 		return hip_fail(e, "batch finish");
 	if (timed && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
 		return hip_fail(e, "hipEventRecord");
@@ -1129,6 +1151,35 @@ int pars_hip_scan_batch(pars_scanner* sc, const uint8_t* d_in, size_t n, const p
 			if (hipEventElapsedTime(&ms, sc->ev[1], sc->ev[2]) == hipSuccess)
 				st->p4 = ms;
 		}
+	}
+	float sort_ms = 0;
+	if (sc->batch_defers_sort) {  // the stack path was tried on the device: did it pair the batch?
+		if (!sc->h_sum->small_done) {  // some document is unbalanced: the sort pairs the batch
+			BatchArgs b = make_batch_args(sc, n, n_docs);
+			b.tile_rows = batch_tile_rows(sc->hplan, sc->path, sc->rows);
+			PairArgs  p = make_batch_pair_args(sc, b, want_pairs != 0);
+			hipError_t e;
+			if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+				return hip_fail(e, "hipEventRecord");
+			if ((e = launch_pairing(p, s)) != hipSuccess)
+				return hip_fail(e, "pairing launch");
+			if ((e = launch_finish(sc, b, true, s)) != hipSuccess)
+				return hip_fail(e, "batch finish");
+			if (st && (e = hipEventRecord(sc->ev[2], s)) != hipSuccess)
+				return hip_fail(e, "hipEventRecord");
+			if ((r = read_summary(sc, s)))
+				return r;
+			if (sc->h_sum->overflow & 4u)
+				return fail(PARS_ERR_ARGUMENT,
+							"batch pairing needs more than 2^32 (document, depth) keys; split the batch");
+			if (st)
+				(void)hipEventElapsedTime(&sort_ms, sc->ev[1], sc->ev[2]);
+		}
+	}
+	if (st && sort_ms > 0) {  // the deferred sort belongs to R6 (and to the chain's wall time)
+		st->p4 += sort_ms;
+		if (graph)
+			st->p1 += sort_ms;
 	}
 	std::memset(out, 0, sizeof *out);
 	out->n_docs		   = n_docs;

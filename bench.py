@@ -282,6 +282,93 @@ def bench_formats(torch, mib: int, reps: int) -> dict:
     return out
 
 
+# The reference's full-parse rates for these documents at 512 MiB (GB/s, mean
+# over its iterations; paper/plots/data/parse.csv:33-34, variant P3): its host
+# hook replaying a structural index already on the host, the scan excluded.
+REF_KV_P3_512 = {"toml": 0.397, "ini": 0.149}
+
+
+def bench_kv(torch, mib: int, reps: int, no_cpu: bool) -> dict:
+    """The INI / TOML stage-2 hooks (bench_parse_host.cu:69-134, 202-248) as one
+    device call, pars_hip_kv_parse: R1-R5 and the key-value table of the
+    reference benchmark's own documents (its record repeated to the target,
+    bench_parse_host.cu:305-312), device-resident.  The table is checked
+    record by record against the restatement.  CPU baseline: the restatement
+    (oracle/kv_hooks.py, pure Python, one core) replaying the oracle's
+    structural index of the document's first MiB (the replay + hook stage)."""
+    import numpy as np
+
+    from pars_amd import Plan, Scanner
+    from pars_amd import _native as N
+    from pars_amd import workloads as W
+
+    out = {"call": "pars_hip_kv_parse (R1-R5 + key-value table, device-resident)"}
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    for fmt in ("ini", "toml"):
+        doc = W.format_doc(fmt, mib << 20)
+        d = torch.from_numpy(doc).to("cuda")
+        torch.cuda.synchronize()
+        sc = Scanner(Plan.load(fmt))
+        st, walls, parts, r = N.StageTimes(), [], [], None
+        for i in range(reps + 1):
+            t0 = time.perf_counter()
+            r = sc.kv_parse(d.data_ptr(), doc.size, fmt, sp, st)
+            if i:
+                walls.append(time.perf_counter() - t0)
+                parts.append((st.p1, st.p2))
+        p = np.median(np.array(parts), axis=0)
+        dev = float(p[0] + p[1])
+        line = {"bytes": int(doc.size), "tokens": int(r.n_tokens), "n_role": r.n_role, "n_struct": int(r.n_struct),
+                "device_ms": round(dev, 3), "gb_per_s": round(doc.size / dev / 1e6, 2),
+                "stage_ms": {"r1_r5": round(float(p[0]), 3), "table": round(float(p[1]), 3)},
+                "table_gb_per_s": round(doc.size / float(p[1]) / 1e6, 2),
+                "call_ms": round(float(np.median(walls)) * 1e3, 3)}
+        # the document is the record repeated: the table must be the record's, shifted
+        from oracle import kv_hooks as K
+        from oracle import oracle as O
+
+        rec = W.FORMAT_RECORDS[fmt]
+        one = K.tokens(fmt, rec, O.scan(Plan.load(fmt).blob, np.frombuffer(rec, np.uint8))["positions"])
+        reps_ = doc.size // len(rec)
+        torch.cuda.synchronize()
+        role = d2h(r.role, r.n_tokens, np.uint8)
+        beg = d2h(r.begin, r.n_tokens, np.uint32)
+        shift = (np.arange(reps_, dtype=np.int64) * len(rec))[:, None]
+        line["parity"] = bool(
+            r.n_tokens == reps_ * len(one)
+            and np.array_equal(role.reshape(reps_, -1), np.broadcast_to(np.array([t[0] for t in one], np.uint8),
+                                                                        (reps_, len(one))))
+            and np.array_equal(beg.reshape(reps_, -1).astype(np.int64),
+                               np.array([t[1] for t in one], np.int64) + shift))
+        if mib == 512:
+            line["reference_p3_gb_s"] = REF_KV_P3_512[fmt]
+            line["vs_reference_p3"] = round(line["gb_per_s"] / REF_KV_P3_512[fmt], 1)
+        if not no_cpu:
+            sample = np.ascontiguousarray(doc[: 1 << 20])
+            pos = O.scan(Plan.load(fmt).blob, sample)["positions"]
+            raw = sample.tobytes()
+            prev = os.sched_getaffinity(0)
+            os.sched_setaffinity(0, {sorted(prev)[0]})
+            try:
+                t0 = time.perf_counter()
+                toks = K.tokens(fmt, raw, pos)
+                dt = time.perf_counter() - t0
+            finally:
+                os.sched_setaffinity(0, prev)
+            line["cpu_baseline"] = {"value": round(sample.size / dt / 1e9, 5), "unit": "GB/s", "cores": 1,
+                                    "kind": "port",
+                                    "stage": "replay + hook over a given structural index (oracle/kv_hooks.py, "
+                                             "pure Python)",
+                                    "sample": f"first {sample.size} bytes of the document, {len(toks)} tokens"}
+        out[fmt] = line
+        log(f"kv: {fmt} {line}")
+        sc.close()
+        del d
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_stream(sc, doc, stream: int, reps: int = 3) -> dict:
     """The host boundary: the same document handed over in host memory and
     streamed through pinned staging (64 MiB chunks, 8 copy threads), uploads and
@@ -621,6 +708,9 @@ def main() -> int:
     ap.add_argument("--formats-mib", type=int, default=512)
     ap.add_argument("--only-formats", action="store_true", help="run only the six per-format documents")
     ap.add_argument("--tape-cpu-mib", type=int, default=256, help="CPU baseline sample of the JSON tape")
+    ap.add_argument("--no-kv", action="store_true", help="skip the INI / TOML key-value tables")
+    ap.add_argument("--only-kv", action="store_true", help="run only the INI / TOML key-value tables")
+    ap.add_argument("--kv-mib", type=int, default=512)
     ap.add_argument("--knobs", default="",
                     help="fixed launch knobs for the document scans instead of tuning, e.g. lb_window=16,rows=4 "
                          "(rows, lb_window, blocks_per_cu, radix_bits, waves); profiles use this")
@@ -642,6 +732,14 @@ def main() -> int:
 
         torch.cuda.init()
         print(json.dumps({"formats": bench_formats(torch, args.formats_mib, max(args.steps // 2, 3))}), flush=True)
+        return 0
+
+    if args.only_kv:
+        import torch
+
+        torch.cuda.init()
+        print(json.dumps({"kv_tables": bench_kv(torch, args.kv_mib, max(args.steps // 2, 3), args.no_cpu)}),
+              flush=True)
         return 0
 
     if args.only_serve or args.only_batch:
@@ -772,6 +870,9 @@ def main() -> int:
     if not args.no_formats:
         torch.cuda.empty_cache()
         line["formats"] = bench_formats(torch, args.formats_mib, max(args.steps // 2, 3))
+    if not args.no_kv:
+        torch.cuda.empty_cache()
+        line["kv_tables"] = bench_kv(torch, args.kv_mib, max(args.steps // 2, 3), args.no_cpu)
     if not args.no_batch:
         line["batched"] = bench_batch(torch, args.batch_docs, args.batch_iters, args.plain_sample, base,
                                       tune=not args.no_tune)

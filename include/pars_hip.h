@@ -424,6 +424,45 @@ int pars_hip_json_tape(
 	pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, pars_stage_times* st, pars_json_tape* out
 );
 
+/* The INI / TOML key-value tables: every token the reference's stage-2 hooks
+ * for these formats (bench/bench_parse_host.cu: ini_parse :69-134, toml_parse
+ * :202-248, splitting gaps with each_token :34-56, driven through
+ * model::replay, visit.hpp:35-48) meet, in document order, with the role the
+ * hook gives it.  A token is a byte range of the input (quoted tokens keep
+ * their quotes, as the hooks' string_views do). */
+typedef enum { PARS_KV_INI = 0, PARS_KV_TOML = 1 } pars_kv_format;
+
+typedef enum {
+	PARS_KV_SECTION = 0, /* INI: sections; TOML: sections */
+	PARS_KV_KEY		= 1, /* INI: keys (ini_parse::add(tok, false)); TOML: a key token (toml_parse::key) */
+	PARS_KV_VALUE	= 2, /* INI: vals; TOML: the value of an entry (entries: (its key, it)) */
+	PARS_KV_ELEMENT = 3, /* TOML: an array element (toml_parse::elems) */
+	PARS_KV_SKIPPED = 4	 /* inside a comment the hook tracks (';' / '#' as a structural byte): not seen */
+} pars_kv_role;
+
+typedef struct {
+	const uint8_t*	role;		/* pars_kv_role per token */
+	const uint32_t* begin;		/* byte offset of its first byte */
+	const uint32_t* end;		/* one past its last byte */
+	const int32_t*	key;		/* TOML: a VALUE's key, the last KEY token before it (-1: none yet);
+								 * -1 for the other roles.  NULL for INI */
+	const uint32_t* structural; /* the structural index the table was built from */
+	uint64_t		n_tokens;
+	uint64_t		n_struct;
+	uint64_t		n_role[5];	/* tokens per role */
+} pars_kv_table;
+
+/* Scan an INI or TOML document on the device (R1-R5; the hooks need no
+ * pairing) and build its key-value table there: the reference's full-parse
+ * benchmark for these formats (bench_parse_host.cu:250-333: scan_host, replay
+ * into the hook) without the host replay.  Every array is device memory, valid
+ * until the scanner's next call.  At most 2^32-1 bytes.  Blocks until done;
+ * st->p1 = R1-R5, p2 = the table's kernels (ms), when st is not NULL. */
+int pars_hip_kv_parse(
+	pars_scanner* sc, const uint8_t* d_in, size_t n, int format, pars_stream stream, pars_stage_times* st,
+	pars_kv_table* out
+);
+
 /* Message for the last failing call on this thread ("" if none). */
 const char* pars_hip_last_error(void);
 

@@ -26,6 +26,7 @@ __all__ = [
     "Scanner",
     "DeviceScan",
     "DeviceTape",
+    "DeviceKvTable",
     "HostScan",
     "ChunkCarry",
     "scan_device",
@@ -148,6 +149,27 @@ class DeviceTape:
     def __repr__(self) -> str:
         return (f"DeviceTape(n_nodes={self.n_nodes}, arena_bytes={self.arena_bytes}, ok={self.ok}, "
                 f"fail_event={self.fail_event})")
+
+
+class DeviceKvTable:
+    """An INI / TOML key-value table (pars_hip_kv_parse): device pointers into
+    the scanner's arena, valid until its next call.  Per token: role (uint8,
+    N.PARS_KV_*), begin and end (uint32 byte offsets), and for TOML key (int32:
+    a VALUE's key token, -1 otherwise)."""
+
+    def __init__(self, r: N.KvTable, fmt: str):
+        self.format = fmt
+        self.role = r.role or 0
+        self.begin = r.begin or 0
+        self.end = r.end or 0
+        self.key = r.key or 0
+        self.structural = r.structural or 0
+        self.n_tokens = r.n_tokens
+        self.n_struct = r.n_struct
+        self.n_role = list(r.n_role)
+
+    def __repr__(self) -> str:
+        return f"DeviceKvTable({self.format}, n_tokens={self.n_tokens}, n_role={self.n_role})"
 
 
 class HostScan:
@@ -364,6 +386,18 @@ class Scanner:
             self._h, C.c_void_p(d_data), n, C.c_void_p(stream or 0),
             C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
         return DeviceTape(r, n)
+
+    def kv_parse(self, d_data: int, n: int, fmt: str, stream: int | None = None,
+                 stage_times: N.StageTimes | None = None) -> DeviceKvTable:
+        """R1-R5 and the key-value table of one device-resident INI or TOML
+        document (pars_hip_kv_parse): the reference's stage-2 hooks for these
+        formats (bench/bench_parse_host.cu:69-134, 202-248) built on the device."""
+        code = {"ini": N.PARS_KV_INI, "toml": N.PARS_KV_TOML}[fmt]
+        r = N.KvTable()
+        _check(self._lib.pars_hip_kv_parse(
+            self._h, C.c_void_p(d_data), n, code, C.c_void_p(stream or 0),
+            C.byref(stage_times) if stage_times is not None else None, C.byref(r)))
+        return DeviceKvTable(r, fmt)
 
     def reserve_struct(self, cap: int) -> None:
         _check(self._lib.pars_hip_reserve_struct(self._h, cap))

@@ -76,6 +76,23 @@ class JsonTape(C.Structure):
     ]
 
 
+class KvTable(C.Structure):
+    _fields_ = [
+        ("role", C.c_void_p),
+        ("begin", C.c_void_p),
+        ("end", C.c_void_p),
+        ("key", C.c_void_p),
+        ("structural", C.c_void_p),
+        ("n_tokens", C.c_uint64),
+        ("n_struct", C.c_uint64),
+        ("n_role", C.c_uint64 * 5),
+    ]
+
+
+PARS_KV_INI, PARS_KV_TOML = 0, 1
+PARS_KV_SECTION, PARS_KV_KEY, PARS_KV_VALUE, PARS_KV_ELEMENT, PARS_KV_SKIPPED = 0, 1, 2, 3, 4
+
+
 class HostResult(C.Structure):
     _fields_ = [
         ("structural", C.POINTER(C.c_uint32)),
@@ -275,6 +292,9 @@ PROTOTYPES = {
          C.POINTER(TuneResult)],
     ),
     "pars_hip_struct_ptr": (C.c_void_p, [C.c_void_p]),
+    "pars_hip_kv_parse": (
+        C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.POINTER(StageTimes), C.POINTER(KvTable)]
+    ),
     "pars_hip_json_tape": (
         C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(StageTimes), C.POINTER(JsonTape)]
     ),

@@ -25,7 +25,7 @@ ARCH = "gfx950"
 HIP_SOURCES = [
     "scan_kernels.hip", "scan_wide_kernels.hip", "batch_kernels.hip", "event_kernels.hip", "event_batch_kernels.hip",
     "pair_kernels.hip", "pair_stack.hip",
-    "dom_kernels.hip", "api.hip", "server.hip",
+    "dom_kernels.hip", "kv_kernels.hip", "api.hip", "server.hip",
 ]
 CXX_SOURCES = ["plan_dev.cpp"]
 HIP_FLAGS = [

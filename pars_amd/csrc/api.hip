@@ -188,6 +188,10 @@ struct pars_scanner {
 		dom_parent, dom_close, dom_sum, tape_kind, tape_beg, tape_end, tape_soff, tape_arena;
 	DomSummary* h_dom = nullptr;  // pinned
 	size_t		h_dom_cap = 0;
+	// the INI / TOML key-value tables: per-tile and per-lane scratch, the table
+	DevBuf		kv_tile, kv_lane, kv_mask, kv_sum, kv_role, kv_beg, kv_end, kv_key;
+	KvSummary*	h_kv	 = nullptr;	 // pinned
+	size_t		h_kv_cap = 0;
 	MemSource	mem;  // every DevBuf above grows through it
 
 	// Every device buffer of the scanner.
@@ -199,7 +203,8 @@ struct pars_scanner {
 						  &doc_final, &doc_min, &doc_max, &doc_last, &bdoc, &doc_bad, &doc_flags, &doc_pack, &doc_kbase,
 						  &kb_part, &d_chunk[0], &d_chunk[1], &carry, &dom_tinfo, &dom_gbeg, &dom_gend, &dom_gdec,
 						  &dom_sub, &dom_top, &dom_tblk, &dom_hblk, &dom_parent, &dom_close, &dom_sum, &tape_kind,
-						  &tape_beg, &tape_end, &tape_soff, &tape_arena})
+						  &tape_beg, &tape_end, &tape_soff, &tape_arena, &kv_tile, &kv_lane, &kv_mask, &kv_sum,
+						  &kv_role, &kv_beg, &kv_end, &kv_key})
 			f(*b);
 	}
 
@@ -220,6 +225,8 @@ struct pars_scanner {
 			mem.budget->users.fetch_sub(1);
 		if (h_dom)
 			(void)hipHostFree(h_dom);
+		if (h_kv)
+			(void)hipHostFree(h_kv);
 		if (h_doctab)
 			(void)hipHostFree(h_doctab);
 		for (int i = 0; i < 2; ++i) {
@@ -1787,6 +1794,118 @@ int pars_hip_json_tape(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_str
 	out->n_struct	 = ns;
 	out->fail_event	 = d.fail_event == kDomNone ? -1 : (int64_t)d.fail_event;
 	out->ok			 = (int32_t)d.ok;
+	return PARS_OK;
+}
+
+int pars_hip_kv_parse(pars_scanner* sc, const uint8_t* d_in, size_t n, int format, pars_stream stream,
+					  pars_stage_times* st, pars_kv_table* out) {
+	if (!sc || !out || (n && !d_in))
+		return fail(PARS_ERR_ARGUMENT, "null argument");
+	if (format != PARS_KV_INI && format != PARS_KV_TOML)
+		return fail(PARS_ERR_ARGUMENT, "unknown key-value format %d (PARS_KV_INI or PARS_KV_TOML)", format);
+	if (n > 0xFFFFFFFFull)
+		return fail(PARS_ERR_ARGUMENT, "the key-value table keeps 32-bit offsets: at most 2^32-1 bytes (%llu here)",
+					(unsigned long long)n);
+	std::memset(out, 0, sizeof *out);
+	if (st)
+		*st = pars_stage_times {0, 0, 0, 0};
+	if (n == 0)
+		return PARS_OK;
+	hipStream_t		   s = (hipStream_t)stream;
+	pars_device_result res;
+	const int		   wb = sc->want_bitmap;
+	sc->want_bitmap		  = 1;	// the passes read the structural bitmap
+	int r				  = pars_hip_scan_device(sc, d_in, n, stream, 0, 0, st, &res);
+	sc->want_bitmap		  = wb;
+	if (r)
+		return r;
+	const bool	   toml	 = format == PARS_KV_TOML;
+	const uint64_t tiles = (n + kKvTile - 1) / kKvTile, lanes = tiles * kKvLanes;
+	if ((r = sc->kv_tile.ensure(tiles * 8 * 4)) || (r = sc->kv_lane.ensure(lanes * 3 * 4)) ||
+		(r = sc->kv_mask.ensure(lanes * 4 * 8)) || (r = sc->kv_sum.ensure(sizeof(KvSummary))))
+		return r;
+	if (!sc->h_kv && (r = ensure_pinned(reinterpret_cast<void**>(&sc->h_kv), sc->h_kv_cap, sizeof(KvSummary))))
+		return r;
+	KvArgs a {};
+	a.data			 = d_in;
+	a.n				 = n;
+	a.bitmap		 = sc->bitmap.as<uint64_t>();
+	a.n_tiles		 = (uint32_t)tiles;
+	a.toml			 = toml ? 1 : 0;
+	uint32_t* tile	 = sc->kv_tile.as<uint32_t>();
+	a.t_tmap		 = tile;
+	a.t_tin			 = tile + tiles;
+	a.t_hmap		 = tile + 2 * tiles;
+	a.t_hin			 = tile + 3 * tiles;
+	a.t_cnt			 = tile + 4 * tiles;
+	a.t_off			 = tile + 5 * tiles;
+	a.t_key			 = reinterpret_cast<int32_t*>(tile + 6 * tiles);
+	a.t_kin			 = reinterpret_cast<int32_t*>(tile + 7 * tiles);
+	uint32_t* lane32 = sc->kv_lane.as<uint32_t>();
+	a.l_tmap		 = lane32;
+	a.l_hmap		 = lane32 + lanes;
+	a.l_off			 = lane32 + 2 * lanes;
+	uint64_t* mask	 = sc->kv_mask.as<uint64_t>();
+	a.l_start		 = mask;
+	a.l_first		 = mask + lanes;
+	a.l_end			 = mask + 2 * lanes;
+	a.l_endq		 = mask + 3 * lanes;
+	a.sum			 = sc->kv_sum.as<KvSummary>();
+	hipError_t e;
+	float	   ms = 0, ms2 = 0;
+	if (st && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = launch_kv_count(a, s)) != hipSuccess)
+		return hip_fail(e, "key-value count launch");
+	if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = hipMemcpyAsync(sc->h_kv, a.sum, sizeof(KvSummary), hipMemcpyDeviceToHost, s)) != hipSuccess)
+		return hip_fail(e, "key-value summary readback");
+	if ((e = hipStreamSynchronize(s)) != hipSuccess)
+		return hip_fail(e, "hipStreamSynchronize");
+	if (st && hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) != hipSuccess)
+		ms = 0;
+	const uint64_t ntok = sc->h_kv->n_tokens;
+	if (ntok > n)
+		return fail(PARS_ERR_DEVICE, "more tokens than bytes (internal error)");
+	const uint64_t cap = std::max<uint64_t>(ntok, 1);
+	if ((r = sc->kv_role.ensure(cap)) || (r = sc->kv_beg.ensure(cap * 4)) || (r = sc->kv_end.ensure(cap * 4)) ||
+		(toml && (r = sc->kv_key.ensure(cap * 4))))
+		return r;
+	a.role = sc->kv_role.as<uint8_t>();
+	a.beg  = sc->kv_beg.as<uint32_t>();
+	a.end  = sc->kv_end.as<uint32_t>();
+	a.key  = toml ? sc->kv_key.as<int32_t>() : nullptr;
+	if (st && (e = hipEventRecord(sc->ev[0], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = launch_kv_emit(a, s)) != hipSuccess)
+		return hip_fail(e, "key-value table launch");
+	if (st && (e = hipEventRecord(sc->ev[1], s)) != hipSuccess)
+		return hip_fail(e, "hipEventRecord");
+	if ((e = hipMemcpyAsync(sc->h_kv, a.sum, sizeof(KvSummary), hipMemcpyDeviceToHost, s)) != hipSuccess)
+		return hip_fail(e, "key-value summary readback");
+	if ((e = hipStreamSynchronize(s)) != hipSuccess)
+		return hip_fail(e, "hipStreamSynchronize");
+	if (st) {
+		if (hipEventElapsedTime(&ms2, sc->ev[0], sc->ev[1]) != hipSuccess)
+			ms2 = 0;
+		st->p2 = ms + ms2;
+	}
+	const KvSummary& k = *sc->h_kv;
+	if (k.bad)
+		return fail(PARS_ERR_DEVICE, "the key-value table passes disagree on the token count (internal error)");
+	uint64_t		 tot = 0;
+	for (int i = 0; i < 5; ++i) tot += k.roles[i];
+	if (tot != ntok)
+		return fail(PARS_ERR_DEVICE, "token roles do not add up to the token count (internal error)");
+	out->role		= a.role;
+	out->begin		= a.beg;
+	out->end		= a.end;
+	out->key		= a.key;
+	out->structural = res.structural;
+	out->n_tokens	= ntok;
+	out->n_struct	= res.n_struct;
+	for (int i = 0; i < 5; ++i) out->n_role[i] = k.roles[i];
 	return PARS_OK;
 }
 

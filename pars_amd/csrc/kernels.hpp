@@ -338,4 +338,57 @@ constexpr uint64_t kDomSub = 1024;	// tokens per block of the per-gap kernels
 
 hipError_t launch_json_tape(const DomArgs& a, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// The INI / TOML key-value tables (kv_kernels.hip): the reference's stage-2
+// hooks for these formats (bench/bench_parse_host.cu:30-134, 202-248)
+// computed from a document's bytes and structural bitmap.  A tile is 256
+// lanes of 64 bytes; per-lane arrays are indexed tile * 256 + lane.
+constexpr uint32_t kKvLanes = 256;
+constexpr uint32_t kKvTile	= kKvLanes * 64;
+
+struct KvSummary {
+	uint64_t n_tokens;
+	uint32_t open_end;	// a token runs to the document's end
+	uint32_t bad;		// the table pass met more tokens than the count (internal error)
+	uint32_t roles[5];	// tokens per role
+	uint32_t pad2;
+};
+
+struct KvArgs {
+	const uint8_t*	data;
+	uint64_t		n;
+	const uint64_t* bitmap;	 // structural bitmap, ceil(n / 64) words
+	uint32_t		n_tiles;
+	int				toml;	 // 0 INI, 1 TOML
+	// per tile (n_tiles each)
+	uint32_t* t_tmap;	// tokenizer map of the tile (packed)
+	uint32_t* t_tin;	// tokenizer state entering the tile
+	uint32_t* t_hmap;	// hook map of the tile (packed)
+	uint32_t* t_hin;	// hook state entering the tile
+	uint32_t* t_cnt;	// tokens starting in the tile
+	uint32_t* t_off;	// tokens starting before the tile
+	int32_t*  t_key;	// TOML: the tile's last KEY token, -1 if none
+	int32_t*  t_kin;	// TOML: the last KEY token before the tile, -1 if none
+	// per lane
+	uint32_t* l_tmap;  // tokenizer map from the tile's start to the lane's (packed)
+	uint32_t* l_hmap;  // hook map from the tile's start to the lane's (packed)
+	uint32_t* l_off;   // tokens starting earlier in the tile | (a token is open at the lane's start) << 31
+	uint64_t* l_start; // bit j: a token starts at byte j
+	uint64_t* l_first; // ... and is the first of its gap
+	uint64_t* l_end;   // bit j: a token ends at byte j (before it) or with it (closing quote)
+	uint64_t* l_endq;  // bit j: the token ending at j ends with it (a closing quote)
+	KvSummary* sum;
+	// outputs
+	uint8_t*  role;
+	uint32_t* beg;
+	uint32_t* end;
+	int32_t*  key;	// TOML only
+};
+
+// The tokenizer and hook passes up to the token count (sum->n_tokens is ready
+// once they are done) ...
+hipError_t launch_kv_count(const KvArgs& a, hipStream_t s);
+// ... and the passes that write the table (role, beg, end and, TOML, key).
+hipError_t launch_kv_emit(const KvArgs& a, hipStream_t s);
+
 }  // namespace pars_hip

@@ -2,11 +2,15 @@
 // backend's template API (include/pars/backend/cuda/pipeline.cuh:1164-1281):
 //
 //   pars::hip::scanner<S>      scan_device / scan_host / reserve_struct /
-//                              scan_chunk / struct_ptr, and json_tape (the
+//                              scan_chunk / struct_ptr, json_tape (the
 //                              reference's stage-2 JSON hook, bench_parse.cu:
-//                              336-348, built on the device)
+//                              336-348, built on the device) and kv_parse
+//                              (its INI / TOML hooks, bench_parse_host.cu)
 //   pars::hip::scan_device<S>, scan_host<S>, scan_chunk<S>
 //                              one scanner per plan per thread (:1260-1281)
+//   pars::hip::replay, structural_view, structural_cursor, trim
+//                              the consumer side over a host result
+//                              (include/pars/model/visit.hpp, cursor.hpp)
 //
 // S is the caller's compiled plan (a `pars::scan_plan` constant from
 // compile::make_scan_plan<G>(), or a `pars_plan`); its bytes are handed over
@@ -23,6 +27,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
 namespace pars::hip {
@@ -79,6 +84,10 @@ using stage_times = pars_stage_times;
 // The JSON tape (pars_hip_json_tape): device pointers, valid until the
 // scanner's next call.
 using json_tape = pars_json_tape;
+
+// The INI / TOML key-value table (pars_hip_kv_parse): device pointers, valid
+// until the scanner's next call.
+using kv_table = pars_kv_table;
 
 // A scanner for a plan known at run time.
 class runtime_scanner {
@@ -149,6 +158,16 @@ public:
 		return t;
 	}
 
+	// R1-R5 and the key-value table of a device-resident INI or TOML document:
+	// the reference's full-parse hooks (bench_parse_host.cu:69-134, 202-248)
+	// built on the device instead of replayed on the host.
+	kv_table kv_parse_device(const std::uint8_t* d_data, std::size_t n, pars_kv_format format,
+							 hipStream_t s = nullptr, stage_times* st = nullptr) const {
+		kv_table t {};
+		check(pars_hip_kv_parse(h_, d_data, n, format, s, st, &t));
+		return t;
+	}
+
 	pars_scanner* handle() const noexcept { return h_; }
 
 private:
@@ -182,6 +201,81 @@ chunk_carry scan_chunk(const std::uint8_t* d_chunk, std::size_t n, chunk_carry i
 					   std::size_t byte_base = 0) {
 	static thread_local scanner<S> sc;
 	return sc.scan_chunk(d_chunk, n, in, s, byte_base);
+}
+
+// ---------------------------------------------------------------------------
+// Consumer side (host): the structural index of a scan_host result handed to
+// user code, as the reference's model::replay (include/pars/model/visit.hpp:
+// 28-48) and model::structural_cursor (include/pars/model/cursor.hpp:16-56)
+// hand it.  Format-agnostic: byte offsets and bracket partners only.
+// ---------------------------------------------------------------------------
+
+// visit.hpp:28-32: offsets of the structural bytes, and per structural the
+// index of its partner bracket (-1: none), or no partners at all.
+struct structural_view {
+	const std::int32_t* pos	 = nullptr;
+	const std::int32_t* pair = nullptr;
+	std::size_t			n	 = 0;
+};
+
+// A view of a scan_host result (pairs when it was asked for them).
+inline structural_view view_of(const host_result& r) {
+	return {r.structural.data(), r.pair_pos.empty() ? nullptr : r.pair_pos.data(), r.structural.size()};
+}
+
+// visit.hpp:35-48: one pass over the n input bytes as a covering,
+// non-overlapping event stream - v.scalar(begin, end) for every non-empty run
+// of bytes between structural bytes (and before the first / after the last),
+// v.symbol(offset, byte, partner offset or -1) for every structural byte.
+template<class Visitor>
+void replay(const std::uint8_t* base, std::size_t n, const structural_view& sv, Visitor&& v) {
+	std::size_t next = 0;  // first byte not yet reported
+	for (std::size_t i = 0; i < sv.n; ++i) {
+		const auto at = static_cast<std::size_t>(sv.pos[i]);
+		if (at > next)
+			v.scalar(next, at);
+		std::int32_t partner = -1;
+		if (sv.pair) {
+			const std::int32_t j = sv.pair[i];
+			if (j >= 0)
+				partner = sv.pos[j];
+		}
+		v.symbol(at, static_cast<char>(base[at]), partner);
+		next = at + 1;
+	}
+	if (n > next)
+		v.scalar(next, n);
+}
+
+// cursor.hpp:16-56: random access to the index - offset, byte and partner of
+// structural i, and the run of bytes just before it (its gap).
+struct structural_cursor {
+	const std::int32_t* pos	 = nullptr;
+	const std::int32_t* pair = nullptr;
+	const std::uint8_t* base = nullptr;
+	std::size_t			n	 = 0;
+
+	bool		 valid(std::size_t i) const noexcept { return i < n; }
+	std::int32_t off(std::size_t i) const noexcept { return pos[i]; }
+	char		 ch(std::size_t i) const noexcept { return static_cast<char>(base[pos[i]]); }
+	std::int32_t match_index(std::size_t i) const noexcept { return pair ? pair[i] : -1; }
+	std::size_t	 gap_begin(std::size_t i) const noexcept { return i == 0 ? 0 : static_cast<std::size_t>(pos[i - 1]) + 1; }
+	std::size_t	 gap_end(std::size_t i) const noexcept { return static_cast<std::size_t>(pos[i]); }
+
+	std::string_view gap(std::size_t i) const noexcept {
+		const std::size_t b = gap_begin(i);
+		return std::string_view(reinterpret_cast<const char*>(base) + b, gap_end(i) - b);
+	}
+};
+
+// cursor.hpp:58-66: ASCII whitespace (space, tab, newline, carriage return)
+// removed from both ends.
+inline std::string_view trim(std::string_view s) noexcept {
+	auto ws = [](char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; };
+	std::size_t b = 0, e = s.size();
+	while (b < e && ws(s[b])) ++b;
+	while (e > b && ws(s[e - 1])) --e;
+	return s.substr(b, e - b);
 }
 
 }  // namespace pars::hip

@@ -27,6 +27,9 @@ __all__ = [
     "DeviceScan",
     "DeviceTape",
     "DeviceKvTable",
+    "replay",
+    "trim",
+    "StructuralCursor",
     "HostScan",
     "ChunkCarry",
     "scan_device",
@@ -170,6 +173,73 @@ class DeviceKvTable:
 
     def __repr__(self) -> str:
         return f"DeviceKvTable({self.format}, n_tokens={self.n_tokens}, n_role={self.n_role})"
+
+
+# ---------------------------------------------------------------------------
+# Consumer side (host): the structural index handed to user code as the
+# reference's model::replay (include/pars/model/visit.hpp:35-48) and
+# model::structural_cursor (include/pars/model/cursor.hpp:16-66) hand it; the
+# same functions as pars::hip::replay / structural_cursor in pars_hip.hpp.
+# ---------------------------------------------------------------------------
+_WS = b" \t\n\r"
+
+
+def replay(base, positions, visitor, pairs=None) -> None:
+    """One pass over the input as a covering, non-overlapping event stream:
+    visitor.scalar(begin, end) for every non-empty run of bytes between
+    structural bytes (and before the first / after the last), and
+    visitor.symbol(offset, byte, partner offset or -1) for every structural
+    byte.  `pairs`: per structural, its partner's structural index or -1."""
+    mv = memoryview(base).cast("B")
+    nxt = 0
+    for i, at in enumerate(positions):
+        at = int(at)
+        if at > nxt:
+            visitor.scalar(nxt, at)
+        partner = -1
+        if pairs is not None and int(pairs[i]) >= 0:
+            partner = int(positions[int(pairs[i])])
+        visitor.symbol(at, mv[at], partner)
+        nxt = at + 1
+    if len(mv) > nxt:
+        visitor.scalar(nxt, len(mv))
+
+
+def trim(s: bytes) -> bytes:
+    """ASCII whitespace (space, tab, newline, carriage return) removed from both ends."""
+    return bytes(s).strip(_WS)
+
+
+class StructuralCursor:
+    """Random access to a structural index: offset, byte and partner of
+    structural i, and the run of bytes just before it (its gap)."""
+
+    def __init__(self, base, positions, pairs=None):
+        self.base = memoryview(base).cast("B")
+        self.pos = positions
+        self.pair = pairs
+        self.n = len(positions)
+
+    def valid(self, i: int) -> bool:
+        return 0 <= i < self.n
+
+    def off(self, i: int) -> int:
+        return int(self.pos[i])
+
+    def ch(self, i: int) -> int:
+        return self.base[int(self.pos[i])]
+
+    def match_index(self, i: int) -> int:
+        return -1 if self.pair is None else int(self.pair[i])
+
+    def gap_begin(self, i: int) -> int:
+        return 0 if i == 0 else int(self.pos[i - 1]) + 1
+
+    def gap_end(self, i: int) -> int:
+        return int(self.pos[i])
+
+    def gap(self, i: int) -> bytes:
+        return bytes(self.base[self.gap_begin(i):self.gap_end(i)])
 
 
 class HostScan:

@@ -95,6 +95,16 @@ def build_facade_check() -> str:
     return out
 
 
+def build_replay_check() -> str:
+    """tests/cpp/replay_check: the facade's consumer side (replay,
+    structural_cursor, trim), host only."""
+    src = os.path.join(ROOT, "tests", "cpp", "replay_check.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "replay_check")
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          "-I" + os.path.join(ROOT, "include"), src, "-o", out])
+    return out
+
+
 def build_gap_check() -> str:
     """tests/cpp/gap_check: the JSON tape's gap parser (device code) built for
     the host, checked against the oracle by tests/test_gap_parser.py."""
@@ -127,6 +137,7 @@ def build_all() -> None:
     build_oracle()
     build_hip()
     build_facade_check()
+    build_replay_check()
     build_gap_check()
     build_ring_check()
     build_event_check()

@@ -22,12 +22,17 @@ def short(name: str, width: int = 70) -> str:
 def summarize(path: str) -> str:
     con = sqlite3.connect(path)
     out = [f"# {path}"]
-    kernels = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
+    names = {r[0] for r in con.execute("select name from sqlite_master where type in ('table', 'view')")}
+    kernels = []
+    if "top_kernels" in names:
+        kernels = list(con.execute("select name, total_calls, total_duration, average, percentage from top_kernels"))
     if kernels:
         out.append(f"{'kernel':70s} {'calls':>6s} {'avg_ms':>10s} {'total_ms':>11s} {'pct':>6s}")
         for name, calls, tot, avg, pct in kernels:
             out.append(f"{short(name):70s} {calls:6d} {avg / 1e3:10.4f} {tot / 1e3:11.3f} {pct:6.2f}")
-    rows = list(con.execute("select kernel_name, counter_name, value, duration from counters_collection"))
+    rows = []
+    if "counters_collection" in names:
+        rows = list(con.execute("select kernel_name, counter_name, value, duration from counters_collection"))
     if rows:
         acc = defaultdict(list)
         for kname, cname, value, _dur in rows:

@@ -232,6 +232,20 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_tokmap(KvArgs a) {
 constexpr int kChainThreads = 1024;
 enum { CHAIN_MAPS = 0, CHAIN_SUMS = 1, CHAIN_MAX = 2 };
 
+// f(t, in[t]) for t in [b, e) in order, eight loads issued per round trip.
+template<class F>
+__device__ __forceinline__ void each8(const uint32_t* in, uint32_t b, uint32_t e, F&& f) {
+	for (uint32_t t = b; t < e; t += 8) {
+		uint32_t v[8];
+#pragma unroll
+		for (int u = 0; u < 8; ++u) v[u] = t + u < e ? in[t + u] : 0u;
+#pragma unroll
+		for (int u = 0; u < 8; ++u)
+			if (t + u < e)
+				f(t + u, v[u]);
+	}
+}
+
 __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, uint32_t* out, uint32_t T, int kind,
 															 uint32_t init, uint64_t* total) {
 	constexpr int	   NW = kChainThreads / kWave;
@@ -242,7 +256,7 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 	const uint32_t	   b = min(T, (uint32_t)tid * per), e = min(T, b + per);
 	if (kind == CHAIN_MAPS) {
 		Map8 f = M8::id();
-		for (uint32_t t = b; t < e; ++t) f = M8::compose(M8::unpack(in[t]), f);
+		each8(in, b, e, [&](uint32_t, uint32_t v) { f = M8::compose(M8::unpack(v), f); });
 		const Map8 incl = wave_scan_maps<true>(f);
 		const Map8 excl = wave_shift_maps<true>(incl);
 		if (lane_id() == kWave - 1)
@@ -251,13 +265,13 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 		Map8 pre = M8::id();
 		for (int k = 0; k < w; ++k) pre = M8::compose(s_m[k], pre);
 		uint32_t s = M8::apply(M8::compose(excl, pre), init);
-		for (uint32_t t = b; t < e; ++t) {
+		each8(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = s;
-			s	   = M8::apply(M8::unpack(in[t]), s);
-		}
+			s	   = M8::apply(M8::unpack(v), s);
+		});
 	} else if (kind == CHAIN_SUMS) {
 		uint64_t f = 0;
-		for (uint32_t t = b; t < e; ++t) f += in[t];
+		each8(in, b, e, [&](uint32_t, uint32_t v) { f += v; });
 		// the document has fewer than 2^32 bytes, so fewer tokens
 		const uint32_t incl = wave_incl_sum((uint32_t)f);
 		if (lane_id() == kWave - 1)
@@ -270,15 +284,15 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 			all += s_v[k];
 		}
 		uint32_t s = pre + incl - (uint32_t)f;
-		for (uint32_t t = b; t < e; ++t) {
+		each8(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = s;
-			s += in[t];
-		}
+			s += v;
+		});
 		if (tid == 0)
 			*total = all;
 	} else {
 		int f = -1;
-		for (uint32_t t = b; t < e; ++t) f = max(f, (int)in[t]);
+		each8(in, b, e, [&](uint32_t, uint32_t v) { f = max(f, (int)v); });
 		int incl = f;
 #pragma unroll
 		for (int o = 1; o < kWave; o <<= 1) {
@@ -295,10 +309,10 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 		if (lane_id() == 0)
 			excl = -1;
 		int s = max(pre, excl);
-		for (uint32_t t = b; t < e; ++t) {
+		each8(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = (uint32_t)s;
-			s	   = max(s, (int)in[t]);
-		}
+			s	   = max(s, (int)v);
+		});
 	}
 }
 
@@ -426,13 +440,22 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_walk(KvArgs a) {
 // a first start then updates it, a structural byte updates it.  The k-th end
 // in a lane closes the token started k-th in it, or the open one before it.
 // ---------------------------------------------------------------------------
+// Tiles with at most this many token starts stage their rows in LDS and write
+// them out coalesced; denser tiles write each row from its lane.
+constexpr uint32_t kEmitStage = 3072;
+
 template<bool TOML>
 __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 	__shared__ uint2	s_hm[256];
 	__shared__ uint32_t s_b[kKvLanes * kStage];
 	__shared__ uint32_t s_r[5][kWaves];
 	__shared__ int		s_k[kWaves];
+	__shared__ uint8_t	s_role[kEmitStage];
+	__shared__ uint32_t s_beg[kEmitStage], s_end[kEmitStage];
 	const int			tid = threadIdx.x;
+	const uint64_t		t0 = a.t_off[blockIdx.x];
+	const uint32_t		tcnt = a.t_cnt[blockIdx.x];
+	const bool			staged = tcnt <= kEmitStage;  // the same in every lane
 	{
 		const Map8 h = hook_symbol<TOML>((uint32_t)tid);
 		s_hm[tid]	 = make_uint2(h.lo, h.hi);
@@ -448,7 +471,7 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 	const uint64_t end = valid ? a.l_end[lane] : 0ull, endq = valid ? a.l_endq[lane] : 0ull;
 	const uint32_t offw = valid ? a.l_off[lane] : 0u;
 	uint32_t	   h	= valid ? M8::apply(M8::unpack(a.l_hmap[lane]), a.t_hin[blockIdx.x]) : 0u;
-	const uint64_t tb	= (uint64_t)a.t_off[blockIdx.x] + (offw & 0x7FFFFFFFu);
+	const uint64_t tb	= t0 + (offw & 0x7FFFFFFFu);
 	uint64_t	   nr = 0;	// tokens per role, a byte each (a lane starts at most 64)
 	int			   lastkey = -1;
 	const uint64_t ntok	   = a.sum->n_tokens;
@@ -463,12 +486,17 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 			h				 = M8::apply(Map8 {t.x, t.y}, h);
 		} else {
 			const uint32_t r = hook_role<TOML>(h);
-			if (k >= ntok) {  // the passes disagree on the count: refuse, write nothing out of bounds
+			if (k >= ntok || k - t0 >= tcnt) {	// the passes disagree on a count: refuse, write nothing out of bounds
 				a.sum->bad = 1u;
 				break;
 			}
-			a.role[k] = (uint8_t)r;
-			a.beg[k]  = (uint32_t)(base + j);
+			if (staged) {
+				s_role[k - t0] = (uint8_t)r;
+				s_beg[k - t0]  = (uint32_t)(base + j);
+			} else {
+				a.role[k] = (uint8_t)r;
+				a.beg[k]  = (uint32_t)(base + j);
+			}
 			if (TOML && r == 1u)
 				lastkey = (int)k;
 			nr += 1ull << (8 * r);
@@ -480,11 +508,16 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 	uint64_t ke = tb - (offw >> 31);
 	for (uint64_t e = end; e; e &= e - 1) {
 		const int j = __builtin_ctzll(e);
-		if (ke >= ntok) {
+		if (ke >= ntok || (ke >= t0 && ke - t0 >= tcnt)) {
 			a.sum->bad = 1u;
 			break;
 		}
-		a.end[ke++] = (uint32_t)(base + j + ((endq >> j) & 1u));
+		const uint32_t e1 = (uint32_t)(base + j + ((endq >> j) & 1u));
+		if (staged && ke >= t0)
+			s_end[ke - t0] = e1;
+		else  // a token started in an earlier tile, or a dense tile
+			a.end[ke] = e1;
+		++ke;
 	}
 	if (valid && base + valid == a.n && a.sum->open_end && ntok)
 		a.end[ntok - 1] = (uint32_t)a.n;
@@ -502,6 +535,19 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 			s_k[w] = m;
 	}
 	__syncthreads();
+	if (staged) {
+		// the tile's last token may run on into the next tile (or to the
+		// document's end): its end is written by whoever meets it
+		const bool	   open_out = blockIdx.x + 1 < a.n_tiles ? a.t_tin[blockIdx.x + 1] >= T_PL : a.sum->open_end != 0;
+		const uint32_t n_end	= (open_out && tcnt) ? tcnt - 1 : tcnt;
+		const uint32_t n_row	= (uint32_t)min<uint64_t>(tcnt, ntok > t0 ? ntok - t0 : 0);
+		for (uint32_t i = tid; i < n_row; i += kKvLanes) {
+			a.role[t0 + i] = s_role[i];
+			a.beg[t0 + i]  = s_beg[i];
+			if (i < n_end)
+				a.end[t0 + i] = s_end[i];
+		}
+	}
 	if (tid < 5) {
 		uint32_t t = 0;
 		for (int q = 0; q < (int)kWaves; ++q) t += s_r[tid][q];
@@ -521,19 +567,27 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_emit(KvArgs a) {
 // no key came yet.  One block per tile's tokens, a max-scan in chunks.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kKvLanes) void k_kv_keys(KvArgs a) {
-	__shared__ int s_w[kWaves];
-	const int	   tid = threadIdx.x, w = tid / kWave;
-	const uint64_t b = a.t_off[blockIdx.x], e = b + a.t_cnt[blockIdx.x];
-	int			   carry = (int)a.t_kin[blockIdx.x];
-	for (uint64_t i0 = b; i0 < e; i0 += kKvLanes) {
-		const uint64_t i = i0 + tid;
-		const uint32_t r = i < e ? a.role[i] : 4u;
-		int			   v = r == 1u ? (int)i : -1;
+	constexpr uint32_t PER = 4;	 // consecutive tokens per thread
+	__shared__ int	   s_w[kWaves];
+	const int		   tid = threadIdx.x, w = tid / kWave;
+	const uint64_t	   b = a.t_off[blockIdx.x], e = b + a.t_cnt[blockIdx.x];
+	int				   carry = (int)a.t_kin[blockIdx.x];
+	for (uint64_t i0 = b; i0 < e; i0 += (uint64_t)kKvLanes * PER) {
+		const uint64_t i = i0 + (uint64_t)tid * PER;
+		uint32_t	   r[PER];
+		int			   mine = -1;  // this thread's last KEY
+#pragma unroll
+		for (uint32_t u = 0; u < PER; ++u) {
+			r[u] = i + u < e ? a.role[i + u] : 4u;
+			if (r[u] == 1u)
+				mine = (int)(i + u);
+		}
+		int v = mine;
 #pragma unroll
 		for (int o = 1; o < kWave; o <<= 1) {
-			const int u = __shfl_up(v, o, kWave);
+			const int q = __shfl_up(v, o, kWave);
 			if (lane_id() >= o)
-				v = max(v, u);
+				v = max(v, q);
 		}
 		if (lane_id() == kWave - 1)
 			s_w[w] = v;
@@ -544,8 +598,17 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_keys(KvArgs a) {
 				pre = max(pre, s_w[q]);
 			all = max(all, s_w[q]);
 		}
-		if (i < e)
-			a.key[i] = r == 2u ? max(pre, v) : -1;
+		int ex = __shfl_up(v, 1, kWave);  // last KEY of the wave's earlier threads
+		if (lane_id() == 0)
+			ex = -1;
+		int last = max(pre, ex);
+#pragma unroll
+		for (uint32_t u = 0; u < PER; ++u) {
+			if (i + u < e)
+				a.key[i + u] = r[u] == 2u ? last : -1;
+			if (r[u] == 1u)
+				last = (int)(i + u);
+		}
 		carry = all;
 		__syncthreads();
 	}

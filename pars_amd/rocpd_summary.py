@@ -9,6 +9,7 @@ counter (FETCH_SIZE / WRITE_SIZE are in KiB per dispatch).
 """
 from __future__ import annotations
 
+import os
 import sqlite3
 import sys
 from collections import defaultdict
@@ -20,6 +21,8 @@ def short(name: str, width: int = 70) -> str:
 
 
 def summarize(path: str) -> str:
+    if not os.path.isfile(path):
+        raise SystemExit(f"{path}: no such result database")
     con = sqlite3.connect(path)
     out = [f"# {path}"]
     names = {r[0] for r in con.execute("select name from sqlite_master where type in ('table', 'view')")}

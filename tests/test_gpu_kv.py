@@ -182,3 +182,16 @@ def test_reference_documents_at_64_mib_repeat_the_record(torch_cuda):
             key = np.array([t[3] for t in one], np.int64)
             want = np.where(key >= 0, key + np.arange(reps, dtype=np.int64)[:, None] * len(one), -1)
             assert np.array_equal(g["key"].reshape(reps, -1).astype(np.int64), want)
+
+
+@pytest.mark.parametrize("fmt", ["ini", "toml"])
+def test_dense_and_sparse_tiles_mixed(torch_cuda, fmt):
+    """Tiles with few token starts write their rows through LDS, dense ones
+    (more than 3072 starts in 16 KiB) from each lane: a document alternating
+    the two, with tokens running across the tiles between them."""
+    dense = (b"a=b c\n" if fmt == "ini" else b"a = [b, c]\n") * 6000
+    sparse = W.format_doc(fmt, 40_000).tobytes()
+    d = sparse + b"k = " + dense + b" long" * 5000 + sparse[:33_333] + dense + b"tail"
+    say(f"[gpu] kv {fmt} dense/sparse n={len(d)}")
+    check(torch_cuda, fmt, d)
+    check(torch_cuda, fmt, d, offset=9)

@@ -12,14 +12,18 @@
 //
 //   thread   32 consecutive brackets, the stack a bit mask of unmatched opens
 //            (the partner of a close is the highest set bit);
-//   chunk    256 threads = 8192 brackets, staged in LDS: a thread's unmatched
-//            closes sit at the levels [m, e) below its entry depth e, one per
-//            level, its unmatched opens at [m, x) below its exit depth x
-//            (m = its minimum depth).  The close at level k pairs with the
-//            open at level k of the last earlier thread whose minimum is <= k,
-//            found by binary lifting over a sparse table of the threads' minima
-//            in LDS; partners are collected in LDS, checked and stored
-//            coalesced;
+//   chunk    256 threads = 8192 brackets, staged in LDS as match keys (an
+//            open: a flag and the id of its closing byte; a close: its id): a
+//            thread's unmatched closes sit at the levels [m, e) below its entry
+//            depth e, one per level, its unmatched opens at [m, x) below its
+//            exit depth x (m = its minimum depth).  The close at level k pairs
+//            with the open at level k of the last earlier thread whose minimum
+//            is <= k.  Each thread finds its previous smaller minimum once
+//            (binary lifting over a sparse table of the minima in LDS); a
+//            thread's closes descend in level, so it walks that chain back
+//            from its neighbour, one LDS read per step.  Partners are collected
+//            in LDS, checked (keys equal but for the flag) and stored
+//            coalesced, four per thread and store for one document;
 //   document what no thread of a chunk matches is the chunk's residue - one
 //            close per level below its entry depth, one open per level below
 //            its exit depth - matched across chunks by one wave per chunk that
@@ -127,11 +131,12 @@ __device__ __forceinline__ int range_min(const int16_t (*st)[kStkThreads], int l
 __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 	__shared__ int16_t	s_st[kStkLevels][kStkThreads];	// thread minima relative to the chunk's entry depth
 	__shared__ uint32_t s_ro[kStkThreads];				// a thread's unmatched opens (bit per item)
-	__shared__ uint8_t	s_ch[kStkChunk];				// the chunk's bracket bytes
-	__shared__ uint16_t s_part[kStkChunk];				// the chunk's partners (chunk-local) or kPartLater
-	__shared__ uint8_t	s_flags[256], s_cof[256];
+	__shared__ __align__(16) uint8_t s_ch[kStkChunk];	// the chunk's brackets as match keys
+	__shared__ __align__(16) uint16_t s_part[kStkChunk];				// the chunk's partners (chunk-local) or kPartLater
+	__shared__ uint8_t	s_flags[256], s_cof[256], s_kid[256], s_kt[256];
 	__shared__ int32_t	s_wsum[kStkThreads / 64];
 	__shared__ int32_t	s_entry;
+	__shared__ uint32_t s_mp[kStkThreads];	// per thread: its minimum (int16) | (previous smaller + 1) << 16
 	if (!stack_applies(a))
 		return;
 	const int	   tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -142,6 +147,22 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 	const uint32_t epoch = a.sum->epoch;
 	s_flags[tid] = a.plan->br_flags[tid];
 	s_cof[tid]	 = a.plan->br_close_of[tid];
+	{
+		// a small id per closing byte value (its rank among them), then each
+		// byte's key: an open -> 0x80 | id of its close, a close -> its id;
+		// a pair matches iff the low 7 bits of its keys agree
+		const bool	   isc = (a.plan->br_flags[tid] & 2u) != 0;
+		const uint64_t bal = __ballot(isc);
+		if (lane == 0)
+			s_wsum[wid] = __popcll(bal);
+		__syncthreads();
+		uint32_t id = (uint32_t)__popcll(bal & lanemask_lt());
+		for (int q = 0; q < wid; ++q) id += (uint32_t)s_wsum[q];
+		s_kid[tid] = (uint8_t)min(id, 127u);
+		__syncthreads();
+		const uint32_t f = s_flags[tid];
+		s_kt[tid]		 = (f & 1u) ? (uint8_t)(0x80u | s_kid[s_cof[tid]]) : s_kid[tid];
+	}
 	bool bad = false, lost = false;	 // lost: a residue outside its chunk's range (cannot happen)
 	for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
 		const uint32_t base = c * kStkChunk, lim = min(nb - base, (uint32_t)kStkChunk);
@@ -152,9 +173,15 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		// ---- the chunk's bytes into LDS, coalesced ----
 		for (uint32_t o = (uint32_t)tid * 16u; o < lim; o += kStkThreads * 16u) {
 			if (o + 16u <= lim) {
-				*reinterpret_cast<uint4*>(s_ch + o) = *reinterpret_cast<const uint4*>(a.bchar + base + o);
+				const uint4 v = *reinterpret_cast<const uint4*>(a.bchar + base + o);
+				uint32_t	x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+				for (int q = 0; q < 4; ++q)
+					x[q] = (uint32_t)s_kt[x[q] & 0xFFu] | (uint32_t)s_kt[(x[q] >> 8) & 0xFFu] << 8 |
+						   (uint32_t)s_kt[(x[q] >> 16) & 0xFFu] << 16 | (uint32_t)s_kt[x[q] >> 24] << 24;
+				*reinterpret_cast<uint4*>(s_ch + o) = make_uint4(x[0], x[1], x[2], x[3]);
 			} else {
-				for (uint32_t q = o; q < lim; ++q) s_ch[q] = a.bchar[base + q];
+				for (uint32_t q = o; q < lim; ++q) s_ch[q] = s_kt[a.bchar[base + q]];
 			}
 		}
 		__syncthreads();
@@ -168,7 +195,7 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		uint32_t opn = 0;
 #pragma unroll
 		for (uint32_t q = 0; q < (uint32_t)kStkItems; ++q)
-			opn |= (uint32_t)(s_flags[(w[q >> 2] >> (8 * (q & 3))) & 0xFFu] & 1u) << q;
+			opn |= ((w[q >> 2] >> (8 * (q & 3) + 7)) & 1u) << q;
 		const uint32_t valid = cnt >= 32u ? ~0u : ((1u << cnt) - 1u);
 		opn &= valid;
 		const uint32_t cls	= valid & ~opn;
@@ -195,7 +222,7 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		if (lane == 63)
 			s_wsum[wid] = winc;
 		if (tid == 0)
-			s_entry = key0 + ((s_flags[w[0] & 0xFFu] & 1u) ? 0 : 1);	// depth before the first bracket
+			s_entry = key0 + ((w[0] & 0x80u) ? 0 : 1);	// depth before the first bracket
 		__syncthreads();
 		int32_t e = winc - delta, xc = 0;
 #pragma unroll
@@ -216,13 +243,29 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		}
 		const int32_t mc	= range_min(s_st, 0, kStkThreads - 1);	// <= 0: thread 0 enters at 0
 		const int32_t entry = s_entry;
+		{
+			// the last earlier thread whose minimum is below this one's (all threads at
+			// once): the closes below walk this chain instead of searching per level
+			const int32_t mt  = (int32_t)s_st[0][tid];
+			const int	  psm = last_at_most(s_st, tid, mt - 1);
+			s_mp[tid]		  = (uint32_t)(uint16_t)(int16_t)mt | (uint32_t)(psm + 1) << 16;
+		}
+		__syncthreads();
 		// ---- the thread's unmatched closes: an earlier thread's open, or the chunk's residue ----
 		{
-			uint32_t r = rcm;
+			uint32_t r	= rcm;
+			int		 u	= tid - 1;	// candidate: the last thread before t with minimum <= k
+			uint32_t wu = u >= 0 ? s_mp[u] : 0u;
 			for (uint32_t j = 0; r; ++j, r &= r - 1) {
 				const int32_t  k  = e - 1 - (int32_t)j;
 				const uint32_t lc = l0 + (uint32_t)(__ffs(r) - 1);
-				const int	   u  = last_at_most(s_st, tid, k);
+				// levels descend, so the answer only moves back: past a thread whose
+				// minimum is above k, every thread up to its previous smaller one has a
+				// minimum above k too
+				while (u >= 0 && (int32_t)(int16_t)(wu & 0xFFFFu) > k) {
+					u  = (int)(wu >> 16) - 1;
+					wu = u >= 0 ? s_mp[u] : 0u;
+				}
 				if (u >= 0) {
 					const int32_t jo = k - (int32_t)s_st[0][u];
 					if (jo < 0 || jo >= __popc(s_ro[u])) {	// cannot happen: the open at level k
@@ -270,13 +313,35 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 		// ---- the chunk's partners, checked and stored coalesced: a pair whose bytes do
 		// not match is -1 on both sides (nest.hpp:80-88); residues get -1 until the
 		// resolve pass ----
+		uint32_t l_from = (uint32_t)tid;
+		if (!a.bdoc && !a.pair_pos) {
+			// one document without pair_pos: four partners per thread, one 16-byte store
+			const uint32_t l4 = lim & ~3u;
+#pragma unroll 2
+			for (uint32_t l = (uint32_t)tid * 4u; l < l4; l += kStkThreads * 4u) {
+				const uint2	   pv2 = *reinterpret_cast<const uint2*>(s_part + l);
+				const uint32_t kl  = *reinterpret_cast<const uint32_t*>(s_ch + l);
+				const uint32_t v4[4] = {pv2.x & 0xFFFFu, pv2.x >> 16, pv2.y & 0xFFFFu, pv2.y >> 16};
+				int32_t		   o[4];
+#pragma unroll
+				for (int q = 0; q < 4; ++q) {
+					o[q] = -1;
+					if (v4[q] < kStkChunk) {
+						const bool ok = (((kl >> (8 * q)) ^ s_ch[v4[q]]) & 0x7Fu) == 0u;
+						bad			  = bad || !ok;
+						o[q]		  = ok ? (int32_t)(base + v4[q]) : -1;
+					}
+				}
+				*reinterpret_cast<int4*>(a.pair + base + l) = make_int4(o[0], o[1], o[2], o[3]);
+			}
+			l_from = l4 + (uint32_t)tid;
+		}
 #pragma unroll 4
-		for (uint32_t l = (uint32_t)tid; l < lim; l += kStkThreads) {
+		for (uint32_t l = l_from; l < lim; l += kStkThreads) {
 			const uint16_t v  = s_part[l];
 			int32_t		   pv = -1;
 			if (v < (uint16_t)kStkChunk) {
-				const uint32_t cl = s_ch[l], cv = s_ch[v];
-				const bool	   ok = (s_flags[cl] & 1u) ? s_cof[cl] == cv : s_cof[cv] == cl;
+				const bool ok = ((s_ch[l] ^ s_ch[v]) & 0x7Fu) == 0u;
 				bad				  = bad || !ok;
 				pv				  = ok ? (int32_t)(base + v) : -1;
 				if (!ok && a.doc_bad)
@@ -398,7 +463,11 @@ hipError_t launch_stack_pairing(const StackArgs& a, hipStream_t s) {
 	(void)hipGetDevice(&dev);
 	(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	cus					 = std::max(cus, 1);
-	const unsigned lgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(a.max_chunks, 1), 8ull * cus);
+	// as many blocks as stay resident, so every block takes the same share of chunks
+	int occ = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stack_local, kStkThreads, 0) != hipSuccess || occ < 1)
+		occ = 4;
+	const unsigned lgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(a.max_chunks, 1), (uint64_t)occ * cus);
 	const unsigned rgrid = (unsigned)std::min<uint64_t>(
 		std::max<uint64_t>((a.max_chunks + kResolveBlock / 64 - 1) / (kResolveBlock / 64), 1), 8ull * cus);
 	hipLaunchKernelGGL(k_stack_local, dim3(lgrid), dim3(kStkThreads), 0, s, a);

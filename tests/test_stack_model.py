@@ -1,7 +1,8 @@
 """CPU model of the stack pairing path (pars_amd/csrc/pair_stack.hip), index
 for index: threads of T brackets with a bit-mask stack, chunks of W threads
 whose unmatched closes find the last earlier thread with a small enough
-minimum depth by binary lifting over a sparse table, chunk residues laid out
+minimum depth by walking each thread's previous-smaller chain (found once per
+thread by binary lifting over a sparse table), chunk residues laid out
 as the kernels lay them out (closes from the entry level down, then opens from
 the minimum up) and resolved across chunks by walking back to the nearest
 chunk whose minimum is below the highest open level, skipping groups of G and
@@ -96,11 +97,17 @@ def stack_pairs(chars: np.ndarray, T: int, W: int, G: int = 64):
         assert mc <= 0
         first = int(chars[base])
         entry = int(depth_before[base])
+        # each thread's previous smaller minimum (one lifting per thread); a
+        # thread's closes descend in level and walk that chain back
+        psm = [last_at_most(st, t, m[t] - 1) for t in range(W)]
         for t in range(W):
             i0, cnt, rcl, rol = th[t]
+            u = t - 1
             for j, q in enumerate(rcl):
                 k = int(e[t]) - 1 - j
-                u = last_at_most(st, t, k)
+                while u >= 0 and m[u] > k:
+                    u = psm[u]
+                assert u == last_at_most(st, t, k)
                 gc = i0 + q
                 if u >= 0:
                     jo = k - m[u]

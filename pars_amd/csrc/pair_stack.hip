@@ -20,8 +20,10 @@
 //            with the open at level k of the last earlier thread whose minimum
 //            is <= k.  Each thread finds its previous smaller minimum once
 //            (binary lifting over a sparse table of the minima in LDS); a
-//            thread's closes descend in level, so it walks that chain back
-//            from its neighbour, one LDS read per step.  Partners are collected
+//            thread's closes descend in level, so they walk that chain back
+//            from its neighbour, one LDS read per step.  The chunk's unmatched
+//            closes (0 to 32 per thread) are listed and spread evenly over the
+//            threads, each taking a run of the list.  Partners are collected
 //            in LDS, checked (keys equal but for the flag) and stored
 //            coalesced, four per thread and store for one document;
 //   document what no thread of a chunk matches is the chunk's residue - one
@@ -55,7 +57,8 @@ constexpr int	  kStkItems	  = 32;
 constexpr int	  kStkLevels  = 8;	// 2^7 = 128: lifting and range minima over 256 threads
 constexpr int16_t kNoMin	  = INT16_MAX;	// a thread without brackets
 constexpr uint16_t kPartLater = 0xFFFEu;	// a chunk residue: the resolve pass writes it
-constexpr int	  kResolveBlock = 256;		// four chunks per block in the resolve pass
+constexpr int	  kResolveBlock = 256;
+constexpr uint32_t kListCap = kStkLevels * kStkThreads / 2;	// unmatched closes listed per chunk		// four chunks per block in the resolve pass
 static_assert(kStkThreads * kStkItems == (int)kStkChunk, "chunk = threads x items");
 
 __device__ __forceinline__ uint32_t stack_nb(const StackArgs& a) {
@@ -129,7 +132,8 @@ __device__ __forceinline__ int range_min(const int16_t (*st)[kStkThreads], int l
 }
 
 __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
-	__shared__ int16_t	s_st[kStkLevels][kStkThreads];	// thread minima relative to the chunk's entry depth
+	__shared__ __align__(16) int16_t s_st[kStkLevels][kStkThreads];	// thread minima relative to the chunk's entry depth
+	__shared__ int16_t	s_e[kStkThreads];	// entry depths, for the list of unmatched closes
 	__shared__ uint32_t s_ro[kStkThreads];				// a thread's unmatched opens (bit per item)
 	__shared__ __align__(16) uint8_t s_ch[kStkChunk];	// the chunk's brackets as match keys
 	__shared__ __align__(16) uint16_t s_part[kStkChunk];				// the chunk's partners (chunk-local) or kPartLater
@@ -242,6 +246,7 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 			__syncthreads();
 		}
 		const int32_t mc	= range_min(s_st, 0, kStkThreads - 1);	// <= 0: thread 0 enters at 0
+		const int32_t suf	= tid + 1 < kStkThreads ? range_min(s_st, tid + 1, kStkThreads - 1) : INT_MAX;
 		const int32_t entry = s_entry;
 		{
 			// the last earlier thread whose minimum is below this one's (all threads at
@@ -250,45 +255,83 @@ __global__ __launch_bounds__(kStkThreads) void k_stack_local(StackArgs a) {
 			const int	  psm = last_at_most(s_st, tid, mt - 1);
 			s_mp[tid]		  = (uint32_t)(uint16_t)(int16_t)mt | (uint32_t)(psm + 1) << 16;
 		}
-		__syncthreads();
-		// ---- the thread's unmatched closes: an earlier thread's open, or the chunk's residue ----
-		{
-			uint32_t r	= rcm;
-			int		 u	= tid - 1;	// candidate: the last thread before t with minimum <= k
-			uint32_t wu = u >= 0 ? s_mp[u] : 0u;
-			for (uint32_t j = 0; r; ++j, r &= r - 1) {
-				const int32_t  k  = e - 1 - (int32_t)j;
-				const uint32_t lc = l0 + (uint32_t)(__ffs(r) - 1);
-				// levels descend, so the answer only moves back: past a thread whose
-				// minimum is above k, every thread up to its previous smaller one has a
-				// minimum above k too
-				while (u >= 0 && (int32_t)(int16_t)(wu & 0xFFFFu) > k) {
-					u  = (int)(wu >> 16) - 1;
-					wu = u >= 0 ? s_mp[u] : 0u;
-				}
-				if (u >= 0) {
-					const int32_t jo = k - (int32_t)s_st[0][u];
-					if (jo < 0 || jo >= __popc(s_ro[u])) {	// cannot happen: the open at level k
-						lost = true;
-						continue;
-					}
-					const uint32_t lo = (uint32_t)u * kStkItems + select_bit(s_ro[u], (uint32_t)jo);
-					s_part[lc]		  = (uint16_t)lo;
-					s_part[lo]		  = (uint16_t)lc;
-				} else {
-					const uint32_t idx = (uint32_t)(-1 - k);  // closes: level entry-1, entry-2, ...
-					if (k < 0 && idx < lim)
-						a.res[base + idx] = base + lc;
-					else
-						lost = true;
-					s_part[lc] = kPartLater;
-				}
+		const uint32_t rinc = wave_incl_sum(rc);
+		if (lane == 63)
+			s_wsum[wid] = (int32_t)rinc;
+		__syncthreads();  // the chain is complete and the sparse table free
+		uint32_t roff = rinc - rc, rtot = 0;
+#pragma unroll
+		for (int q = 0; q < kStkThreads / 64; ++q) {
+			if (q < wid)
+				roff += (uint32_t)s_wsum[q];
+			rtot += (uint32_t)s_wsum[q];
+		}
+		// The close at level k (entry depth e, its j-th unmatched close, k = e - 1 - j)
+		// pairs with the open at level k of the last thread before t whose minimum is
+		// <= k, or is the chunk's residue.  u, wu: the candidate and its chain word;
+		// levels only descend from one close to the next of a thread, so the answer
+		// only moves back along the chain.
+		auto close_at = [&](int32_t k, uint32_t lc, int& u, uint32_t& wu) {
+			while (u >= 0 && (int32_t)(int16_t)(wu & 0xFFFFu) > k) {
+				u  = (int)(wu >> 16) - 1;
+				wu = u >= 0 ? s_mp[u] : 0u;
 			}
+			if (u >= 0) {
+				const int32_t jo = k - (int32_t)(int16_t)(wu & 0xFFFFu);
+				if (jo < 0 || jo >= __popc(s_ro[u])) {	// cannot happen: the open at level k
+					lost = true;
+					return;
+				}
+				const uint32_t lo = (uint32_t)u * kStkItems + select_bit(s_ro[u], (uint32_t)jo);
+				s_part[lc]		  = (uint16_t)lo;
+				s_part[lo]		  = (uint16_t)lc;
+			} else {
+				const uint32_t idx = (uint32_t)(-1 - k);  // closes: level entry-1, entry-2, ...
+				if (k < 0 && idx < lim)
+					a.res[base + idx] = base + lc;
+				else
+					lost = true;
+				s_part[lc] = kPartLater;
+			}
+		};
+		// ---- the unmatched closes, spread evenly over the threads through a list in
+		// the sparse table's space (a thread's own can number 0 to 32) ----
+		uint32_t* s_list = reinterpret_cast<uint32_t*>(&s_st[0][0]);
+		const bool listed = rtot <= kListCap;  // the same in every lane
+		if (listed) {
+			uint32_t r = rcm;
+			for (uint32_t j = 0; r; ++j, r &= r - 1)
+				s_list[roff + j] = (l0 + (uint32_t)(__ffs(r) - 1)) | j << 13 | (uint32_t)tid << 18;
+			s_e[tid] = (int16_t)e;
+		}
+		__syncthreads();
+		if (listed) {
+			const uint32_t per = (rtot + kStkThreads - 1) / kStkThreads;
+			const uint32_t i0 = min(rtot, (uint32_t)tid * per), i1 = min(rtot, i0 + per);
+			int			   t_cur = -1, u = -1;
+			uint32_t	   wu = 0;
+			int32_t		   et = 0;
+			for (uint32_t i = i0; i < i1; ++i) {
+				const uint32_t w = s_list[i];
+				const int	   t = (int)(w >> 18);
+				if (t != t_cur) {
+					t_cur = t;
+					et	  = s_e[t];
+					u	  = t - 1;
+					wu	  = u >= 0 ? s_mp[u] : 0u;
+				}
+				close_at(et - 1 - (int32_t)((w >> 13) & 31u), w & 0x1FFFu, u, wu);
+			}
+		} else {
+			uint32_t r	= rcm;
+			int		 u	= tid - 1;
+			uint32_t wu = u >= 0 ? s_mp[u] : 0u;
+			for (uint32_t j = 0; r; ++j, r &= r - 1)
+				close_at(e - 1 - (int32_t)j, l0 + (uint32_t)(__ffs(r) - 1), u, wu);
 		}
 		// ---- the thread's unmatched opens no later thread closes: the chunk's residue ----
 		{
-			const int32_t suf = tid + 1 < kStkThreads ? range_min(s_st, tid + 1, kStkThreads - 1) : INT_MAX;
-			uint32_t	  r	  = mask;
+			uint32_t r = mask;
 			for (uint32_t j = 0; r; ++j, r &= r - 1) {
 				const int32_t k = m + (int32_t)j;
 				if (k < suf) {

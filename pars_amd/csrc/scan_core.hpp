@@ -11,7 +11,9 @@
 //   3. masks -> unmasked structural / bracket bits; counts and the +-1 depth
 //      delta; block scans give each lane's output offsets inside the tile;
 //   4. row-local positions and bracket records staged in LDS per row and
-//      written with coalesced stores; the bitmap word per lane.
+//      written with coalesced stores (four consecutive records per thread,
+//      16-byte stores on the aligned part of each run); the bitmap word per
+//      lane.
 //
 // Chains.  A plan with one descriptor (JSON, CSV, every single-automaton
 // grammar) runs ONE chain: a tile publishes its state map together with its
@@ -1114,6 +1116,20 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 					 else
 						 return (uint32_t)(pbase + off);
 				};
+				if (!a.chars) {
+					// four consecutive positions per thread, 16-byte stores where the
+					// output is 16-byte aligned; a head and a tail one by one
+					const uint64_t g0 = rb[0];
+					const uint32_t hn = min((uint32_t)((4u - (uint32_t)(g0 & 3u)) & 3u), mp);
+					if ((uint32_t)tid < hn)
+						a.positions[g0 + tid] = out_pos(s_pos[tid]);
+					for (uint32_t i = hn + 4u * (uint32_t)tid; i + 4u <= mp; i += 4u * NT)
+						*reinterpret_cast<uint4*>(a.positions + g0 + i) =
+							make_uint4(out_pos(s_pos[i]), out_pos(s_pos[i + 1]), out_pos(s_pos[i + 2]), out_pos(s_pos[i + 3]));
+					const uint32_t body = hn + ((mp - hn) & ~3u);
+					if ((uint32_t)tid < mp - body)
+						a.positions[g0 + body + tid] = out_pos(s_pos[body + tid]);
+				} else
 				for (uint32_t i0 = 0; i0 < mp; i0 += U * NT) {
 					uint8_t ch[U];
 					if (a.chars)
@@ -1133,6 +1149,53 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 					}
 				}
 				const int32_t kb = BATCH ? 0 : (int32_t)rb[2];
+				bool		  bdone = false;
+				if constexpr (E::kBitParallel) {
+					if (bit_chr && !want_sidx) {
+						// four consecutive bracket records per thread: positions in one
+						// 16-byte store, bytes in one 4-byte store (aligned part), keys
+						// only at the stack chunks' starts
+						const uint64_t g0	= rb[1];
+						const uint32_t hn	= min((uint32_t)((4u - (uint32_t)(g0 & 3u)) & 3u), mb);
+						auto		   one	= [&](uint32_t i) {
+							  const uint64_t g = g0 + i;
+							  a.bpos[g]		   = out_pos(s_bpos[i]);
+							  if (all_keys || (g & (kStkChunk - 1u)) == 0)
+								  a.bkey[g] = kb + (int32_t)s_bkey[i];
+							  a.bchar[g] = s_bch[i];
+							  if constexpr (BATCH)
+								  a.b.bdoc[g] = (uint32_t)d0 + slot_of(r * NT + (int)(s_bpos[i] >> 6));
+						};
+						if ((uint32_t)tid < hn)
+							one((uint32_t)tid);
+						for (uint32_t i = hn + 4u * (uint32_t)tid; i + 4u <= mb; i += 4u * NT) {
+							const uint64_t g = g0 + i;
+							*reinterpret_cast<uint4*>(a.bpos + g) = make_uint4(
+								out_pos(s_bpos[i]), out_pos(s_bpos[i + 1]), out_pos(s_bpos[i + 2]), out_pos(s_bpos[i + 3]));
+							*reinterpret_cast<uint32_t*>(a.bchar + g) =
+								(uint32_t)s_bch[i] | (uint32_t)s_bch[i + 1] << 8 | (uint32_t)s_bch[i + 2] << 16 |
+								(uint32_t)s_bch[i + 3] << 24;
+							if (all_keys) {
+#pragma unroll
+								for (int u = 0; u < 4; ++u) a.bkey[g + u] = kb + (int32_t)s_bkey[i + u];
+							} else if ((g & (kStkChunk - 1u)) == 0) {  // a chunk start is a multiple of 4
+								a.bkey[g] = kb + (int32_t)s_bkey[i];
+							}
+							if constexpr (BATCH) {
+								uint32_t d[4];
+#pragma unroll
+								for (int u = 0; u < 4; ++u)
+									d[u] = (uint32_t)d0 + slot_of(r * NT + (int)(s_bpos[i + u] >> 6));
+								*reinterpret_cast<uint4*>(a.b.bdoc + g) = make_uint4(d[0], d[1], d[2], d[3]);
+							}
+						}
+						const uint32_t body = hn + ((mb - hn) & ~3u);
+						if ((uint32_t)tid < mb - body)
+							one(body + (uint32_t)tid);
+						bdone = true;
+					}
+				}
+				if (!bdone)
 				for (uint32_t i0 = 0; i0 < mb; i0 += U * NT) {
 					uint8_t ch[U];
 #pragma unroll

@@ -49,8 +49,9 @@ typedef enum {
 
 /* pipeline.cuh:745-748: per-stage milliseconds measured with events on the
  * call's own stream.  p1 = R1-R5 (one fused kernel: the stages have no
- * separate launches), p2 = the JSON tape's kernels (pars_hip_json_tape; else
- * 0), p3 = 0, p4 = R6 pairing (all of its kernels; a kernel trace splits them). */
+ * separate launches), p2 = the stage-2 kernels (the JSON tape's,
+ * pars_hip_json_tape, or the key-value table's, pars_hip_kv_parse; else 0),
+ * p3 = 0, p4 = R6 pairing (all of its kernels; a kernel trace splits them). */
 typedef struct {
 	double p1, p2, p3, p4;
 } pars_stage_times;

@@ -191,11 +191,13 @@ int pars_hip_scanner_create_ex(const pars_plan* plan, const pars_allocator* allo
 int pars_hip_scanner_destroy(pars_scanner* sc);
 int pars_hip_set_option(pars_scanner* sc, int option, int64_t value);
 
-/* pipeline.cuh:1167-1172 scanner::scan_device.  d_in: device bytes [0, n).
- * want_pairs: fill pair_pos (structural-index convention) and bracket_pair.
- * run_pairs: run R6 (flags stay as computed by the depth scan otherwise).
- * st: optional stage times.  Blocks until results are ready (like the
- * reference, which returns counts by value). */
+/* pipeline.cuh:1167-1172 scanner::scan_device.  d_in: device bytes [0, n),
+ * n < 2^32.  want_pairs: fill pair_pos (structural-index convention) and
+ * bracket_pair.  run_pairs: run R6 (flags stay as computed by the depth scan
+ * otherwise); pairs are int32 bracket indices, so a document with more than
+ * 2^31-1 brackets is refused with pairing (PARS_ERR_ARGUMENT).  st: optional
+ * stage times.  Blocks until results are ready (like the reference, which
+ * returns counts by value). */
 int pars_hip_scan_device(
 	pars_scanner* sc, const uint8_t* d_in, size_t n, pars_stream stream, int want_pairs, int run_pairs,
 	pars_stage_times* st, pars_device_result* out
@@ -457,7 +459,8 @@ typedef struct {
  * pairing) and build its key-value table there: the reference's full-parse
  * benchmark for these formats (bench_parse_host.cu:250-333: scan_host, replay
  * into the hook) without the host replay.  Every array is device memory, valid
- * until the scanner's next call.  At most 2^32-1 bytes.  Blocks until done;
+ * until the scanner's next call.  At most 2^32-1 bytes (TOML: and 2^31-1
+ * tokens, as keys are int32 token indices).  Blocks until done;
  * st->p1 = R1-R5, p2 = the table's kernels (ms), when st is not NULL. */
 int pars_hip_kv_parse(
 	pars_scanner* sc, const uint8_t* d_in, size_t n, int format, pars_stream stream, pars_stage_times* st,

@@ -1025,6 +1025,9 @@ int pars_hip_scan_device(pars_scanner* sc, const uint8_t* d_in, size_t n, pars_s
 		if ((r = ensure_outputs(sc, n_tiles, sc->cap_struct, sc->h_sum->n_brackets, want_pairs != 0)))
 			return r;
 	}
+	if (run_pairs && sc->h_sum->n_brackets > (uint32_t)INT32_MAX)	// pairs are int32 bracket indices
+		return fail(PARS_ERR_ARGUMENT, "pairing holds at most 2^31-1 brackets (%u here): scan without pairs",
+					sc->h_sum->n_brackets);
 	if (st) {
 		float ms = 0;
 		if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
@@ -1666,6 +1669,9 @@ int pars_hip_scan_stream(pars_scanner* sc, const uint8_t* h_in, size_t n, size_t
 		if ((r = ensure_outputs(sc, n_tiles, sc->cap_struct, sc->h_sum->n_brackets, want_pairs != 0)))
 			return r;
 	}
+	if (sc->h_sum->n_brackets > (uint32_t)INT32_MAX)	// pairs are int32 bracket indices
+		return fail(PARS_ERR_ARGUMENT, "pairing holds at most 2^31-1 brackets (%u here): scan without pairs",
+					sc->h_sum->n_brackets);
 	if (st) {
 		float ms = 0;
 		if (hipEventElapsedTime(&ms, sc->ev[0], sc->ev[1]) == hipSuccess)
@@ -1868,6 +1874,9 @@ int pars_hip_kv_parse(pars_scanner* sc, const uint8_t* d_in, size_t n, int forma
 	const uint64_t ntok = sc->h_kv->n_tokens;
 	if (ntok > n)
 		return fail(PARS_ERR_DEVICE, "more tokens than bytes (internal error)");
+	if (toml && ntok > (uint64_t)INT32_MAX)	 // keys are int32 token indices
+		return fail(PARS_ERR_ARGUMENT, "a TOML key-value table holds at most 2^31-1 tokens (%llu here)",
+					(unsigned long long)ntok);
 	const uint64_t cap = std::max<uint64_t>(ntok, 1);
 	if ((r = sc->kv_role.ensure(cap)) || (r = sc->kv_beg.ensure(cap * 4)) || (r = sc->kv_end.ensure(cap * 4)) ||
 		(toml && (r = sc->kv_key.ensure(cap * 4))))

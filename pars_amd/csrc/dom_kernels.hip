@@ -276,6 +276,32 @@ __device__ __forceinline__ void load_gap_tables(GapTables& L) {
 	__syncthreads();
 }
 
+// Bytes of a dword that end a string's plain run - '"', '\\' or a byte below
+// 0x20 - as bit 7 of each byte.  Exact for the lowest such byte (a borrow can
+// flag a byte only above a truly flagged one).
+__host__ __device__ __forceinline__ uint32_t str_stops(uint32_t x) {
+	const uint32_t q = x ^ 0x22222222u, b = x ^ 0x5C5C5C5Cu;
+	return (((q - 0x01010101u) & ~q) | ((b - 0x01010101u) & ~b) | ((x - 0x20202020u) & ~x)) & 0x80808080u;
+}
+
+// How many bytes from j on (before e) continue a string unchanged (S_STR to
+// S_STR, one decoded byte each), as far as the window that holds j reaches; 0
+// when j is not in a window (the buffer's edges, read byte by byte).
+__host__ __device__ __forceinline__ uint32_t str_run(Bytes& d, uint32_t j, uint32_t e) {
+	(void)d[j];	 // the window holding j, when there is one
+	const uint32_t k = j - d.base;
+	if (k >= 32u)
+		return 0u;
+	const uint32_t w[8] = {d.v0.x, d.v0.y, d.v0.z, d.v0.w, d.v1.x, d.v1.y, d.v1.z, d.v1.w};
+	uint32_t	   m	= 0;  // bit i: window byte i ends a run
+#pragma unroll
+	for (int q = 0; q < 8; ++q) m |= (((str_stops(w[q]) >> 7) * 0x00204081u) >> 21 & 0xFu) << (4 * q);
+	m &= ~0u << k;
+	const uint32_t stop = m ? (uint32_t)__builtin_ctz(m) : 32u;	 // window-relative
+	const uint32_t lim	= e - d.base < 32u ? e - d.base : 32u;
+	return (stop < lim ? stop : lim) - k;
+}
+
 // kind, whether it parsed with only whitespace after it, its first byte, one
 // past its last byte (0 for a string that did not close), and the decoded
 // bytes a string mixed into the hash.  EMIT writes those bytes to `out`.
@@ -285,6 +311,15 @@ __host__ __device__ __forceinline__ Gap parse_gap(const GapTables& L, Bytes& d, 
 	uint32_t s = S_LEAD, dec = 0, v = 0, first = K_OTHER;
 	bool	 done = false;
 	for (uint32_t j = b;; ++j) {  // through j == e, the gap's end class
+		if constexpr (!EMIT) {
+			// inside a string, a run of plain bytes only counts decoded bytes:
+			// skip to the run's end (the window's end at most)
+			if (s == S_STR && j < e) {
+				const uint32_t r = str_run(d, j, e);
+				j += r;
+				dec += r;
+			}
+		}
 		const bool	   end = j == e;
 		const uint32_t c   = end ? 0u : d[j];
 		const uint32_t k   = end ? (uint32_t)K_END : (uint32_t)L.cls[c];

@@ -59,6 +59,13 @@ def test_gap_parser_matches_oracle(tmp_path):
     gaps = [b"".join(rng.choice(parts) for _ in range(rng.randint(0, 8))) for _ in range(20000)]
     gaps += [b'"' + bytes(rng.randrange(32, 256) for _ in range(rng.randint(0, 80))).replace(b'"', b"x")
              .replace(b"\\", b"y") + b'"' for _ in range(1000)]
+    # long strings whose plain runs (skipped a window at a time in the count
+    # pass) end at every kind of byte that stops them, at every window offset
+    stops = [b'\\n', b'\\"', b"\\u00e9", b"\x01", b"\t", b"\x1f", b'"', b"\\"]
+    for _ in range(3000):
+        body = b"".join(bytes(rng.randrange(32, 256) for _ in range(rng.randint(0, 70))).replace(b'"', b"x")
+                        .replace(b"\\", b"y") + rng.choice(stops) for _ in range(rng.randint(1, 4)))
+        gaps.append(b" " * rng.randint(0, 40) + b'"' + body + (b'"' if rng.random() < 0.8 else b"") + b" " * rng.randint(0, 3))
     path = tmp_path / "gaps.bin"
     with open(path, "wb") as f:
         for g in gaps:

@@ -7,6 +7,8 @@ and a server bounds what they hold together.  Results through a caller's
 allocator or under a budget equal the oracle's."""
 from __future__ import annotations
 
+import gc
+
 import numpy as np
 import pytest
 
@@ -52,6 +54,8 @@ def test_buffers_from_the_torch_caching_allocator(torch_cuda):
     torch = torch_cuda
     doc = W.json(8, 3 << 20, 10)
     ta = TorchAllocator()
+    gc.collect()  # tensors earlier tests left for the collector go now, not part way through
+    torch.cuda.synchronize()
     before = torch.cuda.memory_allocated()
     sc = Scanner("json", allocator=ta)
     sc.set_option(N.PARS_OPT_WANT_CHARS, 1)

@@ -25,8 +25,8 @@
 // across lines) is a max-scan over the table.
 //
 // Passes: k_kv_tokmap (lane tokenizer maps) -> k_kv_chain (tile entry states)
-// -> k_kv_walk (masks, counts, lane hook maps) -> k_kv_chain x2 (tile hook
-// states, token offsets) | host sizes the table | k_kv_emit (roles, begins,
+// -> k_kv_walk (masks, counts, lane hook maps) -> k_kv_chain, two blocks
+// (tile hook states, token offsets) | host sizes the table | k_kv_emit (roles, begins,
 // ends) -> TOML: k_kv_chain (last key before each tile) -> k_kv_keys.
 #include "dev_common.hpp"
 #include "kernels.hpp"
@@ -232,22 +232,39 @@ __global__ __launch_bounds__(kKvLanes) void k_kv_tokmap(KvArgs a) {
 constexpr int kChainThreads = 1024;
 enum { CHAIN_MAPS = 0, CHAIN_SUMS = 1, CHAIN_MAX = 2 };
 
-// f(t, in[t]) for t in [b, e) in order, eight loads issued per round trip.
+// f(t, in[t]) for t in [b, e) in order, 32 loads issued per round trip.
 template<class F>
-__device__ __forceinline__ void each8(const uint32_t* in, uint32_t b, uint32_t e, F&& f) {
-	for (uint32_t t = b; t < e; t += 8) {
-		uint32_t v[8];
+__device__ __forceinline__ void each32(const uint32_t* in, uint32_t b, uint32_t e, F&& f) {
+	constexpr uint32_t K = 32;
+	for (uint32_t t = b; t < e; t += K) {
+		uint32_t v[K];
 #pragma unroll
-		for (int u = 0; u < 8; ++u) v[u] = t + u < e ? in[t + u] : 0u;
+		for (uint32_t u = 0; u < K; ++u) v[u] = t + u < e ? in[t + u] : 0u;
 #pragma unroll
-		for (int u = 0; u < 8; ++u)
+		for (uint32_t u = 0; u < K; ++u)
 			if (t + u < e)
 				f(t + u, v[u]);
 	}
 }
 
-__global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, uint32_t* out, uint32_t T, int kind,
-															 uint32_t init, uint64_t* total) {
+// One scan over the tiles: what k_kv_chain computes (see there).
+struct ChainJob {
+	const uint32_t* in;
+	uint32_t*		out;
+	uint32_t		T;
+	int				kind;
+	uint32_t		init;
+	uint64_t*		total;
+};
+
+// Independent scans run as the blocks of one launch (block i: job i).
+__global__ __launch_bounds__(kChainThreads) void k_kv_chain(ChainJob j0, ChainJob j1) {
+	const bool		   b0	 = blockIdx.x == 0;
+	const uint32_t*	   in	 = b0 ? j0.in : j1.in;
+	uint32_t*		   out	 = b0 ? j0.out : j1.out;
+	const uint32_t	   T	 = b0 ? j0.T : j1.T, init = b0 ? j0.init : j1.init;
+	const int		   kind	 = b0 ? j0.kind : j1.kind;
+	uint64_t*		   total = b0 ? j0.total : j1.total;
 	constexpr int	   NW = kChainThreads / kWave;
 	__shared__ Map8	   s_m[NW];
 	__shared__ uint32_t s_v[NW];
@@ -256,7 +273,7 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 	const uint32_t	   b = min(T, (uint32_t)tid * per), e = min(T, b + per);
 	if (kind == CHAIN_MAPS) {
 		Map8 f = M8::id();
-		each8(in, b, e, [&](uint32_t, uint32_t v) { f = M8::compose(M8::unpack(v), f); });
+		each32(in, b, e, [&](uint32_t, uint32_t v) { f = M8::compose(M8::unpack(v), f); });
 		const Map8 incl = wave_scan_maps<true>(f);
 		const Map8 excl = wave_shift_maps<true>(incl);
 		if (lane_id() == kWave - 1)
@@ -265,13 +282,13 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 		Map8 pre = M8::id();
 		for (int k = 0; k < w; ++k) pre = M8::compose(s_m[k], pre);
 		uint32_t s = M8::apply(M8::compose(excl, pre), init);
-		each8(in, b, e, [&](uint32_t t, uint32_t v) {
+		each32(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = s;
 			s	   = M8::apply(M8::unpack(v), s);
 		});
 	} else if (kind == CHAIN_SUMS) {
 		uint64_t f = 0;
-		each8(in, b, e, [&](uint32_t, uint32_t v) { f += v; });
+		each32(in, b, e, [&](uint32_t, uint32_t v) { f += v; });
 		// the document has fewer than 2^32 bytes, so fewer tokens
 		const uint32_t incl = wave_incl_sum((uint32_t)f);
 		if (lane_id() == kWave - 1)
@@ -284,7 +301,7 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 			all += s_v[k];
 		}
 		uint32_t s = pre + incl - (uint32_t)f;
-		each8(in, b, e, [&](uint32_t t, uint32_t v) {
+		each32(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = s;
 			s += v;
 		});
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 			*total = all;
 	} else {
 		int f = -1;
-		each8(in, b, e, [&](uint32_t, uint32_t v) { f = max(f, (int)v); });
+		each32(in, b, e, [&](uint32_t, uint32_t v) { f = max(f, (int)v); });
 		int incl = f;
 #pragma unroll
 		for (int o = 1; o < kWave; o <<= 1) {
@@ -309,7 +326,7 @@ __global__ __launch_bounds__(kChainThreads) void k_kv_chain(const uint32_t* in, 
 		if (lane_id() == 0)
 			excl = -1;
 		int s = max(pre, excl);
-		each8(in, b, e, [&](uint32_t t, uint32_t v) {
+		each32(in, b, e, [&](uint32_t t, uint32_t v) {
 			out[t] = (uint32_t)s;
 			s	   = max(s, (int)v);
 		});
@@ -620,16 +637,17 @@ hipError_t launch_kv_count(const KvArgs& a, hipStream_t s) {
 	if (a.n_tiles == 0)
 		return hipSuccess;
 	hipLaunchKernelGGL(k_kv_tokmap, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);
-	hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s, a.t_tmap, a.t_tin, a.n_tiles, (int)CHAIN_MAPS,
-					   (uint32_t)T_G0, (uint64_t*)nullptr);
+	const ChainJob none {};
+	hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s,
+					   ChainJob {a.t_tmap, a.t_tin, a.n_tiles, (int)CHAIN_MAPS, (uint32_t)T_G0, nullptr}, none);
 	if (a.toml)
 		hipLaunchKernelGGL(k_kv_walk<true>, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);
 	else
 		hipLaunchKernelGGL(k_kv_walk<false>, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);
-	hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s, a.t_hmap, a.t_hin, a.n_tiles, (int)CHAIN_MAPS,
-					   0u, (uint64_t*)nullptr);
-	hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s, a.t_cnt, a.t_off, a.n_tiles, (int)CHAIN_SUMS,
-					   0u, &a.sum->n_tokens);
+	// the tiles' hook states and token offsets: two blocks of one launch
+	hipLaunchKernelGGL(k_kv_chain, dim3(2), dim3(kChainThreads), 0, s,
+					   ChainJob {a.t_hmap, a.t_hin, a.n_tiles, (int)CHAIN_MAPS, 0u, nullptr},
+					   ChainJob {a.t_cnt, a.t_off, a.n_tiles, (int)CHAIN_SUMS, 0u, &a.sum->n_tokens});
 	return hipGetLastError();
 }
 
@@ -638,8 +656,9 @@ hipError_t launch_kv_emit(const KvArgs& a, hipStream_t s) {
 		return hipSuccess;
 	if (a.toml) {
 		hipLaunchKernelGGL(k_kv_emit<true>, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);
-		hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s, (const uint32_t*)a.t_key,
-						   (uint32_t*)a.t_kin, a.n_tiles, (int)CHAIN_MAX, 0u, (uint64_t*)nullptr);
+		hipLaunchKernelGGL(k_kv_chain, dim3(1), dim3(kChainThreads), 0, s,
+						   ChainJob {(const uint32_t*)a.t_key, (uint32_t*)a.t_kin, a.n_tiles, (int)CHAIN_MAX, 0u, nullptr},
+						   ChainJob {});
 		hipLaunchKernelGGL(k_kv_keys, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);
 	} else {
 		hipLaunchKernelGGL(k_kv_emit<false>, dim3(a.n_tiles), dim3(kKvLanes), 0, s, a);

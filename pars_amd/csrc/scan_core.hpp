@@ -11,9 +11,9 @@
 //   3. masks -> unmasked structural / bracket bits; counts and the +-1 depth
 //      delta; block scans give each lane's output offsets inside the tile;
 //   4. row-local positions and bracket records staged in LDS per row and
-//      written with coalesced stores (four consecutive records per thread,
-//      16-byte stores on the aligned part of each run); the bitmap word per
-//      lane.
+//      written with coalesced stores (four consecutive records per thread:
+//      positions as 16-byte stores, bracket records on the 16-byte-aligned
+//      part of each run); the bitmap word per lane.
 //
 // Chains.  A plan with one descriptor (JSON, CSV, every single-automaton
 // grammar) runs ONE chain: a tile publishes its state map together with its
@@ -571,7 +571,7 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 	__shared__ int32_t		   s_vpost[NWV];		   // one-chain batch: per-wave depth after the last head
 	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
 	__shared__ int32_t		   s_mm[NWV][2];
-	__shared__ uint16_t		   s_pos[CAPP];	 // row-local byte offset
+	__shared__ __align__(16) uint16_t s_pos[CAPP];	 // row-local byte offset
 	__shared__ uint16_t		   s_bpos[CAPB];	 // row-local byte offset
 	__shared__ uint16_t		   s_bsidx[CAPB];	 // row-local structural index
 	__shared__ KeyT			   s_bkey[CAPB];	 // key: relative to the row's entry depth / document-relative
@@ -1117,16 +1117,17 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 						 return (uint32_t)(pbase + off);
 				};
 				if (!a.chars) {
-					// four consecutive positions per thread, 16-byte stores where the
-					// output is 16-byte aligned; a head and a tail one by one
+					// four consecutive positions per thread: one 8-byte LDS read, one
+					// 16-byte store (dword-aligned: the output need not be 16-byte
+					// aligned); a tail one by one
 					const uint64_t g0 = rb[0];
-					const uint32_t hn = min((uint32_t)((4u - (uint32_t)(g0 & 3u)) & 3u), mp);
-					if ((uint32_t)tid < hn)
-						a.positions[g0 + tid] = out_pos(s_pos[tid]);
-					for (uint32_t i = hn + 4u * (uint32_t)tid; i + 4u <= mp; i += 4u * NT)
-						*reinterpret_cast<uint4*>(a.positions + g0 + i) =
-							make_uint4(out_pos(s_pos[i]), out_pos(s_pos[i + 1]), out_pos(s_pos[i + 2]), out_pos(s_pos[i + 3]));
-					const uint32_t body = hn + ((mp - hn) & ~3u);
+					for (uint32_t i = 4u * (uint32_t)tid; i + 4u <= mp; i += 4u * NT) {
+						const uint2	   p2 = *reinterpret_cast<const uint2*>(s_pos + i);
+						const uint32_t v[4] = {out_pos(p2.x & 0xFFFFu), out_pos(p2.x >> 16), out_pos(p2.y & 0xFFFFu),
+											   out_pos(p2.y >> 16)};
+						__builtin_memcpy(a.positions + g0 + i, v, 16);
+					}
+					const uint32_t body = mp & ~3u;
 					if ((uint32_t)tid < mp - body)
 						a.positions[g0 + body + tid] = out_pos(s_pos[body + tid]);
 				} else

@@ -12,8 +12,8 @@
 //      delta; block scans give each lane's output offsets inside the tile;
 //   4. row-local positions and bracket records staged in LDS per row and
 //      written with coalesced stores (four consecutive records per thread:
-//      positions as 16-byte stores, bracket records on the 16-byte-aligned
-//      part of each run); the bitmap word per lane.
+//      positions and bracket positions as 16-byte stores, bracket bytes as
+//      4-byte stores, each from one LDS read); the bitmap word per lane.
 //
 // Chains.  A plan with one descriptor (JSON, CSV, every single-automaton
 // grammar) runs ONE chain: a tile publishes its state map together with its
@@ -572,13 +572,13 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 	__shared__ uint32_t		   s_v[3][NS];			   // one-chain: tile counts per entry state
 	__shared__ int32_t		   s_mm[NWV][2];
 	__shared__ __align__(16) uint16_t s_pos[CAPP];	 // row-local byte offset
-	__shared__ uint16_t		   s_bpos[CAPB];	 // row-local byte offset
+	__shared__ __align__(16) uint16_t s_bpos[CAPB];	 // row-local byte offset
 	__shared__ uint16_t		   s_bsidx[CAPB];	 // row-local structural index
 	__shared__ KeyT			   s_bkey[CAPB];	 // key: relative to the row's entry depth / document-relative
 	// bit-parallel evaluators: each lane's bracket ordinal bits per row (from the
 	// walk) and the staged bracket bytes rebuilt from them
 	__shared__ uint64_t		   s_bo[E::kBitParallel ? W : 1][E::kBitParallel ? NT : 1];
-	__shared__ uint8_t		   s_bch[E::kBitParallel ? CAPB : 1];
+	__shared__ __align__(16) uint8_t s_bch[E::kBitParallel ? CAPB : 1];
 	__shared__ uint64_t		   s_hw[BATCH ? NHW : 1];		// the tile's head bits
 	__shared__ uint32_t		   s_hpre[BATCH ? NHW : 1];		// heads in the words before
 	__shared__ int32_t		   s_dhead[BATCH ? NL : 1];		// tile-relative depth at each head, by slot
@@ -1153,12 +1153,12 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 				bool		  bdone = false;
 				if constexpr (E::kBitParallel) {
 					if (bit_chr && !want_sidx) {
-						// four consecutive bracket records per thread: positions in one
-						// 16-byte store, bytes in one 4-byte store (aligned part), keys
-						// only at the stack chunks' starts
-						const uint64_t g0	= rb[1];
-						const uint32_t hn	= min((uint32_t)((4u - (uint32_t)(g0 & 3u)) & 3u), mb);
-						auto		   one	= [&](uint32_t i) {
+						// four consecutive bracket records per thread: positions from one
+						// 8-byte LDS read in one 16-byte store, bytes from one 4-byte LDS
+						// read in one 4-byte store (both only element-aligned in the
+						// output), keys only at the stack chunks' starts; a tail one by one
+						const uint64_t g0  = rb[1];
+						auto		   one = [&](uint32_t i) {
 							  const uint64_t g = g0 + i;
 							  a.bpos[g]		   = out_pos(s_bpos[i]);
 							  if (all_keys || (g & (kStkChunk - 1u)) == 0)
@@ -1167,30 +1167,32 @@ __global__ __launch_bounds__(NWV * kWave) __attribute__((amdgpu_waves_per_eu(kMi
 							  if constexpr (BATCH)
 								  a.b.bdoc[g] = (uint32_t)d0 + slot_of(r * NT + (int)(s_bpos[i] >> 6));
 						};
-						if ((uint32_t)tid < hn)
-							one((uint32_t)tid);
-						for (uint32_t i = hn + 4u * (uint32_t)tid; i + 4u <= mb; i += 4u * NT) {
-							const uint64_t g = g0 + i;
-							*reinterpret_cast<uint4*>(a.bpos + g) = make_uint4(
-								out_pos(s_bpos[i]), out_pos(s_bpos[i + 1]), out_pos(s_bpos[i + 2]), out_pos(s_bpos[i + 3]));
-							*reinterpret_cast<uint32_t*>(a.bchar + g) =
-								(uint32_t)s_bch[i] | (uint32_t)s_bch[i + 1] << 8 | (uint32_t)s_bch[i + 2] << 16 |
-								(uint32_t)s_bch[i + 3] << 24;
+						for (uint32_t i = 4u * (uint32_t)tid; i + 4u <= mb; i += 4u * NT) {
+							const uint64_t g	= g0 + i;
+							const uint2	   p2	= *reinterpret_cast<const uint2*>(s_bpos + i);
+							const uint32_t o[4] = {p2.x & 0xFFFFu, p2.x >> 16, p2.y & 0xFFFFu, p2.y >> 16};
+							const uint32_t v[4] = {out_pos(o[0]), out_pos(o[1]), out_pos(o[2]), out_pos(o[3])};
+							__builtin_memcpy(a.bpos + g, v, 16);
+							const uint32_t c4 = *reinterpret_cast<const uint32_t*>(s_bch + i);
+							__builtin_memcpy(a.bchar + g, &c4, 4);
 							if (all_keys) {
+								int32_t k[4];
 #pragma unroll
-								for (int u = 0; u < 4; ++u) a.bkey[g + u] = kb + (int32_t)s_bkey[i + u];
-							} else if ((g & (kStkChunk - 1u)) == 0) {  // a chunk start is a multiple of 4
-								a.bkey[g] = kb + (int32_t)s_bkey[i];
+								for (int u = 0; u < 4; ++u) k[u] = kb + (int32_t)s_bkey[i + u];
+								__builtin_memcpy(a.bkey + g, k, 16);
+							} else {  // the chunk start in [g, g + 4), if there is one
+								const uint64_t c = (g + 3u) & ~(uint64_t)(kStkChunk - 1u);
+								if (c >= g)
+									a.bkey[c] = kb + (int32_t)s_bkey[i + (uint32_t)(c - g)];
 							}
 							if constexpr (BATCH) {
 								uint32_t d[4];
 #pragma unroll
-								for (int u = 0; u < 4; ++u)
-									d[u] = (uint32_t)d0 + slot_of(r * NT + (int)(s_bpos[i + u] >> 6));
-								*reinterpret_cast<uint4*>(a.b.bdoc + g) = make_uint4(d[0], d[1], d[2], d[3]);
+								for (int u = 0; u < 4; ++u) d[u] = (uint32_t)d0 + slot_of(r * NT + (int)(o[u] >> 6));
+								__builtin_memcpy(a.b.bdoc + g, d, 16);
 							}
 						}
-						const uint32_t body = hn + ((mb - hn) & ~3u);
+						const uint32_t body = mb & ~3u;
 						if ((uint32_t)tid < mb - body)
 							one(body + (uint32_t)tid);
 						bdone = true;
